@@ -1,0 +1,203 @@
+"""Benchmark: copy attempts per second of the exact-replay engine on one
+MI355X, on the batched run_batch workload (4096 theta-replicas, 2D periodic
+lattices with side drawn from {48, 64, 96, 128}, 2 cell types, phi = 0.7,
+summaries every 500 MCS).
+
+A step is one sampling interval (500 MCS) of every replica in the batch,
+advanced on the device by the batched exact-replay kernel with all state
+resident in HBM.  Beside it the script measures the plain
+one-replica-per-launch kernel on a 64x64 replica, the dominant kernel's
+HBM roofline fraction from its algorithmic bytes, and the CPU oracle on the
+host cores (a bounded sample of the same workload).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--replicas N]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cpm-rust_amd"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+SIDES = (48, 64, 96, 128)
+INTERVAL = 500
+MASTER_SEED = 42
+
+
+def draw_batch(n: int, seed: int = 2026):
+    """The seeded prior box of the batched workload: J entries U[1,20]
+    (J[medium][medium] = 0), lambda_V U[1,20], lambda_I U[0.5,5],
+    V* U[30,80], I* U[20,60] (rounded), side uniform over SIDES."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        side = int(rng.choice(SIDES))
+        j = rng.uniform(1, 20, size=6)
+        J = [[0.0, j[0], j[1]], [j[0], j[2], j[3]], [j[1], j[3], j[4]]]
+        lv, li = rng.uniform(1, 20, 2), rng.uniform(0.5, 5, 2)
+        tv = int(round(rng.uniform(30, 80)))
+        ti = int(round(rng.uniform(20, 60)))
+        ncell = max(2, int(round(0.7 * side * side / tv)))
+        out.append(dict(side=side, J=J, lv=lv.tolist(), li=li.tolist(), tv=tv, ti=ti,
+                        counts=[ncell - ncell // 2, ncell // 2], seed=None, index=i))
+    return out
+
+
+def make_sim(cpm, p, seed):
+    sim = cpm.CPM(grid=(p["side"], p["side"]), boundary="periodic", seed=seed)
+    for t, name in enumerate(("a", "b")):
+        sim.register_cell_type(name, p["tv"], p["ti"], p["lv"][t], p["li"][t])
+        sim.add_cells(name, p["counts"][t])
+    sim.set_adhesion(p["J"])
+    sim.initialize(warn_on_default_init=False)
+    return sim
+
+
+def algorithmic_bytes(c: dict) -> float:
+    """2D bytes per launch: 8 per attempt (sigma_t, sigma_s), 28 per
+    topology read (the other 7 neighbours), 56 per energy evaluation
+    (types of 8 neighbours, V and I of both cells, types of L and G) and
+    356 per accepted copy (sigma write, V/I, position sums, 16 contact
+    read-modify-writes, energy)."""
+    return 8.0 * c["attempts"] + 28.0 * c["topology"] + 56.0 * c["energy"] + 356.0 * c["accepted"]
+
+
+def cpu_baseline(params, seconds: float = 12.0):
+    """The oracle (C restatement of the reference loop) on the host cores:
+    replicas of the same workload advanced by one sampling interval each,
+    one replica per thread (ctypes releases the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import oracle as O
+
+    cores = min(os.cpu_count() or 1, 16)
+    models = []
+    for p in params[: cores * 2]:
+        cfg = O.Config(grid=(p["side"], p["side"]), boundary="periodic", seed=p["seed"],
+                       cell_types=[O.CellTypeSpec("a", p["tv"], p["ti"], p["lv"][0], p["li"][0]),
+                                   O.CellTypeSpec("b", p["tv"], p["ti"], p["lv"][1], p["li"][1])],
+                       cell_counts=p["counts"], adhesion=p["J"], warn_on_default_init=False)
+        m = O.Model(cfg)
+        m.cfg_side = p["side"]
+        models.append(m)
+    # calibrate MCS per replica so the sample takes about `seconds`
+    probe = models[0]
+    t0 = time.perf_counter()
+    probe.run_mcs(2)
+    per_site = (time.perf_counter() - t0) / (2 * probe.n_sites)
+    total_sites = sum(m.n_sites for m in models)
+    mcs = max(1, int(seconds * cores / (per_site * total_sites)))
+    before = sum(int(m.counters()[0]) for m in models)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(cores) as ex:
+        list(ex.map(lambda m: m.run_mcs(mcs), models))
+    dt = time.perf_counter() - t0
+    attempts = sum(int(m.counters()[0]) for m in models) - before
+    return {"value": attempts / dt, "unit": "copy attempts/s", "cores": cores, "kind": "port",
+            "sample": f"{len(models)} replicas of the batched workload x {mcs} MCS "
+                      f"({attempts} attempts in {dt:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--replicas", type=int, default=4096)
+    ap.add_argument("--interval", type=int, default=INTERVAL)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain-mcs", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the engine shards replicas across GPUs only by "
+              "index range with no exchange, which this build does not run; use --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import cpm
+
+    t_init0 = time.perf_counter()
+    params = draw_batch(args.replicas)
+    for p in params:
+        p["seed"] = cpm.derive_seed_u64(MASTER_SEED, p["index"])
+    from concurrent.futures import ThreadPoolExecutor
+
+    sims = [make_sim(cpm, p, p["seed"]) for p in params]
+    with ThreadPoolExecutor(min(os.cpu_count() or 1, 16)) as ex:  # init is host C++ code
+        reps = list(ex.map(lambda s: cpm.Replica(s, 0, 10000, INTERVAL), sims))
+    t_init = time.perf_counter() - t_init0
+    print(f"[bench] {len(reps)} replicas initialised in {t_init:.1f} s", flush=True)
+
+    for w in range(args.warmup):
+        c = cpm.Replica.run_mcs_many(reps, args.interval)
+        print(f"[bench] warmup {w}: {c['kernel_ms']:.0f} ms", flush=True)
+
+    totals = {k: 0.0 for k in ("attempts", "non_null", "topology", "energy", "accepted",
+                               "exp_near_ties", "kernel_ms")}
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        c = cpm.Replica.run_mcs_many(reps, args.interval)
+        for k in totals:
+            totals[k] += c[k]
+        print(f"[bench] step {s}: {c['kernel_ms']:.0f} ms, {c['attempts'] / c['kernel_ms'] / 1e6:.2f}"
+              f" G attempts/s", flush=True)
+    wall = time.perf_counter() - t0
+    value = totals["attempts"] / wall
+    avg_launch_s = totals["kernel_ms"] / 1e3 / args.steps
+    bytes_per_launch = algorithmic_bytes(totals) / args.steps
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+
+    # plain one-replica-per-launch kernel on a 64x64 replica
+    p64 = next(p for p in params if p["side"] == 64)
+    plain = cpm.Replica(make_sim(cpm, p64, p64["seed"]), 0, 10000, INTERVAL)
+    plain.run_mcs_plain(1)
+    t1 = time.perf_counter()
+    pc = plain.run_mcs_plain(args.plain_mcs)
+    plain_rate = pc["attempts"] / (time.perf_counter() - t1)
+    print(f"[bench] plain: {plain_rate / 1e6:.2f} M attempts/s", flush=True)
+
+    # run_batch sims/hour projected from the measured step (20 intervals per
+    # simulation) plus the measured host initialisation
+    step_s = wall / args.steps
+    sims_per_hour = len(reps) * 3600.0 / (t_init + 20 * step_s)
+
+    cpu = None if args.no_cpu else cpu_baseline(params)
+    line = {
+        "metric": "copy attempts/s on 1 GPU (batched exact-replay run_batch)",
+        "value": value,
+        "unit": "copy attempts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32+f64",
+        "data": "synthetic",
+        "config": {"workload": "run_batch: 4096 theta-replicas, 2D periodic sides "
+                               "{48,64,96,128}, 2 types, phi=0.7, 500 MCS per step",
+                   "replicas": len(reps), "interval_mcs": args.interval,
+                   "parallelism": "1 GPU, replicas batched in one launch"},
+        "plain_attempts_per_s": plain_rate,
+        "batched_vs_plain": value / plain_rate,
+        "sims_per_hour_projected": sims_per_hour,
+        "counters": {k: totals[k] for k in totals},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

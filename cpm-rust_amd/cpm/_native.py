@@ -1,0 +1,334 @@
+"""ctypes binding of the engine's C ABI (include/cpm_engine.h).
+
+This module plays the role of the reference's PyO3 extension `cpm._cpm`
+(crates/cpm-py/src/): it marshals builder state into `cpm_config`, maps
+every CONFIG/INIT failure to `ValueError` with the engine's message
+(crates/cpm-py/src/errors.rs:16-33) and converts results into NumPy-backed
+objects (crates/cpm-py/src/convert.rs).  The library is loaded from this
+package directory; there is no fallback implementation — importing the
+package without the built library raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcpm_engine.so")
+
+CPM_OK, CPM_ERR_CONFIG, CPM_ERR_INIT, CPM_ERR_DEVICE, CPM_ERR_OOM, CPM_ERR_ARG = range(6)
+BOUNDARY = {"fixed": 0, "periodic": 1}
+NB = {"von_neumann": 0, "moore": 1, "eighteen": 2, "twenty_six": 3, "twenty_six_weighted": 4}
+NB_DEFAULT = -1
+ACCEPT = {"metropolis": 0, "metropolis_hastings": 1}
+PROPOSAL = {"uniform": 0, "edge_list": 1}
+MODE = {"exact_replay": 0, "sublattice": 1}
+
+u32p = C.POINTER(C.c_uint32)
+u64p = C.POINTER(C.c_uint64)
+i64p = C.POINTER(C.c_int64)
+f64p = C.POINTER(C.c_double)
+u8p = C.POINTER(C.c_uint8)
+
+
+class CpmConfig(C.Structure):
+    _fields_ = [
+        ("dim", C.c_uint32), ("grid", C.c_uint64 * 3), ("boundary", C.c_int32),
+        ("copy_neighborhood", C.c_int32), ("energy_neighborhood", C.c_int32),
+        ("connectivity_neighborhood", C.c_int32), ("acceptance", C.c_int32),
+        ("proposal", C.c_int32), ("act_active", C.c_int32), ("n_types", C.c_uint32),
+        ("type_names", C.POINTER(C.c_char_p)), ("target_volume", u32p),
+        ("target_interface", u32p), ("lambda_volume", f64p), ("lambda_interface", f64p),
+        ("lambda_act", f64p), ("max_act", u32p), ("cell_counts", u64p),
+        ("adhesion_rows", C.c_uint32), ("adhesion_row_len", u32p), ("adhesion", f64p),
+        ("init_explicit", C.c_int32), ("init_lattice", u32p), ("init_lattice_len", C.c_uint64),
+        ("init_cell_type_of", u64p), ("init_n_cells", C.c_uint64),
+        ("warn_on_default_init", C.c_int32), ("burn_in_mcs", C.c_uint64),
+        ("readout_mcs", C.c_uint64), ("sampling_interval_mcs", C.c_uint64), ("seed", C.c_uint64),
+    ]
+
+
+class CpmCounters(C.Structure):
+    _fields_ = [("attempts", C.c_uint64), ("non_null", C.c_uint64), ("topology", C.c_uint64),
+                ("energy", C.c_uint64), ("accepted", C.c_uint64),
+                ("exp_near_ties", C.c_uint64), ("kernel_ms", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class CpmModelInfo(C.Structure):
+    _fields_ = [("n_sites", C.c_uint64), ("n_cells", C.c_uint64), ("mcs", C.c_uint64),
+                ("conservative_energy", C.c_double), ("rng_state", C.c_uint64 * 4)]
+
+
+class CpmResultInfo(C.Structure):
+    _fields_ = [
+        ("cpm_version", C.c_char_p), ("convention_version", C.c_uint32),
+        ("convention_hash", C.c_char_p), ("model_hash", C.c_char_p), ("seed", C.c_uint64),
+        ("seed_policy", C.c_char_p), ("derived_phi", C.c_double),
+        ("theta_layout_version", C.c_uint32), ("status_kind", C.c_char_p),
+        ("status_has_cell", C.c_int32), ("status_cell_id", C.c_uint32),
+        ("status_at_mcs", C.c_uint64), ("n_samples", C.c_uint64), ("n_cells", C.c_uint64),
+        ("dim", C.c_uint32), ("cell_type_index", u32p), ("used_default", C.c_int32),
+    ]
+
+
+class CpmSampleView(C.Structure):
+    _fields_ = [
+        ("mcs", C.c_uint64), ("conservative_energy", C.c_double), ("n_cells", C.c_uint64),
+        ("ids", u32p), ("volume", u32p), ("interface", u32p), ("centroid", f64p),
+        ("eigenvalues", f64p), ("radius_of_gyration_sq", f64p), ("kappa_sq", f64p),
+        ("contact", u32p), ("n_fragmented", C.c_uint64), ("fragmented_cells", u32p),
+        ("n_medium_components", C.c_uint64), ("medium_component_volumes", u32p),
+        ("medium_component_touches_boundary", u8p), ("lattice", u32p),
+    ]
+
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+def lib():
+    """The loaded engine library.  Raises if it has not been built."""
+    global _lib
+    with _lib_lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"cpm engine library not built: {LIB_PATH} is missing "
+                "(run `python cpm-rust_amd/build.py`)")
+        L = C.CDLL(LIB_PATH)
+        st = C.c_int
+        vp = C.c_void_p
+        L.cpm_last_error.restype = C.c_char_p
+        L.cpm_version.restype = C.c_char_p
+        L.cpm_engine_create.argtypes = [C.c_int, C.POINTER(vp)]
+        L.cpm_engine_create.restype = st
+        L.cpm_engine_destroy.argtypes = [vp]
+        L.cpm_model_create.argtypes = [C.POINTER(CpmConfig), C.POINTER(vp)]
+        L.cpm_model_create.restype = st
+        L.cpm_model_destroy.argtypes = [vp]
+        L.cpm_model_initialize.argtypes = [vp, C.POINTER(C.c_int32)]
+        L.cpm_model_initialize.restype = st
+        L.cpm_run_mcs.argtypes = [vp, C.POINTER(vp), C.c_size_t, C.c_uint64, C.c_int32,
+                                  C.POINTER(CpmCounters)]
+        L.cpm_run_mcs.restype = st
+        L.cpm_run_mcs_plain.argtypes = [vp, vp, C.c_uint64, C.POINTER(CpmCounters)]
+        L.cpm_run_mcs_plain.restype = st
+        L.cpm_model_get_info.argtypes = [vp, C.POINTER(CpmModelInfo)]
+        L.cpm_model_get_info.restype = st
+        L.cpm_model_get_state.argtypes = [vp, u32p, u32p, u32p, u32p, i64p, i64p, u32p]
+        L.cpm_model_get_state.restype = st
+        L.cpm_model_hashes.argtypes = [vp, C.c_char_p, C.c_char_p]
+        L.cpm_model_hashes.restype = st
+        L.cpm_model_global_energy.argtypes = [vp, f64p]
+        L.cpm_model_global_energy.restype = st
+        L.cpm_run.argtypes = [vp, vp, C.c_int32, C.c_int32, C.POINTER(vp)]
+        L.cpm_run.restype = st
+        L.cpm_run_batch.argtypes = [vp, C.POINTER(CpmConfig), f64p, C.c_size_t, C.c_size_t,
+                                    C.c_uint64, C.c_int32, C.c_int32, C.POINTER(vp),
+                                    C.POINTER(C.c_int32)]
+        L.cpm_run_batch.restype = st
+        L.cpm_run_batch_configs.argtypes = [vp, C.POINTER(C.POINTER(CpmConfig)), C.c_size_t,
+                                            C.c_int32, C.c_int32, C.POINTER(vp),
+                                            C.POINTER(C.c_int32), C.POINTER(CpmCounters)]
+        L.cpm_run_batch_configs.restype = st
+        L.cpm_result_get_info.argtypes = [vp, C.POINTER(CpmResultInfo)]
+        L.cpm_result_get_info.restype = st
+        L.cpm_result_get_sample.argtypes = [vp, C.c_uint64, C.POINTER(CpmSampleView)]
+        L.cpm_result_get_sample.restype = st
+        L.cpm_result_destroy.argtypes = [vp]
+        L.cpm_theta_len.argtypes = [C.c_uint32]
+        L.cpm_theta_len.restype = C.c_size_t
+        L.cpm_theta_extract.argtypes = [C.POINTER(CpmConfig), f64p, C.c_size_t]
+        L.cpm_theta_extract.restype = st
+        L.cpm_theta_name.argtypes = [C.POINTER(CpmConfig), C.c_size_t, C.c_char_p, C.c_size_t]
+        L.cpm_theta_name.restype = st
+        L.cpm_derive_seed_u64.argtypes = [C.c_uint64, C.c_uint64]
+        L.cpm_derive_seed_u64.restype = C.c_uint64
+        L.cpm_engine_sp_tables.argtypes = [vp, u8p, u64p]
+        L.cpm_engine_sp_tables.restype = st
+        _lib = L
+        return L
+
+
+class EngineError(RuntimeError):
+    """A device-side failure (no GPU, HIP error, out of memory)."""
+
+
+def check(status: int) -> None:
+    if status == CPM_OK:
+        return
+    msg = lib().cpm_last_error().decode("utf-8", "replace")
+    if status in (CPM_ERR_CONFIG, CPM_ERR_INIT, CPM_ERR_ARG):
+        raise ValueError(msg)
+    raise EngineError(msg)
+
+
+_engine = None
+_engine_lock = threading.Lock()
+
+
+def engine() -> C.c_void_p:
+    """The process-wide engine on HIP device 0 (created on first use)."""
+    global _engine
+    with _engine_lock:
+        if _engine is None:
+            h = C.c_void_p()
+            check(lib().cpm_engine_create(0, C.byref(h)))
+            _engine = h
+        return _engine
+
+
+def _arr(values, dtype):
+    return np.ascontiguousarray(np.asarray(values, dtype=dtype))
+
+
+class ConfigHolder:
+    """A filled `cpm_config` plus the buffers it points into."""
+
+    def __init__(self, spec: dict):
+        self.keep = []
+        c = CpmConfig()
+        grid = list(spec["grid"])
+        c.dim = len(grid)
+        for i, g in enumerate(grid[:3]):
+            c.grid[i] = int(g)
+        c.boundary = spec["boundary"]
+        c.copy_neighborhood = spec["copy_neighborhood"]
+        c.energy_neighborhood = spec["energy_neighborhood"]
+        c.connectivity_neighborhood = spec["connectivity_neighborhood"]
+        c.acceptance = spec["acceptance"]
+        c.proposal = spec["proposal"]
+        c.act_active = 1 if spec["act"] else 0
+        types = spec["cell_types"]
+        c.n_types = len(types)
+        names = (C.c_char_p * max(len(types), 1))(*[t["name"].encode("utf-8") for t in types])
+        self.keep.append(names)
+        c.type_names = C.cast(names, C.POINTER(C.c_char_p))
+
+        def buf(values, dtype, ptype):
+            a = _arr(values if len(values) else [0], dtype)
+            self.keep.append(a)
+            return a.ctypes.data_as(ptype)
+
+        c.target_volume = buf([t["target_volume"] for t in types], np.uint32, u32p)
+        c.target_interface = buf([t["target_interface"] for t in types], np.uint32, u32p)
+        c.lambda_volume = buf([t["lambda_volume"] for t in types], np.float64, f64p)
+        c.lambda_interface = buf([t["lambda_interface"] for t in types], np.float64, f64p)
+        c.lambda_act = buf([t["lambda_act"] for t in types], np.float64, f64p)
+        c.max_act = buf([t["max_act"] for t in types], np.uint32, u32p)
+        c.cell_counts = buf(spec["cell_counts"], np.uint64, u64p)
+        adh = spec["adhesion"]
+        if adh is None:
+            c.adhesion_rows = 0
+        else:
+            c.adhesion_rows = len(adh)
+            c.adhesion_row_len = buf([len(r) for r in adh], np.uint32, u32p)
+            flat = [float(v) for r in adh for v in r]
+            c.adhesion = buf(flat, np.float64, f64p)
+        init = spec["init"]
+        if init is not None and init["lattice"] is not None:
+            c.init_explicit = 1
+            lat = init["lattice"]
+            c.init_lattice = buf(lat, np.uint32, u32p)
+            c.init_lattice_len = len(lat)
+            cto = init["cell_type_of"]
+            c.init_cell_type_of = buf(cto, np.uint64, u64p)
+            c.init_n_cells = len(cto)
+        c.warn_on_default_init = 1 if (init is None or init["warn"]) else 0
+        c.burn_in_mcs = spec.get("burn_in_mcs", 1)
+        c.readout_mcs = spec.get("readout_mcs", 1)
+        c.sampling_interval_mcs = spec.get("sampling_interval_mcs", 1)
+        c.seed = spec["seed"]
+        self.c = c
+
+    def ref(self):
+        return C.byref(self.c)
+
+
+# ------------------------------------------------------------------ results
+
+class RunMetadata:
+    """crates/cpm-py/src/convert.rs:27-38"""
+
+    def __init__(self, info: CpmResultInfo):
+        self.cpm_version = info.cpm_version.decode()
+        self.convention_version = int(info.convention_version)
+        self.convention_hash = info.convention_hash.decode()
+        self.model_hash = info.model_hash.decode()
+        self.seed = int(info.seed)
+        self.seed_policy = info.seed_policy.decode()
+        self.derived_phi = float(info.derived_phi)
+        self.theta_layout_version = int(info.theta_layout_version)
+
+
+class Status:
+    """crates/cpm-py/src/convert.rs:40-51"""
+
+    def __init__(self, info: CpmResultInfo):
+        self.kind = info.status_kind.decode()
+        self.cell_id = int(info.status_cell_id) if info.status_has_cell else None
+        self.at_mcs = int(info.status_at_mcs) if info.status_has_cell else None
+
+
+def _np(ptr, n, dtype, shape=None):
+    if n == 0:
+        out = np.zeros(0, dtype=dtype)
+    else:
+        out = np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+    return out.reshape(shape) if shape is not None else out
+
+
+class Sample:
+    """crates/cpm-py/src/convert.rs:53-93"""
+
+    def __init__(self, v: CpmSampleView, dim: int, grid):
+        n = int(v.n_cells)
+        self.mcs = int(v.mcs)
+        self.conservative_energy = float(v.conservative_energy)
+        self.ids = _np(v.ids, n, np.uint32)
+        self.volume = _np(v.volume, n, np.uint32)
+        self.interface = _np(v.interface, n, np.uint32)
+        self.centroid = _np(v.centroid, n * dim, np.float64, (n, dim))
+        self.eigenvalues = _np(v.eigenvalues, n * dim, np.float64, (n, dim))
+        self.radius_of_gyration_sq = _np(v.radius_of_gyration_sq, n, np.float64)
+        self.kappa_sq = _np(v.kappa_sq, n, np.float64)
+        self.contact = _np(v.contact, (n + 1) * (n + 1), np.uint32, (n + 1, n + 1))
+        self.fragmented_cells = _np(v.fragmented_cells, int(v.n_fragmented), np.uint32).tolist()
+        k = int(v.n_medium_components)
+        self.medium_component_volumes = _np(v.medium_component_volumes, k, np.uint32).tolist()
+        self.medium_component_touches_boundary = [
+            bool(x) for x in _np(v.medium_component_touches_boundary, k, np.uint8)]
+        if v.lattice:
+            self.lattice = _np(v.lattice, int(np.prod(grid)), np.uint32, tuple(grid))
+        else:
+            self.lattice = None
+
+
+class RunResult:
+    """crates/cpm-py/src/convert.rs:95-115"""
+
+    def __init__(self, handle: C.c_void_p, grid, cell_type_names):
+        L = lib()
+        info = CpmResultInfo()
+        check(L.cpm_result_get_info(handle, C.byref(info)))
+        self.metadata = RunMetadata(info)
+        self.status = Status(info)
+        dim = int(info.dim)
+        samples = []
+        for i in range(int(info.n_samples)):
+            v = CpmSampleView()
+            check(L.cpm_result_get_sample(handle, i, C.byref(v)))
+            samples.append(Sample(v, dim, grid))
+        self.samples = samples
+        self.cell_type_index = _np(info.cell_type_index, int(info.n_cells), np.uint32)
+        self.cell_type_names = list(cell_type_names)
+        self.used_default = bool(info.used_default)
+        L.cpm_result_destroy(handle)

@@ -1,0 +1,144 @@
+"""Engine access at the granularity of cpm-core's Rust API.
+
+`Replica` is the counterpart of `cpm_core::model::CPM<D>` driven by
+`initialization::initialize` and `monte_carlo::run_mcs`
+(crates/cpm-core/src/model.rs:44-76, initialization.rs:39-55,
+monte_carlo.rs:327-357): one model whose state lives on the GPU between
+calls.  `Replica.run_mcs_many` advances many replicas in one launch.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import warnings
+
+import numpy as np
+
+from . import _native as N
+from .warnings import CPMInitializationWarning
+
+
+def derive_seed_u64(master_seed: int, index: int) -> int:
+    """crates/cpm-core/src/rng.rs:37-40"""
+    return int(N.lib().cpm_derive_seed_u64(int(master_seed), int(index)))
+
+
+class Replica:
+    """A validated, initialised model (CPM::new + initialize)."""
+
+    def __init__(self, sim, burn_in_mcs: int = 0, readout_mcs: int = 1,
+                 sampling_interval_mcs: int = 1):
+        spec = sim._windowed(burn_in_mcs, readout_mcs, sampling_interval_mcs)
+        self.spec = spec
+        self.dim = len(spec["grid"])
+        L = N.lib()
+        self._holder = N.ConfigHolder(spec)
+        self.h = C.c_void_p()
+        N.check(L.cpm_model_create(self._holder.ref(), C.byref(self.h)))
+        used = C.c_int32(0)
+        try:
+            N.check(L.cpm_model_initialize(self.h, C.byref(used)))
+        except Exception:
+            L.cpm_model_destroy(self.h)
+            self.h = None
+            raise
+        self.used_default = bool(used.value)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            N.lib().cpm_model_destroy(self.h)
+            self.h = None
+
+    # ------------------------------------------------------------ dynamics
+    def run_mcs(self, n: int, mode: str = "exact_replay") -> dict:
+        return Replica.run_mcs_many([self], n, mode)
+
+    @staticmethod
+    def run_mcs_many(replicas, n: int, mode: str = "exact_replay") -> dict:
+        arr = (C.c_void_p * max(len(replicas), 1))(*[r.h for r in replicas])
+        cnt = N.CpmCounters()
+        N.check(N.lib().cpm_run_mcs(N.engine(), arr, len(replicas), int(n), N.MODE[mode],
+                                    C.byref(cnt)))
+        return cnt.as_dict()
+
+    def run_mcs_plain(self, n: int) -> dict:
+        cnt = N.CpmCounters()
+        N.check(N.lib().cpm_run_mcs_plain(N.engine(), self.h, int(n), C.byref(cnt)))
+        return cnt.as_dict()
+
+    # ------------------------------------------------------------ state
+    def info(self) -> N.CpmModelInfo:
+        out = N.CpmModelInfo()
+        N.check(N.lib().cpm_model_get_info(self.h, C.byref(out)))
+        return out
+
+    @property
+    def mcs(self) -> int:
+        return int(self.info().mcs)
+
+    @property
+    def energy(self) -> float:
+        return float(self.info().conservative_energy)
+
+    def rng_state(self) -> np.ndarray:
+        return np.array(list(self.info().rng_state), dtype=np.uint64)
+
+    def snapshot(self) -> dict:
+        info = self.info()
+        n, D = int(info.n_cells), self.dim
+        lat = np.zeros(int(info.n_sites), dtype=np.uint32)
+        vol = np.zeros(n, dtype=np.uint32)
+        itf = np.zeros(n, dtype=np.uint32)
+        typ = np.zeros(n, dtype=np.uint32)
+        sp = np.zeros(n * D, dtype=np.int64)
+        so = np.zeros(n * D * D, dtype=np.int64)
+        ct = np.zeros((n + 1) * (n + 1), dtype=np.uint32)
+        N.check(N.lib().cpm_model_get_state(
+            self.h, lat.ctypes.data_as(N.u32p), vol.ctypes.data_as(N.u32p),
+            itf.ctypes.data_as(N.u32p), typ.ctypes.data_as(N.u32p), sp.ctypes.data_as(N.i64p),
+            so.ctypes.data_as(N.i64p), ct.ctypes.data_as(N.u32p)))
+        return {
+            "lattice": lat.reshape(self.spec["grid"]), "volume": vol, "interface": itf,
+            "type_index": typ, "sum_position": sp.reshape(n, D),
+            "sum_outer": so.reshape(n, D, D), "contact": ct.reshape(n + 1, n + 1),
+            "energy": float(info.conservative_energy),
+            "rng": np.array(list(info.rng_state), dtype=np.uint64), "mcs": int(info.mcs),
+        }
+
+    def hashes(self) -> tuple[str, str]:
+        """(convention_hash, model_hash) of the current state."""
+        a, b = C.create_string_buffer(65), C.create_string_buffer(65)
+        N.check(N.lib().cpm_model_hashes(self.h, a, b))
+        return a.value.decode(), b.value.decode()
+
+    def global_energy(self) -> float:
+        out = C.c_double()
+        N.check(N.lib().cpm_model_global_energy(self.h, C.byref(out)))
+        return float(out.value)
+
+
+def run_batch_configs(sims, burn_in_mcs: int, readout_mcs: int, sampling_interval_mcs: int,
+                      include_lattice: bool = False, mode: str = "exact_replay",
+                      counters: dict | None = None) -> list:
+    """Runs every configured `CPM` builder in `sims` to completion (its own
+    grid, cell types, theta and seed), all resident on the GPU together: the
+    per-replica `CPM::new` + `initialize` + `run_with` of
+    crates/cpm-core/src/batch.rs:130-145 without a shared grid."""
+    specs = [s._windowed(burn_in_mcs, readout_mcs, sampling_interval_mcs) for s in sims]
+    holders = [N.ConfigHolder(sp) for sp in specs]
+    n = len(holders)
+    ptrs = (C.POINTER(N.CpmConfig) * max(n, 1))(*[C.pointer(h.c) for h in holders])
+    handles = (C.c_void_p * max(n, 1))()
+    any_used = C.c_int32(0)
+    cnt = N.CpmCounters()
+    N.check(N.lib().cpm_run_batch_configs(N.engine(), ptrs, n, 1 if include_lattice else 0,
+                                          N.MODE[mode], handles, C.byref(any_used),
+                                          C.byref(cnt)))
+    if counters is not None:
+        counters.update(cnt.as_dict())
+    out = [N.RunResult(C.c_void_p(handles[i]), specs[i]["grid"],
+                       [t["name"] for t in specs[i]["cell_types"]]) for i in range(n)]
+    if any_used.value and any(s._warn_on_default_init for s in sims):
+        warnings.warn("default (scatter-and-grow) initial placement used in at least one "
+                      "batch member", CPMInitializationWarning, stacklevel=2)
+    return out

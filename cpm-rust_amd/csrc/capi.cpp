@@ -1,0 +1,413 @@
+// extern "C" boundary (include/cpm_engine.h) and the run drivers that sit
+// on top of the device engine: output::run_with (crates/cpm-core/src/
+// output.rs:434-529) and batch::run_batch (crates/cpm-core/src/
+// batch.rs:110-153).
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/cpm_engine.h"
+#include "device.hpp"
+#include "host.hpp"
+
+using namespace cpm;
+
+struct cpm_engine {
+    Engine* e;
+};
+struct cpm_model {
+    Model* m;
+};
+struct cpm_result {
+    Result r;
+};
+
+namespace {
+thread_local std::string g_last_error;
+
+template <class F>
+cpm_status guarded(F&& f) {
+    try {
+        f();
+        return CPM_OK;
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return CPM_ERR_OOM;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return CPM_ERR_DEVICE;
+    }
+}
+
+void need(bool ok, const char* what) {
+    if (!ok) throw Error(CPM_ERR_ARG, std::string("invalid argument: ") + what);
+}
+
+// Runs f(i) for i in [0, n) over the host's cores.
+template <class F>
+void parallel_for(size_t n, F&& f) {
+    size_t nt = std::min<size_t>(std::max(1u, std::thread::hardware_concurrency()), 16);
+    nt = std::min(nt, n);
+    if (nt <= 1) {
+        for (size_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    std::vector<std::exception_ptr> errs(nt);
+    std::vector<std::thread> ts;
+    for (size_t t = 0; t < nt; ++t)
+        ts.emplace_back([&, t] {
+            try {
+                for (size_t i; (i = next.fetch_add(1)) < n;) f(i);
+            } catch (...) {
+                errs[t] = std::current_exception();
+            }
+        });
+    for (auto& t : ts) t.join();
+    for (auto& e : errs)
+        if (e) std::rethrow_exception(e);
+}
+
+// One step of a replica's driver schedule: advance `mcs`, then either push
+// the energy onto the burn-in trace or take a readout sample.
+struct Step {
+    uint64_t mcs;
+    bool trace, sample;
+};
+
+// output.rs:441-468
+std::vector<Step> schedule(const Config& c) {
+    std::vector<Step> s;
+    const uint64_t burn = c.burn_in, interval = c.interval;
+    const uint64_t window = (uint64_t)((double)burn * 0.25);
+    const uint64_t start = burn > window ? burn - window : 0;
+    uint64_t burned = 0;
+    while (burned < burn) {
+        uint64_t chunk = std::min(interval, burn - burned);
+        burned += chunk;
+        s.push_back({chunk, burned >= start, false});
+    }
+    const uint64_t n = std::max<uint64_t>(c.readout / std::max<uint64_t>(interval, 1), 1);
+    for (uint64_t i = 0; i < n; ++i) s.push_back({interval, false, true});
+    return s;
+}
+
+// Drives many initialised models through their run_with schedules in
+// lockstep: each launch advances every replica by its own next chunk.
+void run_lockstep(Engine* e, std::vector<Model*>& ms, bool include_lattice, int mode,
+                  std::vector<Result>& out, cpm_counters* counters) {
+    const size_t n = ms.size();
+    std::vector<std::vector<Step>> sch(n);
+    size_t steps = 0;
+    for (size_t i = 0; i < n; ++i) {
+        sch[i] = schedule(ms[i]->cfg);
+        steps = std::max(steps, sch[i].size());
+    }
+    std::vector<std::vector<double>> traces(n);
+    out.assign(n, Result());
+    std::vector<uint64_t> todo(n);
+    for (size_t k = 0; k < steps; ++k) {
+        for (size_t i = 0; i < n; ++i) todo[i] = k < sch[i].size() ? sch[i][k].mcs : 0;
+        run_mcs_each(e, ms.data(), n, todo.data(), mode, counters);
+        std::vector<size_t> sampled;
+        for (size_t i = 0; i < n; ++i) {
+            if (k >= sch[i].size()) continue;
+            if (sch[i][k].trace) traces[i].push_back(ms[i]->energy);
+            if (sch[i][k].sample) {
+                sync_to_host(ms[i]);
+                sampled.push_back(i);
+            }
+        }
+        parallel_for(sampled.size(), [&](size_t j) {
+            size_t i = sampled[j];
+            out[i].samples.push_back(take_sample(*ms[i], include_lattice));
+        });
+    }
+    parallel_for(n, [&](size_t i) {
+        sync_to_host(ms[i]);
+        finish_result(*ms[i], traces[i], out[i]);
+    });
+}
+}  // namespace
+
+extern "C" {
+
+const char* cpm_last_error(void) { return g_last_error.c_str(); }
+const char* cpm_version(void) { return "0.1.0"; }
+
+cpm_status cpm_engine_create(int device, cpm_engine** out) {
+    return guarded([&] {
+        need(out, "out");
+        *out = new cpm_engine{engine_create(device)};
+    });
+}
+
+void cpm_engine_destroy(cpm_engine* engine) {
+    if (!engine) return;
+    engine_destroy(engine->e);
+    delete engine;
+}
+
+cpm_status cpm_model_create(const cpm_config* config, cpm_model** out) {
+    return guarded([&] {
+        need(config && out, "config/out");
+        Config c = resolve(*config);
+        *out = new cpm_model{new Model(c)};
+    });
+}
+
+void cpm_model_destroy(cpm_model* model) {
+    if (!model) return;
+    delete model->m;
+    delete model;
+}
+
+cpm_status cpm_model_initialize(cpm_model* model, int32_t* used_default) {
+    return guarded([&] {
+        need(model, "model");
+        device_release(model->m);
+        model->m->initialize();
+        if (used_default) *used_default = model->m->used_default ? 1 : 0;
+    });
+}
+
+cpm_status cpm_run_mcs(cpm_engine* engine, cpm_model* const* models, size_t n, uint64_t n_mcs,
+                       int32_t mode, cpm_counters* counters) {
+    return guarded([&] {
+        need(engine && (models || n == 0), "engine/models");
+        std::vector<Model*> ms(n);
+        for (size_t i = 0; i < n; ++i) {
+            need(models[i], "models[i]");
+            ms[i] = models[i]->m;
+        }
+        if (counters) *counters = cpm_counters{};
+        if (n) run_mcs(engine->e, ms.data(), n, n_mcs, mode, counters);
+    });
+}
+
+cpm_status cpm_run_mcs_plain(cpm_engine* engine, cpm_model* model, uint64_t n_mcs,
+                             cpm_counters* counters) {
+    return guarded([&] {
+        need(engine && model, "engine/model");
+        if (counters) *counters = cpm_counters{};
+        run_mcs_plain(engine->e, model->m, n_mcs, counters);
+    });
+}
+
+cpm_status cpm_model_get_info(cpm_model* model, cpm_model_info* out) {
+    return guarded([&] {
+        need(model && out, "model/out");
+        Model* m = model->m;
+        out->n_sites = m->n_sites;
+        out->n_cells = m->n_cells;
+        out->mcs = m->mcs;
+        out->conservative_energy = m->energy;
+        for (int k = 0; k < 4; ++k) out->rng_state[k] = m->rng.s[k];
+    });
+}
+
+cpm_status cpm_model_get_state(cpm_model* model, uint32_t* lattice, uint32_t* volume,
+                               uint32_t* interface, uint32_t* type_index,
+                               int64_t* sum_position, int64_t* sum_outer, uint32_t* contact) {
+    return guarded([&] {
+        need(model, "model");
+        Model* m = model->m;
+        sync_to_host(m);
+        const size_t n = m->n_cells, D = m->D;
+        if (lattice) {
+            auto l = m->unpadded_lattice();
+            std::memcpy(lattice, l.data(), l.size() * 4);
+        }
+        if (volume) std::memcpy(volume, m->volume.data(), n * 4);
+        if (interface) std::memcpy(interface, m->interface.data(), n * 4);
+        if (type_index) std::memcpy(type_index, m->type_index.data(), n * 4);
+        if (sum_position) std::memcpy(sum_position, m->sum_pos.data(), n * D * 8);
+        if (sum_outer) std::memcpy(sum_outer, m->sum_outer.data(), n * D * D * 8);
+        if (contact) std::memcpy(contact, m->contact.data(), (n + 1) * (n + 1) * 4);
+    });
+}
+
+cpm_status cpm_model_hashes(cpm_model* model, char convention_hex[65], char model_hex[65]) {
+    return guarded([&] {
+        need(model && convention_hex && model_hex, "model/out");
+        sync_to_host(model->m);
+        std::string c = convention_hash(model->m->cfg), m = model_hash(*model->m);
+        std::memcpy(convention_hex, c.c_str(), 65);
+        std::memcpy(model_hex, m.c_str(), 65);
+    });
+}
+
+cpm_status cpm_model_global_energy(cpm_model* model, double* out) {
+    return guarded([&] {
+        need(model && out, "model/out");
+        sync_to_host(model->m);
+        *out = model->m->global_energy();
+    });
+}
+
+cpm_status cpm_run(cpm_engine* engine, cpm_model* model, int32_t include_lattice, int32_t mode,
+                   cpm_result** out) {
+    return guarded([&] {
+        need(engine && model && out, "engine/model/out");
+        need(model->m->initialized, "model must be initialized before cpm_run");
+        std::vector<Model*> ms{model->m};
+        std::vector<Result> rs;
+        run_lockstep(engine->e, ms, include_lattice != 0, mode, rs, nullptr);
+        *out = new cpm_result{std::move(rs[0])};
+    });
+}
+
+static void batch_models(cpm_engine* engine, std::vector<Config>& cfgs, int32_t include_lattice,
+                         int32_t mode, cpm_result** out, int32_t* any_used_default,
+                         cpm_counters* counters) {
+    const size_t n = cfgs.size();
+    std::vector<std::unique_ptr<Model>> owned(n);
+    // CPM::new + initialize per replica, over the host cores (batch.rs:126-145)
+    parallel_for(n, [&](size_t i) {
+        owned[i] = std::make_unique<Model>(cfgs[i]);
+        owned[i]->initialize();
+    });
+    std::vector<Model*> ms(n);
+    bool any = false;
+    for (size_t i = 0; i < n; ++i) {
+        ms[i] = owned[i].get();
+        any |= ms[i]->used_default;
+    }
+    if (counters) *counters = cpm_counters{};
+    std::vector<Result> rs;
+    run_lockstep(engine->e, ms, include_lattice != 0, mode, rs, counters);
+    for (size_t i = 0; i < n; ++i) out[i] = new cpm_result{std::move(rs[i])};
+    if (any_used_default) *any_used_default = any ? 1 : 0;
+}
+
+cpm_status cpm_run_batch(cpm_engine* engine, const cpm_config* base, const double* thetas,
+                         size_t n_thetas, size_t theta_len_, uint64_t master_seed,
+                         int32_t include_lattice, int32_t mode, cpm_result** out,
+                         int32_t* any_used_default) {
+    return guarded([&] {
+        need(engine && base && out && (thetas || n_thetas == 0), "engine/base/out/thetas");
+        Config c = resolve(*base);
+        const size_t expected = theta_len(c.types.size());
+        // batch.rs:115-124 — every theta length checked before any work
+        if (theta_len_ != expected)
+            throw Error(CPM_ERR_CONFIG, "theta at index 0 has length " + std::to_string(theta_len_) +
+                                            ", expected " + std::to_string(expected));
+        std::vector<Config> cfgs(n_thetas, c);
+        for (size_t i = 0; i < n_thetas; ++i) {
+            theta_inject(cfgs[i], thetas + i * theta_len_, theta_len_);
+            cfgs[i].seed = derive_seed_u64(master_seed, i);
+            validate(cfgs[i]);
+        }
+        batch_models(engine, cfgs, include_lattice, mode, out, any_used_default, nullptr);
+    });
+}
+
+cpm_status cpm_run_batch_configs(cpm_engine* engine, const cpm_config* const* configs, size_t n,
+                                 int32_t include_lattice, int32_t mode, cpm_result** out,
+                                 int32_t* any_used_default, cpm_counters* counters) {
+    return guarded([&] {
+        need(engine && out && (configs || n == 0), "engine/configs/out");
+        std::vector<Config> cfgs;
+        cfgs.reserve(n);
+        for (size_t i = 0; i < n; ++i) {
+            need(configs[i], "configs[i]");
+            cfgs.push_back(resolve(*configs[i]));
+            validate(cfgs.back());
+        }
+        batch_models(engine, cfgs, include_lattice, mode, out, any_used_default, counters);
+    });
+}
+
+cpm_status cpm_result_get_info(const cpm_result* result, cpm_result_info* o) {
+    return guarded([&] {
+        need(result && o, "result/out");
+        const Result& r = result->r;
+        o->cpm_version = r.cpm_version.c_str();
+        o->convention_version = 1;
+        o->convention_hash = r.convention_hash.c_str();
+        o->model_hash = r.model_hash.c_str();
+        o->seed = r.seed;
+        o->seed_policy = r.seed_policy.c_str();
+        o->derived_phi = r.derived_phi;
+        o->theta_layout_version = 1;
+        o->status_kind = r.status_kind.c_str();
+        o->status_has_cell = r.status_has_cell ? 1 : 0;
+        o->status_cell_id = r.status_cell;
+        o->status_at_mcs = r.status_mcs;
+        o->n_samples = r.samples.size();
+        o->n_cells = r.n_cells;
+        o->dim = r.dim;
+        o->cell_type_index = r.cell_type_index.data();
+        o->used_default = r.used_default ? 1 : 0;
+    });
+}
+
+cpm_status cpm_result_get_sample(const cpm_result* result, uint64_t index, cpm_sample_view* o) {
+    return guarded([&] {
+        need(result && o, "result/out");
+        need(index < result->r.samples.size(), "sample index");
+        const Sample& s = result->r.samples[index];
+        o->mcs = s.mcs;
+        o->conservative_energy = s.energy;
+        o->n_cells = s.ids.size();
+        o->ids = s.ids.data();
+        o->volume = s.volume.data();
+        o->interface = s.interface.data();
+        o->centroid = s.centroid.data();
+        o->eigenvalues = s.eigen.data();
+        o->radius_of_gyration_sq = s.rg2.data();
+        o->kappa_sq = s.kappa.data();
+        o->contact = s.contact.data();
+        o->n_fragmented = s.fragmented.size();
+        o->fragmented_cells = s.fragmented.data();
+        o->n_medium_components = s.medium_volumes.size();
+        o->medium_component_volumes = s.medium_volumes.data();
+        o->medium_component_touches_boundary = s.medium_touch.data();
+        o->lattice = s.has_lattice ? s.lattice.data() : nullptr;
+    });
+}
+
+void cpm_result_destroy(cpm_result* result) { delete result; }
+
+size_t cpm_theta_len(uint32_t n_types) { return theta_len(n_types); }
+
+cpm_status cpm_theta_extract(const cpm_config* config, double* out, size_t cap) {
+    return guarded([&] {
+        need(config && out, "config/out");
+        Config c = resolve(*config);
+        validate(c);
+        auto th = theta_extract(c);
+        need(cap >= th.size(), "capacity");
+        std::copy(th.begin(), th.end(), out);
+    });
+}
+
+cpm_status cpm_theta_name(const cpm_config* config, size_t index, char* buf, size_t cap) {
+    return guarded([&] {
+        need(config && buf && cap, "config/buf");
+        Config c = resolve(*config);
+        validate(c);
+        std::string s = theta_name(c, index);
+        need(s.size() < cap, "capacity");
+        std::memcpy(buf, s.c_str(), s.size() + 1);
+    });
+}
+
+uint64_t cpm_derive_seed_u64(uint64_t master_seed, uint64_t index) {
+    return derive_seed_u64(master_seed, index);
+}
+
+cpm_status cpm_engine_sp_tables(cpm_engine* engine, uint8_t* table2d, uint64_t* table3d_words) {
+    return guarded([&] {
+        need(engine, "engine");
+        engine_sp_tables(engine->e, table2d, table3d_words);
+    });
+}
+
+}  // extern "C"

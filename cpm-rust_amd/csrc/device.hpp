@@ -1,0 +1,35 @@
+// Interface between the host library and the device engine (engine.hip).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/cpm_engine.h"
+
+namespace cpm {
+
+struct Model;
+struct Engine;
+
+Engine* engine_create(int device);
+void engine_destroy(Engine* e);
+
+// Advance every model by n_mcs MCS on the device (uploads on first use).
+void run_mcs(Engine* e, Model* const* models, size_t n, uint64_t n_mcs, int mode,
+             cpm_counters* counters);
+// Advance models by per-model MCS counts in one launch (todo[i] may be 0).
+void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* todo, int mode,
+                  cpm_counters* counters);
+// One model per launch, one launch per MCS.
+void run_mcs_plain(Engine* e, Model* m, uint64_t n_mcs, cpm_counters* counters);
+
+// Bring the host copy of the state up to date (no-op when not stale).
+void sync_to_host(Model* m);
+// Scalars only: energy, rng, mcs.
+void sync_scalars(Model* m);
+// Drop the device copy of a model.
+void device_release(Model* m);
+
+void engine_sp_tables(Engine* e, uint8_t* t2, uint64_t* t3);
+
+}  // namespace cpm

@@ -489,12 +489,12 @@ static void refresh_halo_full(ocpm* m) {
    rescan when the halo was consistent before the write). */
 static void refresh_halo_mirrors(ocpm* m, const size_t* c) {
     int nopt[3];
-    size_t opt[3][2];
+    size_t opt[3][3];
     for (int d = 0; d < m->D; ++d) {
         nopt[d] = 1;
         opt[d][0] = c[d] + 1;
         if (c[d] == 0) opt[d][nopt[d]++] = m->dims[d] + 1;
-        else if (c[d] == m->dims[d] - 1) opt[d][nopt[d]++] = 0;
+        if (c[d] == m->dims[d] - 1) opt[d][nopt[d]++] = 0;
     }
     size_t src = flat_of(m, c);
     int total = 1;

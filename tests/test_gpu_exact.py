@@ -1,0 +1,196 @@
+"""Exact-replay parity on the GPU: the HIP engine against the CPU oracle and
+the reference's own frozen values.  Bar: bit-exact (integer lattice and
+bookkeeping, FP64 energy bits, RNG state)."""
+import hashlib
+import struct
+
+import numpy as np
+import pytest
+
+import cpm
+from oracle import oracle as O
+from tests.helpers import (PINS, frozen_oracle_config, frozen_sim, quickstart,
+                           quickstart_oracle, sorting_pair)
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ["lattice", "volume", "interface", "sum_position", "sum_outer", "contact", "rng", "mcs"]
+
+
+def assert_same_state(dev: dict, ref: dict, label: str = ""):
+    for k in KEYS:
+        a, b = np.asarray(dev[k]), np.asarray(ref[k])
+        assert a.shape == b.shape, f"{label} {k}: shape {a.shape} vs {b.shape}"
+        assert np.array_equal(a, b), f"{label} {k} differs"
+    assert struct.pack("<d", dev["energy"]) == struct.pack("<d", ref["energy"]), (
+        f"{label} energy {dev['energy']!r} vs {ref['energy']!r}")
+
+
+def test_frozen_trajectory_through_the_engine():
+    f = PINS["frozen_trajectory"]
+    rep = cpm.Replica(frozen_sim(cpm), *f["windows"])
+    rep.run_mcs(f["mcs"])
+    snap = rep.snapshot()
+    assert snap["mcs"] == 500
+    assert snap["volume"].tolist() == f["volume"]
+    assert snap["interface"].tolist() == f["interface"]
+    assert struct.unpack("<Q", struct.pack("<d", snap["energy"]))[0] == \
+        f["conservative_energy_bits"]
+    cfg = frozen_oracle_config(O)
+    assert O.model_hash(cfg, snap["lattice"].ravel(), snap["type_index"]) == f["model_hash"]
+
+
+def test_frozen_trajectory_through_python_run():
+    """tests/python/test_reproducibility.py:72-83"""
+    f = PINS["frozen_trajectory"]
+    result = frozen_sim(cpm).run(burn_in_mcs=0, readout_mcs=500, sampling_interval_mcs=500)
+    assert len(result.samples) == 1
+    s = result.samples[0]
+    assert s.mcs == 500
+    assert list(s.ids) == [1, 2]
+    assert list(s.volume) == f["volume"]
+    assert list(s.interface) == f["interface"]
+    assert struct.unpack("<Q", struct.pack("<d", s.conservative_energy))[0] == \
+        f["conservative_energy_bits"]
+
+
+def test_different_seed_diverges():
+    a = frozen_sim(cpm).run(burn_in_mcs=0, readout_mcs=500, sampling_interval_mcs=500)
+    b = frozen_sim(cpm, seed=1).run(burn_in_mcs=0, readout_mcs=500, sampling_interval_mcs=500)
+    assert a.samples[0].conservative_energy != b.samples[0].conservative_energy
+
+
+CASES = [
+    # (label, builder args, mcs)
+    ("quickstart30", dict(kind="quick", grid=(30, 30), seed=1), 200),
+    ("quickstart_rect", dict(kind="quick", grid=(40, 24), seed=5), 100),
+    ("sorting48_periodic", dict(kind="sort", grid=(48, 48), seed=11, n_types=3), 60),
+    ("sorting_fixed_rect", dict(kind="sort", grid=(36, 50), seed=12, n_types=2,
+                                boundary="fixed"), 60),
+    ("sorting64_4types", dict(kind="sort", grid=(64, 64), seed=13, n_types=4, tv=40, ti=30,
+                              phi=0.8), 30),
+    ("sorting3d_periodic", dict(kind="sort", grid=(16, 16, 16), seed=21, n_types=2, tv=100,
+                                ti=150, phi=0.25), 8),
+    ("sorting3d_fixed", dict(kind="sort", grid=(14, 18, 12), seed=22, n_types=3, tv=60,
+                             ti=100, phi=0.5, boundary="fixed"), 8),
+    ("sorting3d_26", dict(kind="sort", grid=(12, 12, 12), seed=23, n_types=2, tv=60, ti=120,
+                          phi=0.4, energy="twenty_six"), 8),
+]
+
+
+def build_case(args):
+    args = dict(args)
+    kind = args.pop("kind")
+    if kind == "quick":
+        sim = quickstart(cpm, seed=args["seed"], grid=args["grid"])
+        sim.initialize(warn_on_default_init=False)
+        cfg = quickstart_oracle(O, seed=args["seed"], grid=args["grid"])
+        return sim, cfg
+    seed = args.pop("seed")
+    return sorting_pair(cpm, O, seed, **args)
+
+
+@pytest.mark.parametrize("label,args,mcs", CASES, ids=[c[0] for c in CASES])
+def test_exact_replay_matches_oracle(label, args, mcs):
+    sim, cfg = build_case(args)
+    print(f"[case] {label} grid={args['grid']} mcs={mcs}", flush=True)
+    rep = cpm.Replica(sim)
+    ref = O.Model(cfg)
+    assert_same_state(rep.snapshot(), ref.snapshot(), label + "@init")
+    cnt = rep.run_mcs(mcs)
+    ref.run_mcs(mcs)
+    assert_same_state(rep.snapshot(), ref.snapshot(), label)
+    oc = ref.counters()
+    assert [cnt["attempts"], cnt["non_null"], cnt["topology"], cnt["energy"],
+            cnt["accepted"]] == [int(x) for x in oc]
+    assert cnt["accepted"] > 0
+
+
+def test_batch_composition_does_not_change_results():
+    """Replicas advanced together in one launch equal each replica run alone
+    (batch.rs:251-264 proves the same for thread counts)."""
+    pairs = [build_case(c[1]) for c in CASES[:5]]
+    reps = [cpm.Replica(s) for s, _ in pairs]
+    cpm.Replica.run_mcs_many(reps, 25)
+    for (sim, cfg), rep in zip(pairs, reps):
+        ref = O.Model(cfg)
+        ref.run_mcs(25)
+        assert_same_state(rep.snapshot(), ref.snapshot())
+
+
+def test_plain_launches_match_batched():
+    sim, _ = build_case(CASES[0][1])
+    a, b = cpm.Replica(sim), cpm.Replica(sim)
+    a.run_mcs_plain(20)
+    b.run_mcs(20)
+    assert_same_state(a.snapshot(), b.snapshot())
+
+
+def test_run_samples_match_oracle_run_with():
+    sim = quickstart(cpm, seed=7)
+    sim.initialize(warn_on_default_init=False)
+    res = sim.run(burn_in_mcs=200, readout_mcs=300, sampling_interval_mcs=100,
+                  include_lattice=True)
+    cfg = quickstart_oracle(O, seed=7, windows=(200, 300, 100))
+    cfg.warn_on_default_init = False
+    ref = O.run_with(O.Model(cfg), include_lattice=True)
+    assert len(res.samples) == len(ref.samples) == 3
+    for s, r in zip(res.samples, ref.samples):
+        assert s.mcs == r.mcs
+        assert s.conservative_energy == r.energy
+        assert np.array_equal(s.volume, r.volume)
+        assert np.array_equal(s.interface, r.interface)
+        assert np.array_equal(s.contact, r.contact)
+        assert np.array_equal(s.lattice, r.lattice)
+        assert np.array_equal(s.centroid, r.centroid)
+        assert np.allclose(s.eigenvalues, r.eigenvalues, rtol=0, atol=1e-9)
+        assert s.fragmented_cells == r.fragmented
+        assert s.medium_component_volumes == r.medium_volumes
+        assert s.medium_component_touches_boundary == r.medium_touch
+    assert res.status.kind == ref.status[0]
+    assert res.metadata.model_hash == ref.model_hash
+    assert res.metadata.convention_hash == ref.convention_hash
+    assert res.metadata.derived_phi == ref.derived_phi
+
+
+def test_run_batch_matches_oracle_per_replica():
+    """batch.rs:110-153: theta injected per replica, seed derive_seed_u64."""
+    sim = cpm.CPM(grid=(20, 20), boundary="periodic", seed=1)
+    sim.register_cell_type("a", 9, 20, 1.0, 0.2)
+    sim.register_cell_type("b", 9, 20, 1.5, 0.3)
+    sim.add_cells("a", 2)
+    sim.add_cells("b", 2)
+    sim.set_adhesion([[0.0, 3.0, 3.0], [3.0, 1.0, 5.0], [3.0, 5.0, 1.0]])
+    sim.initialize(warn_on_default_init=False)
+    base = sim.extract_theta()
+    thetas = [base, [v * 1.1 for v in base], [v * 0.9 for v in base]]
+    results = sim.run_batch(thetas, burn_in_mcs=20, readout_mcs=40, sampling_interval_mcs=20,
+                            master_seed=42)
+    cfg0 = O.Config(grid=(20, 20), boundary="periodic", seed=0,
+                    cell_types=[O.CellTypeSpec("a", 9, 20, 1.0, 0.2),
+                                O.CellTypeSpec("b", 9, 20, 1.5, 0.3)],
+                    cell_counts=[2, 2], adhesion=[[0.0, 3.0, 3.0], [3.0, 1.0, 5.0],
+                                                  [3.0, 5.0, 1.0]],
+                    warn_on_default_init=False, burn_in_mcs=20, readout_mcs=40,
+                    sampling_interval_mcs=20)
+    for i, (th, res) in enumerate(zip(thetas, results)):
+        cfg = O.theta_inject(cfg0, th)
+        cfg.seed = O.derive_seed_u64(42, i)
+        ref = O.run_with(O.Model(cfg))
+        assert res.metadata.seed == cfg.seed == PINS["derived_seeds"]["values"][i]
+        for s, r in zip(res.samples, ref.samples):
+            assert s.conservative_energy == r.energy
+            assert np.array_equal(s.volume, r.volume)
+        assert res.metadata.model_hash == ref.model_hash
+
+
+def test_device_simple_point_tables_match_the_definition():
+    import ctypes as C
+    from cpm import _native as N
+    t2 = np.zeros(256, dtype=np.uint8)
+    t3 = np.zeros(1 << 20, dtype=np.uint64)
+    N.check(N.lib().cpm_engine_sp_tables(N.engine(), t2.ctypes.data_as(N.u8p),
+                                         t3.ctypes.data_as(N.u64p)))
+    assert "".join(str(int(b)) for b in t2) == PINS["sp_table_2d_bits"]
+    assert hashlib.sha256(t3.astype("<u8").tobytes()).hexdigest() == PINS["sp_table_3d_sha256"]
+    del C
