@@ -27,8 +27,8 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math"
              "-pthread"]
 
 HIP_SOURCES = ["engine.hip"]
-CXX_SOURCES = ["host.cpp", "capi.cpp"]
-HEADERS = ["core.hpp", "host.hpp", "device.hpp"]
+CXX_SOURCES = ["host.cpp", "capi.cpp", "rng_jump.cpp"]
+HEADERS = ["core.hpp", "host.hpp", "device.hpp", "rng_jump.hpp"]
 
 
 def _run(cmd):
@@ -62,7 +62,8 @@ def build(verbose: bool = False) -> str:
     if failed:
         raise RuntimeError("engine compilation failed")
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "--offload-arch=gfx950", "-fPIC", "-pthread", *objs, "-o", tmp])
+    _run([HIPCC, "-shared", "--offload-arch=gfx950", "-fPIC", "-pthread", "-Wl,--no-undefined",
+          *objs, "-o", tmp])
     os.replace(tmp, LIB)
     return LIB
 

@@ -1,24 +1,37 @@
 // Device engine: HIP kernels for gfx950 and their host-side launch logic.
 //
-// Exact replay (cpm_run_mcs, CPM_MODE_EXACT_REPLAY) keeps every replica's
-// state resident in HBM and advances one replica per lane: the lane runs
-// the reference's serial attempt loop (crates/cpm-core/src/
-// monte_carlo.rs:37-145, 327-357) against its own haloed lattice, per-cell
-// bookkeeping and Xoshiro256++ stream, so its results are bit-identical to
-// the reference's serial loop.  All FP64 energy arithmetic is compiled
-// with -ffp-contract=off and follows the reference's operation order
-// (crates/cpm-core/src/energy.rs:92-103, 167-202, 242-264, 521-523).
+// Exact replay reproduces the reference's serial attempt loop
+// (crates/cpm-core/src/monte_carlo.rs:37-145, 327-357) bit for bit.  Two
+// execution paths share the same arithmetic:
+//
+//  * LDS path (lattices whose haloed u16 copy fits in LDS): one wave per
+//    replica.  The replica's Xoshiro256++ stream is produced ahead of time
+//    by k_rng_fill — each replica's stream split into 16 segments started
+//    with GF(2) jump polynomials, so the whole chip generates it — into a
+//    2 MiB HBM buffer.  k_lds_engine keeps sigma (haloed, u16), cell types,
+//    V and I resident in LDS, holds a 128-draw window of the stream in
+//    registers, parses attempts with ballots + find-first-set on the scalar
+//    unit, and evaluates the 8/26-site neighbourhood lane-parallel.
+//  * Global path (everything else, e.g. 64^3 lattices): one lane per
+//    replica running the serial loop against HBM-resident state.
+//
+// FP64 energy arithmetic is compiled with -ffp-contract=off and keeps the
+// reference's operation order (crates/cpm-core/src/energy.rs:92-103,
+// 167-202, 242-264, 521-523); sums of +-1.0 are formed from popcounts,
+// which is exact.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <vector>
 
 #include "core.hpp"
 #include "device.hpp"
 #include "host.hpp"
+#include "rng_jump.hpp"
 
 namespace cpm {
 
@@ -30,6 +43,10 @@ namespace cpm {
                         std::string(#expr) + ": " + hipGetErrorString(_e));            \
     } while (0)
 
+constexpr uint32_t kSegLen = 16384;            // draws per stream segment
+constexpr uint32_t kSegs = 16;                 // segments per fill
+constexpr uint32_t kFill = kSegLen * kSegs;    // draws buffered per replica per round
+
 // ---------------------------------------------------------------------------
 // Replica descriptor: geometry, stencils, parameters, state pointers and the
 // scalar state a launch reads and writes back.
@@ -37,7 +54,7 @@ namespace cpm {
 struct Rep {
     uint32_t D, boundary, n_types, n_cells;
     uint32_t dims[3], strides[3];
-    uint32_t n_sites, n_copy, n_energy, n_topo;
+    uint32_t n_sites, n_padded, n_copy, n_energy, n_topo;
     int32_t copy_off[6], energy_off[26], topo_off[26];
     uint32_t* sigma;
     uint32_t* type_index;
@@ -48,19 +65,26 @@ struct Rep {
     uint32_t* contact;
     const double* params;     // lambda_v[T], lambda_i[T], J[(T+1)^2]
     const uint32_t* targets;  // V*[T], I*[T]
-    uint64_t rng[4];
+    uint64_t* draws;          // kFill buffered draws (LDS path)
+    uint64_t rng[4];          // consumer state at the start / end of a call
     double energy;
     uint64_t mcs;
-    uint64_t todo;            // MCS to run in this launch
+    uint64_t todo;            // MCS left to run in this call
+    uint32_t attempt_in_mcs;  // attempts done within the current MCS
+    uint32_t cursor;          // next unconsumed draw in `draws`
+    uint32_t done;
+    uint32_t filled;          // the stream buffer was filled in this call
+    uint32_t error;           // an attempt needed more draws than a window holds
+    uint64_t fill_state[4];   // state at draws[0]
+    uint64_t seg_state[kSegs][4];
     unsigned long long cnt[6];
 };
 
 __constant__ uint32_t c_sp2d[8];  // 256-bit 2D simple-point table
 
-// ---------------------------------------------------------------------------
-// One replica's serial chain.  D is a template parameter so coordinates,
-// strides and offsets live in registers.
-// ---------------------------------------------------------------------------
+// ===========================================================================
+// Global path: one lane per replica (lattices too large for LDS)
+// ===========================================================================
 template <int D>
 __device__ void replica_chain(Rep& R, const uint64_t* __restrict__ sp3d) {
     Xoshiro rng;
@@ -91,7 +115,6 @@ __device__ void replica_chain(Rep& R, const uint64_t* __restrict__ sp3d) {
 
     for (uint64_t m = 0; m < R.todo; ++m) {
         for (uint32_t a = 0; a < R.n_sites; ++a) {
-            // monte_carlo.rs:37-42 — target coordinates, axis 0 first
             uint32_t c[D];
             uint32_t t = 0;
             for (int d = 0; d < D; ++d) {
@@ -101,17 +124,15 @@ __device__ void replica_chain(Rep& R, const uint64_t* __restrict__ sp3d) {
                 c[d] = (uint32_t)v;
                 t += (c[d] + 1) * strides[d];
             }
-            // monte_carlo.rs:57 — copy source
             uint64_t k;
             while (!range_trial(xo_next(rng), n_copy, zone_k, &k)) {
             }
             const uint32_t s = (uint32_t)((int32_t)t + R.copy_off[k]);
             cnt[0]++;
             const uint32_t L = sig[t], G = sig[s];
-            if (L == G) continue;  // early-out
+            if (L == G) continue;
             cnt[1]++;
             if (L != 0) {
-                // connectivity.rs:42-65 — losing-cell pattern over the full shell
                 uint32_t pat = 0;
                 for (uint32_t i = 0; i < n_topo; ++i)
                     pat |= (uint32_t)(sig[(int32_t)t + R.topo_off[i]] == L) << i;
@@ -119,10 +140,9 @@ __device__ void replica_chain(Rep& R, const uint64_t* __restrict__ sp3d) {
                 const bool simple = D == 2 ? ((c_sp2d[pat >> 5] >> (pat & 31)) & 1u)
                                            : ((sp3d[pat >> 6] >> (pat & 63)) & 1ull);
                 if (!simple) continue;
-                if (vol[L - 1] <= 1) continue;  // min_cell_volume
+                if (vol[L - 1] <= 1) continue;
             }
             cnt[3]++;
-            // energy.rs:521-523 — (volume + interface) + adhesion
             const uint32_t tl = L ? typ[L - 1] : 0, tg = G ? typ[G - 1] : 0;
             double dvol = 0.0;
             if (L) {
@@ -158,7 +178,6 @@ __device__ void replica_chain(Rep& R, const uint64_t* __restrict__ sp3d) {
                 ditf += li[tg] * (x1 * x1) - li[tg] * (x0 * x0);
             }
             const double dh = dvol + ditf + dadh;
-            // monte_carlo.rs:158 — Metropolis at T = 1
             bool accept = dh <= 0.0;
             if (!accept) {
                 const double u = gen_f64(rng);
@@ -168,7 +187,6 @@ __device__ void replica_chain(Rep& R, const uint64_t* __restrict__ sp3d) {
             }
             if (!accept) continue;
             cnt[4]++;
-            // dynamics.rs:73-98 — contact counts on the pre-move lattice
             for (uint32_t q = 0; q < n_energy; ++q) {
                 const uint32_t nb = sig[(int32_t)t + R.energy_off[q]];
                 if (nb != L) {
@@ -176,13 +194,9 @@ __device__ void replica_chain(Rep& R, const uint64_t* __restrict__ sp3d) {
                     const uint32_t cur = R.contact[key];
                     R.contact[key] = cur ? cur - 1 : 0;
                 }
-                if (nb != G) {
-                    const uint64_t key = (uint64_t)min(G, nb) * ncell1 + max(G, nb);
-                    R.contact[key] += 1;
-                }
+                if (nb != G) R.contact[(uint64_t)min(G, nb) * ncell1 + max(G, nb)] += 1;
             }
             sig[t] = G;
-            // dynamics.rs:44-53, 106-110
             if (L) {
                 vol[L - 1] -= 1;
                 itf[L - 1] = (uint32_t)((int64_t)itf[L - 1] + (int64_t)round(dl));
@@ -191,7 +205,6 @@ __device__ void replica_chain(Rep& R, const uint64_t* __restrict__ sp3d) {
                 vol[G - 1] += 1;
                 itf[G - 1] = (uint32_t)((int64_t)itf[G - 1] + (int64_t)round(dg));
             }
-            // dynamics.rs:125-183 — unwrapped position sums
             if (L) {
                 int64_t* sp = R.sum_pos + (uint64_t)(L - 1) * D;
                 int64_t* so = R.sum_outer + (uint64_t)(L - 1) * D * D;
@@ -218,8 +231,7 @@ __device__ void replica_chain(Rep& R, const uint64_t* __restrict__ sp3d) {
                             best = lo;
                             bd = dlo;
                         }
-                        const double dhi = fabs((double)hi - cen[d]);
-                        if (dhi < bd) best = hi;
+                        if (fabs((double)hi - cen[d]) < bd) best = hi;
                     }
                     img[d] = best;
                     sp[d] += best;
@@ -227,7 +239,6 @@ __device__ void replica_chain(Rep& R, const uint64_t* __restrict__ sp3d) {
                 for (int d = 0; d < D; ++d)
                     for (int e = 0; e < D; ++e) so[d * D + e] += img[d] * img[e];
             }
-            // dynamics.rs:206-224 — periodic halo: rewrite t's mirrors only
             if (periodic) {
                 uint32_t opt[D][3], nopt[D];
                 bool edge = false;
@@ -251,38 +262,792 @@ __device__ void replica_chain(Rep& R, const uint64_t* __restrict__ sp3d) {
                     }
                 }
             }
-            energy += dh;  // monte_carlo.rs:270
+            energy += dh;
         }
     }
     for (int i = 0; i < 4; ++i) R.rng[i] = rng.s[i];
     R.energy = energy;
     R.mcs += R.todo;
-    for (int i = 0; i < 6; ++i) R.cnt[i] = cnt[i];
+    R.todo = 0;
+    R.done = 1;
+    for (int i = 0; i < 6; ++i) R.cnt[i] += cnt[i];
 }
 
-__global__ void __launch_bounds__(64) k_exact_replay(Rep* __restrict__ reps, uint32_t n,
+__global__ void __launch_bounds__(64) k_exact_replay(Rep* __restrict__ reps,
+                                                     const uint32_t* __restrict__ idx, uint32_t n,
                                                      const uint64_t* __restrict__ sp3d) {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    Rep& R = reps[r];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Rep& R = reps[idx[i]];
     if (R.D == 2) replica_chain<2>(R, sp3d);
     else replica_chain<3>(R, sp3d);
 }
 
-// ---------------------------------------------------------------------------
+// ===========================================================================
+// LDS path
+// ===========================================================================
+
+struct JumpTable {
+    uint64_t q[kSegs][4];  // x^(g * kSegLen) mod p(x)
+};
+
+// Fills every unfinished replica's draw buffer with kFill consecutive
+// draws of its stream starting at fill_state.  One lane per (replica,
+// segment): the lane jumps to its segment start and steps kSegLen times.
+__global__ void __launch_bounds__(256) k_rng_fill(Rep* __restrict__ reps, uint32_t n,
+                                                  JumpTable jt) {
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t r = tid / kSegs, g = tid % kSegs;
+    if (r >= n) return;
+    Rep& R = reps[r];
+    if (R.done || !R.draws) return;
+    if (g == 0) R.filled = 1;
+    Xoshiro s;
+    for (int k = 0; k < 4; ++k) s.s[k] = R.fill_state[k];
+    if (g) {  // state kSegLen * g draws ahead: sum_i q_i T^i s
+        Xoshiro acc{{0, 0, 0, 0}};
+        for (int i = 0; i < 256; ++i) {
+            if ((jt.q[g][i >> 6] >> (i & 63)) & 1)
+                for (int k = 0; k < 4; ++k) acc.s[k] ^= s.s[k];
+            xo_next(s);
+        }
+        s = acc;
+    }
+    for (int k = 0; k < 4; ++k) R.seg_state[g][k] = s.s[k];
+    uint64_t* __restrict__ out = R.draws + (size_t)g * kSegLen;
+    for (uint32_t i = 0; i < kSegLen; ++i) out[i] = xo_next(s);
+}
+
+// Consumer RNG state after a call: the segment state containing the cursor,
+// stepped to the cursor.
+__global__ void __launch_bounds__(64) k_rng_finalize(Rep* __restrict__ reps, uint32_t n) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    Rep& R = reps[r];
+    if (!R.draws || !R.filled) return;
+    const uint32_t g = min(R.cursor / kSegLen, kSegs - 1);
+    Xoshiro s;
+    for (int k = 0; k < 4; ++k) s.s[k] = R.seg_state[g][k];
+    for (uint32_t i = g * kSegLen; i < R.cursor; ++i) xo_next(s);
+    for (int k = 0; k < 4; ++k) R.rng[k] = s.s[k];
+}
+
+__device__ __forceinline__ uint32_t rl32(uint32_t v, uint32_t lane) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
+}
+__device__ __forceinline__ uint64_t rl64(uint64_t v, uint32_t lane) {
+    return ((uint64_t)rl32((uint32_t)(v >> 32), lane) << 32) | rl32((uint32_t)v, lane);
+}
+__device__ __forceinline__ double rlf64(double v, uint32_t lane) {
+    return __longlong_as_double((long long)rl64((uint64_t)__double_as_longlong(v), lane));
+}
+__device__ __forceinline__ uint32_t uni(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+// First set bit at or after p (< 128) in the 128-bit mask {m0, m1}; 128 if none.
+__device__ __forceinline__ uint32_t next_bit(uint64_t m0, uint64_t m1, uint32_t p) {
+    if (p < 64) {
+        const uint64_t m = m0 & (~0ull << p);
+        if (m) return (uint32_t)__builtin_ctzll(m);
+        p = 64;
+    }
+    if (p >= 128) return 128;
+    const uint64_t m = m1 & (~0ull << (p - 64));
+    return m ? 64 + (uint32_t)__builtin_ctzll(m) : 128;
+}
+
+// LDS layout of one replica (host computes the same offsets).
+struct LdsLayout {
+    uint32_t sigma, type, vol, itf, J, lv, li, tv, ti, total;
+};
+
+__host__ __device__ inline LdsLayout lds_layout(uint32_t n_padded, uint32_t n_cells, uint32_t T) {
+    LdsLayout L;
+    uint32_t off = 0;
+    auto take = [&](uint32_t bytes, uint32_t align) {
+        off = (off + align - 1) / align * align;
+        uint32_t at = off;
+        off += bytes;
+        return at;
+    };
+    L.J = take((T + 1) * (T + 1) * 8, 8);
+    L.lv = take(T * 8, 8);
+    L.li = take(T * 8, 8);
+    L.vol = take(n_cells * 4 + 4, 4);
+    L.itf = take(n_cells * 4 + 4, 4);
+    L.tv = take(T * 4, 4);
+    L.ti = take(T * 4, 4);
+    L.sigma = take(n_padded * 2, 4);
+    L.type = take(n_cells + 1, 4);
+    L.total = (off + 15) / 16 * 16;
+    return L;
+}
+
+// One consumer's per-lane sample for one 64-draw half of the window.
+struct Half {
+    uint64_t w;        // raw draw
+    uint32_t hi[4];    // sample for consumers 0..D-1 (axes) and D (copy index)
+};
+
+template <int D>
+__device__ __forceinline__ void prep_half(Half& h, uint64_t w, const uint32_t* range,
+                                          const uint64_t* zone, uint64_t* mask) {
+    h.w = w;
+#pragma unroll
+    for (int c = 0; c <= D; ++c) {
+        uint64_t v;
+        const bool ok = range_trial(w, range[c], zone[c], &v);
+        h.hi[c] = (uint32_t)v;
+        mask[c] = __ballot(ok);
+    }
+}
+
+// First set bit at or after the per-lane position p (0..129) of the 128-bit
+// mask {m0, m1}; 128 if there is none.
+__device__ __forceinline__ uint32_t lane_next_bit(uint64_t m0, uint64_t m1, uint32_t p) {
+    const uint64_t lo = p < 64 ? (m0 & (~0ull << (p & 63))) : 0ull;
+    const uint32_t ph = p < 64 ? 0u : p - 64;
+    const uint64_t hi = ph < 64 ? (m1 & (~0ull << (ph & 63))) : 0ull;
+    return lo ? (uint32_t)__builtin_ctzll(lo) : (hi ? 64u + (uint32_t)__builtin_ctzll(hi) : 128u);
+}
+
+__device__ __forceinline__ uint32_t bperm(uint32_t v, uint32_t src_lane) {
+    return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src_lane << 2), (int)v);
+}
+
+// Exact replay on the LDS path: one wave per replica.  Every lane decodes
+// the attempt that would start at its own position of the 64-draw window
+// (the first accepted draw of each consumer, found in the 128-draw window
+// by mask search) and reads sigma at its target and source; the wave then
+// walks the actual chain of attempts through those decodes: a null attempt
+// costs a bit test and a lane read.  Non-null attempts are resolved one at
+// a time, in order, with the neighbourhood spread over lanes.
+template <int D>
+__global__ void __launch_bounds__(64) k_lds_engine(Rep* __restrict__ reps,
+                                                   const uint32_t* __restrict__ idx,
+                                                   const uint64_t* __restrict__ sp3d) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    Rep& R = reps[idx[blockIdx.x]];
+    if (R.done) return;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t T = R.n_types, T1 = T + 1, n_cells = R.n_cells;
+    const uint32_t n_padded = R.n_padded, n_sites = R.n_sites;
+    const LdsLayout Ly = lds_layout(n_padded, n_cells, T);
+    uint16_t* sig = (uint16_t*)(smem + Ly.sigma);
+    uint8_t* typ = smem + Ly.type;  // typ[id], typ[0] unused
+    uint32_t* vol = (uint32_t*)(smem + Ly.vol);
+    uint32_t* itf = (uint32_t*)(smem + Ly.itf);
+    double* J = (double*)(smem + Ly.J);
+    double* lv = (double*)(smem + Ly.lv);
+    double* li = (double*)(smem + Ly.li);
+    uint32_t* tv = (uint32_t*)(smem + Ly.tv);
+    uint32_t* ti = (uint32_t*)(smem + Ly.ti);
+
+    // ---- stage the replica into LDS
+    for (uint32_t i = lane; i < n_padded; i += 64) sig[i] = (uint16_t)R.sigma[i];
+    for (uint32_t i = lane; i < n_cells; i += 64) {
+        typ[i + 1] = (uint8_t)R.type_index[i];
+        vol[i] = R.volume[i];
+        itf[i] = R.interface[i];
+    }
+    for (uint32_t i = lane; i < T1 * T1; i += 64) J[i] = R.params[2 * T + i];
+    for (uint32_t i = lane; i < T; i += 64) {
+        lv[i] = R.params[i];
+        li[i] = R.params[T + i];
+        tv[i] = R.targets[i];
+        ti[i] = R.targets[T + i];
+    }
+    __syncthreads();
+
+    // ---- per-lane stencil offsets, uniform geometry
+    const uint32_t n_topo = R.n_topo, n_energy = R.n_energy, n_copy = R.n_copy;
+    const int32_t toff = lane < n_topo ? R.topo_off[lane] : 0;
+    const int32_t eoff = lane < n_energy ? R.energy_off[lane] : 0;
+    const uint32_t coff = lane < n_copy ? (uint32_t)R.copy_off[lane] : 0u;
+    uint32_t range[D + 1], stride[D], dims[D];
+    uint64_t zone[D + 1];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        dims[d] = R.dims[d];
+        range[d] = dims[d];
+        stride[d] = R.strides[d];
+        zone[d] = range_zone(range[d]);
+    }
+    range[D] = n_copy;
+    zone[D] = range_zone(n_copy);
+    uint32_t t_safe = 0;  // an interior site, for lanes whose decode is void
+#pragma unroll
+    for (int d = 0; d < D; ++d) t_safe += stride[d];
+    const uint64_t sp2[4] = {(uint64_t)c_sp2d[0] | (uint64_t)c_sp2d[1] << 32,
+                             (uint64_t)c_sp2d[2] | (uint64_t)c_sp2d[3] << 32,
+                             (uint64_t)c_sp2d[4] | (uint64_t)c_sp2d[5] << 32,
+                             (uint64_t)c_sp2d[6] | (uint64_t)c_sp2d[7] << 32};
+    const bool periodic = R.boundary == CPM_BOUNDARY_PERIODIC;
+    const uint64_t ncell1 = (uint64_t)n_cells + 1;
+    const uint64_t* __restrict__ draws = R.draws;
+
+    uint64_t todo = R.todo;
+    uint32_t ain = R.attempt_in_mcs;
+    uint64_t mcs = R.mcs;
+    double energy = R.energy;
+    uint32_t c_att = 0, c_nn = 0, c_top = 0, c_en = 0, c_acc = 0, c_tie = 0;
+    bool dirty = false, finished = false, error = false;
+
+    // ---- the 128-draw register window: halves [base, base+64), [base+64, base+128)
+    uint32_t base = R.cursor, p = 0;
+    Half h0, h1;
+    uint64_t m0[D + 1], m1[D + 1];
+    bool h1_valid = false;
+    uint64_t w_next = 0;
+    // per-lane decode of the attempt starting at this lane's window position
+    uint32_t dec_t = 0, dec_s = 0, dec_end = 0, dec_L = 0, dec_G = 0, dec_c[D];
+    uint64_t VALID = 0, NULLM = 0;
+
+    auto load_window = [&]() {
+        h1_valid = base + 128 <= kFill;
+        prep_half<D>(h0, draws[base + lane], range, zone, m0);
+        prep_half<D>(h1, h1_valid ? draws[base + 64 + lane] : 0ull, range, zone, m1);
+        w_next = base + 192 <= kFill ? draws[base + 128 + lane] : 0ull;
+    };
+    auto reread = [&]() {
+        dec_L = sig[dec_t];
+        dec_G = sig[dec_s];
+        NULLM = __ballot(((VALID >> lane) & 1) && dec_L == dec_G);
+    };
+    auto decode = [&]() {
+        const uint64_t hv = h1_valid ? ~0ull : 0ull;
+        uint32_t pos = lane, t = 0, kv = 0;
+        bool valid = true;
+#pragma unroll
+        for (int c = 0; c <= D; ++c) {
+            const uint32_t q = lane_next_bit(m0[c], m1[c] & hv, pos);
+            valid = valid && q < 128;
+            const uint32_t v0 = bperm(h0.hi[c], q & 63), v1 = bperm(h1.hi[c], q & 63);
+            const uint32_t v = q < 64 ? v0 : v1;
+            if (c < D) {
+                dec_c[c] = v;
+                t += (v + 1) * stride[c];
+            } else {
+                kv = v;
+            }
+            pos = q + 1;
+        }
+        const uint32_t off = bperm(coff, kv);
+        dec_t = valid ? t : t_safe;
+        dec_s = valid ? (uint32_t)((int32_t)t + (int32_t)off) : t_safe;
+        dec_end = pos;
+        VALID = __ballot(valid);
+        reread();
+    };
+
+    if (base + 64 > kFill) goto exit_round;  // no full window left in this fill
+    load_window();
+    decode();
+
+    while (todo > 0) {
+        if (p >= 64) {  // slide the window by one half
+            if (!h1_valid) break;
+            base += 64;
+            p -= 64;
+            h0 = h1;
+#pragma unroll
+            for (int c = 0; c <= D; ++c) m0[c] = m1[c];
+            h1_valid = base + 128 <= kFill;
+            prep_half<D>(h1, h1_valid ? w_next : 0ull, range, zone, m1);
+            w_next = base + 192 <= kFill ? draws[base + 128 + lane] : 0ull;
+            decode();
+            continue;
+        }
+        if (!((VALID >> p) & 1)) {
+            // The attempt at p does not fit in the window: restart the
+            // window at p, or stop the round (the next round's window
+            // begins here).  An attempt needing more than 128 draws even
+            // from a window of its own is reported, never looped on.
+            if (p == 0 && h1_valid) {
+                error = true;
+                break;
+            }
+            if (base + p + 128 > kFill) break;
+            base += p;
+            p = 0;
+            load_window();
+            decode();
+            continue;
+        }
+        ++c_att;
+        if ((NULLM >> p) & 1) {
+            p = rl32(dec_end, p);  // early-out: sigma_t == sigma_s
+        } else {
+            ++c_nn;
+            const uint32_t t = rl32(dec_t, p), s = rl32(dec_s, p);
+            const uint32_t L = rl32(dec_L, p), G = rl32(dec_G, p);
+            const uint32_t e = rl32(dec_end, p);
+            (void)s;
+            // neighbourhood reads for topology and energy, issued together
+            const uint32_t nbt = sig[(int32_t)t + toff];
+            const uint32_t nbe = sig[(int32_t)t + eoff];
+            const uint32_t vL = L ? uni(vol[L - 1]) : 0u, vG = G ? uni(vol[G - 1]) : 0u;
+            const uint32_t iL = L ? uni(itf[L - 1]) : 0u, iG = G ? uni(itf[G - 1]) : 0u;
+            const uint32_t tl = L ? uni(typ[L]) : 0u, tg = G ? uni(typ[G]) : 0u;
+            bool go = true;
+            if (L != 0) {
+                const uint32_t pat = (uint32_t)__ballot(lane < n_topo && nbt == L);
+                ++c_top;
+                bool simple;
+                if (D == 2) {
+                    const uint64_t w = pat < 128 ? (pat < 64 ? sp2[0] : sp2[1])
+                                                 : (pat < 192 ? sp2[2] : sp2[3]);
+                    simple = (w >> (pat & 63)) & 1ull;
+                } else {
+                    simple = (sp3d[pat >> 6] >> (pat & 63)) & 1ull;
+                }
+                go = simple && vL > 1;
+            }
+            uint32_t p_next = e;
+            if (go) {
+                ++c_en;
+                // ---- local dH (energy.rs:521-523)
+                const uint64_t mL = __ballot(lane < n_energy && nbe == L);
+                const uint64_t mG = __ballot(lane < n_energy && nbe == G);
+                const int dl_i = 2 * __popcll(mL) - (int)n_energy;
+                const int dg_i = (int)n_energy - 2 * __popcll(mG);
+                const uint32_t lidx = L ? tl + 1 : 0u, gidx = G ? tg + 1 : 0u;
+                const uint32_t ni = nbe ? (uint32_t)typ[nbe] + 1 : 0u;
+                const double before = nbe != L ? J[lidx * T1 + ni] : 0.0;
+                const double after = nbe != G ? J[gidx * T1 + ni] : 0.0;
+                const double term = 1.0 * (after - before);
+                double dvol = 0.0, ditf = 0.0;
+                if (L) {
+                    const double v = (double)vL, T0 = (double)tv[tl];
+                    const double x1 = (v - 1.0) - T0, x0 = v - T0;
+                    dvol += lv[tl] * (x1 * x1) - lv[tl] * (x0 * x0);
+                    const double i0 = (double)iL, I0 = (double)ti[tl];
+                    const double y1 = (i0 + (double)dl_i) - I0, y0 = i0 - I0;
+                    ditf += li[tl] * (y1 * y1) - li[tl] * (y0 * y0);
+                }
+                if (G) {
+                    const double v = (double)vG, T0 = (double)tv[tg];
+                    const double x1 = (v + 1.0) - T0, x0 = v - T0;
+                    dvol += lv[tg] * (x1 * x1) - lv[tg] * (x0 * x0);
+                    const double i0 = (double)iG, I0 = (double)ti[tg];
+                    const double y1 = (i0 + (double)dg_i) - I0, y0 = i0 - I0;
+                    ditf += li[tg] * (y1 * y1) - li[tg] * (y0 * y0);
+                }
+                double dadh = 0.0;  // summed in stencil order
+                for (uint32_t q = 0; q < n_energy; ++q) dadh += rlf64(term, q);
+                const double dh = dvol + ditf + dadh;
+                bool accept = dh <= 0.0;
+                if (!accept) {
+                    // one more draw: rng.gen::<f64>()
+                    if (e >= 128 || (e >= 64 && !h1_valid)) {
+                        // the window ends here: redo the whole attempt next round
+                        if (p == 0 && h1_valid) error = true;
+                        --c_att;
+                        --c_nn;
+                        if (L != 0) --c_top;
+                        --c_en;
+                        break;
+                    }
+                    const uint64_t w = e < 64 ? rl64(h0.w, e) : rl64(h1.w, e - 64);
+                    p_next = e + 1;
+                    const double u = (double)(w >> 11) * (1.0 / 9007199254740992.0);
+                    const double pr = exp(-dh);
+                    accept = u < pr;
+                    if (fabs(u - pr) <= pr * 0x1p-50) ++c_tie;
+                }
+                if (accept) {
+                    ++c_acc;
+                    dirty = true;
+                    uint32_t cval[D];
+#pragma unroll
+                    for (int d = 0; d < D; ++d) cval[d] = rl32(dec_c[d], p);
+                    // dynamics.rs:73-98 — contact counts, pre-move lattice
+                    if (lane < n_energy) {
+                        if (nbe != L)
+                            atomicSub(&R.contact[(uint64_t)min(L, nbe) * ncell1 + max(L, nbe)], 1u);
+                        if (nbe != G)
+                            atomicAdd(&R.contact[(uint64_t)min(G, nbe) * ncell1 + max(G, nbe)], 1u);
+                    }
+                    if (lane == 0) {
+                        sig[t] = (uint16_t)G;
+                        if (L) {
+                            vol[L - 1] = vL - 1;
+                            itf[L - 1] = (uint32_t)((int32_t)iL + dl_i);
+                        }
+                        if (G) {
+                            vol[G - 1] = vG + 1;
+                            itf[G - 1] = (uint32_t)((int32_t)iG + dg_i);
+                        }
+                        // dynamics.rs:125-183 — unwrapped position sums
+                        if (L) {
+                            int64_t* sp = R.sum_pos + (uint64_t)(L - 1) * D;
+                            int64_t* so = R.sum_outer + (uint64_t)(L - 1) * D * D;
+                            for (int d = 0; d < D; ++d) sp[d] -= (int64_t)cval[d];
+                            for (int d = 0; d < D; ++d)
+                                for (int e2 = 0; e2 < D; ++e2)
+                                    so[d * D + e2] -= (int64_t)cval[d] * (int64_t)cval[e2];
+                        }
+                        if (G) {
+                            int64_t* sp = R.sum_pos + (uint64_t)(G - 1) * D;
+                            int64_t* so = R.sum_outer + (uint64_t)(G - 1) * D * D;
+                            const uint32_t v = vG + 1;
+                            const uint32_t vbg = max(v > 0 ? v - 1 : 0u, 1u);
+                            double cen[D];
+                            for (int d = 0; d < D; ++d) cen[d] = (double)sp[d] / (double)vbg;
+                            int64_t img[D];
+                            for (int d = 0; d < D; ++d) {
+                                int64_t best = (int64_t)cval[d];
+                                if (periodic) {
+                                    double bd = fabs((double)best - cen[d]);
+                                    const int64_t lo = (int64_t)cval[d] - (int64_t)dims[d];
+                                    const int64_t hi = (int64_t)cval[d] + (int64_t)dims[d];
+                                    const double dlo = fabs((double)lo - cen[d]);
+                                    if (dlo < bd) {
+                                        best = lo;
+                                        bd = dlo;
+                                    }
+                                    if (fabs((double)hi - cen[d]) < bd) best = hi;
+                                }
+                                img[d] = best;
+                                sp[d] += best;
+                            }
+                            for (int d = 0; d < D; ++d)
+                                for (int e2 = 0; e2 < D; ++e2) so[d * D + e2] += img[d] * img[e2];
+                        }
+                        // dynamics.rs:206-224 — rewrite t's halo mirrors
+                        if (periodic) {
+                            uint32_t opt[D][3], nopt[D], total = 1;
+                            for (int d = 0; d < D; ++d) {
+                                nopt[d] = 1;
+                                opt[d][0] = cval[d] + 1;
+                                if (cval[d] == 0) opt[d][nopt[d]++] = dims[d] + 1;
+                                if (cval[d] == dims[d] - 1) opt[d][nopt[d]++] = 0;
+                                total *= nopt[d];
+                            }
+                            for (uint32_t q = 1; q < total; ++q) {
+                                uint32_t r = q, f = 0;
+                                for (int d = D - 1; d >= 0; --d) {
+                                    f += opt[d][r % nopt[d]] * stride[d];
+                                    r /= nopt[d];
+                                }
+                                sig[f] = (uint16_t)G;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                    energy += dh;
+                    reread();  // the lattice changed: refresh the early-out flags
+                }
+            }
+            p = p_next;
+        }
+        // monte_carlo.rs:330-356 — MCS accounting
+        if (++ain == n_sites) {
+            ain = 0;
+            ++mcs;
+            --todo;
+        }
+    }
+    if (todo == 0) finished = true;
+    base += p;  // absolute cursor of the next unconsumed draw
+
+exit_round:
+    __syncthreads();
+    if (dirty) {
+        for (uint32_t i = lane; i < n_padded; i += 64) R.sigma[i] = sig[i];
+        for (uint32_t i = lane; i < n_cells; i += 64) {
+            R.volume[i] = vol[i];
+            R.interface[i] = itf[i];
+        }
+    }
+    if (lane == 0) {
+        R.todo = todo;
+        R.attempt_in_mcs = ain;
+        R.mcs = mcs;
+        R.energy = energy;
+        R.cnt[0] += c_att;
+        R.cnt[1] += c_nn;
+        R.cnt[2] += c_top;
+        R.cnt[3] += c_en;
+        R.cnt[4] += c_acc;
+        R.cnt[5] += c_tie;
+        if (error) {
+            R.error = 1;
+            R.done = 1;
+        } else if (finished) {
+            R.cursor = base;
+            R.done = 1;
+        } else {
+            // rebase the stream onto the segment holding the cursor; the
+            // next fill regenerates from that segment's start
+            const uint32_t g = min(base / kSegLen, kSegs - 1);
+            for (int k = 0; k < 4; ++k) R.fill_state[k] = R.seg_state[g][k];
+            R.cursor = base - g * kSegLen;
+        }
+    }
+}
+
+// ===========================================================================
+// Sublattice-parallel mode: halo-disjoint checkerboard colours updated in
+// parallel.  An attempt at t reads only t's 3^D neighbourhood and writes
+// only sigma[t]; sites of one colour (c_d mod 2 on every axis) are at
+// Chebyshev distance >= 2, so within a colour phase no site reads what
+// another writes (periodic sides are even, so the wrap keeps the
+// colouring).  Same-colour removals are never 8/26-adjacent, so each stays
+// a simple point after the others: topology checks stay exact.  Per-cell V
+// and I are read as they stand (possibly mid-phase) and updated with
+// atomics.  Each site is the target of exactly one attempt per MCS; the
+// colour order is redrawn every MCS.  Random numbers come from SplitMix64
+// indexed by (replica key, mcs, colour, site, draw).
+// ===========================================================================
+__device__ __forceinline__ uint64_t splitmix_at(uint64_t key, uint64_t ctr) {
+    uint64_t z = key + ctr * 0x9e3779b97f4a7c15ULL;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+constexpr uint32_t kSlThreads = 256;
+
+template <int D>
+__global__ void __launch_bounds__(kSlThreads) k_sublattice(Rep* __restrict__ reps,
+                                                          const uint32_t* __restrict__ idx,
+                                                          const uint64_t* __restrict__ sp3d) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ unsigned long long s_cnt[6];
+    Rep& R = reps[idx[blockIdx.x]];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t T = R.n_types, T1 = T + 1, n_cells = R.n_cells;
+    const uint32_t n_padded = R.n_padded;
+    const LdsLayout Ly = lds_layout(n_padded, n_cells, T);
+    uint16_t* sig = (uint16_t*)(smem + Ly.sigma);
+    uint8_t* typ = smem + Ly.type;
+    uint32_t* vol = (uint32_t*)(smem + Ly.vol);
+    uint32_t* itf = (uint32_t*)(smem + Ly.itf);
+    double* J = (double*)(smem + Ly.J);
+    double* lv = (double*)(smem + Ly.lv);
+    double* li = (double*)(smem + Ly.li);
+    uint32_t* tv = (uint32_t*)(smem + Ly.tv);
+    uint32_t* ti = (uint32_t*)(smem + Ly.ti);
+    for (uint32_t i = tid; i < n_padded; i += kSlThreads) sig[i] = (uint16_t)R.sigma[i];
+    for (uint32_t i = tid; i < n_cells; i += kSlThreads) {
+        typ[i + 1] = (uint8_t)R.type_index[i];
+        vol[i] = R.volume[i];
+        itf[i] = R.interface[i];
+    }
+    for (uint32_t i = tid; i < T1 * T1; i += kSlThreads) J[i] = R.params[2 * T + i];
+    for (uint32_t i = tid; i < T; i += kSlThreads) {
+        lv[i] = R.params[i];
+        li[i] = R.params[T + i];
+        tv[i] = R.targets[i];
+        ti[i] = R.targets[T + i];
+    }
+    if (tid < 6) s_cnt[tid] = 0;
+    __syncthreads();
+
+    uint32_t dims[D], stride[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        dims[d] = R.dims[d];
+        stride[d] = R.strides[d];
+    }
+    const uint32_t n_copy = R.n_copy, n_energy = R.n_energy, n_topo = R.n_topo;
+    const bool periodic = R.boundary == CPM_BOUNDARY_PERIODIC;
+    const uint64_t ncell1 = (uint64_t)n_cells + 1;
+    const uint64_t key = R.fill_state[0];
+    unsigned long long c_att = 0, c_nn = 0, c_top = 0, c_en = 0, c_acc = 0;
+    constexpr uint32_t NC = 1u << D;
+
+    for (uint64_t m = 0; m < R.todo; ++m) {
+        const uint64_t mcs = R.mcs + m;
+        // colour order for this MCS: Fisher-Yates from one hash
+        uint32_t order[NC];
+        for (uint32_t c = 0; c < NC; ++c) order[c] = c;
+        uint64_t hperm = splitmix_at(key, mcs * 0x100000001ull + 0xC01);
+        for (uint32_t c = NC - 1; c > 0; --c) {
+            const uint32_t j = (uint32_t)((hperm & 0xffff) % (c + 1));
+            hperm >>= 8;
+            const uint32_t tmp = order[c];
+            order[c] = order[j];
+            order[j] = tmp;
+        }
+        for (uint32_t ci = 0; ci < NC; ++ci) {
+            const uint32_t col = order[ci];
+            uint32_t half[D], nsite = 1;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const uint32_t cd = (col >> d) & 1u;
+                half[d] = dims[d] > cd ? (dims[d] - cd + 1) / 2 : 0;
+                nsite *= half[d];
+            }
+            for (uint32_t j = tid; j < nsite; j += kSlThreads) {
+                uint32_t rem = j, c[D], t = 0;
+#pragma unroll
+                for (int d = D - 1; d >= 0; --d) {
+                    c[d] = 2 * (rem % half[d]) + ((col >> d) & 1u);
+                    rem /= half[d];
+                    t += (c[d] + 1) * stride[d];
+                }
+                const uint64_t ctr = ((mcs * NC + col) * (uint64_t)R.n_sites + j) * 2;
+                const uint64_t hk = splitmix_at(key, ctr);
+                const uint32_t k = (uint32_t)(((hk >> 32) * (uint64_t)n_copy) >> 32);
+                const uint32_t s = (uint32_t)((int32_t)t + R.copy_off[k]);
+                c_att++;
+                const uint32_t L = sig[t], G = sig[s];
+                if (L == G) continue;
+                c_nn++;
+                if (L != 0) {
+                    uint32_t pat = 0;
+                    for (uint32_t i = 0; i < n_topo; ++i)
+                        pat |= (uint32_t)(sig[(int32_t)t + R.topo_off[i]] == L) << i;
+                    c_top++;
+                    const bool simple = D == 2 ? ((c_sp2d[pat >> 5] >> (pat & 31)) & 1u)
+                                               : ((sp3d[pat >> 6] >> (pat & 63)) & 1ull);
+                    if (!simple) continue;
+                    if (__atomic_load_n(&vol[L - 1], __ATOMIC_RELAXED) <= 1) continue;
+                }
+                c_en++;
+                const uint32_t tl = L ? typ[L] : 0u, tg = G ? typ[G] : 0u;
+                const uint32_t lidx = L ? tl + 1 : 0u, gidx = G ? tg + 1 : 0u;
+                int nl = 0, ng = 0;
+                double dadh = 0.0;
+                for (uint32_t q = 0; q < n_energy; ++q) {
+                    const uint32_t nb = sig[(int32_t)t + R.energy_off[q]];
+                    nl += nb == L;
+                    ng += nb == G;
+                    const uint32_t ni = nb ? (uint32_t)typ[nb] + 1 : 0u;
+                    const double before = nb != L ? J[lidx * T1 + ni] : 0.0;
+                    const double after = nb != G ? J[gidx * T1 + ni] : 0.0;
+                    dadh += 1.0 * (after - before);
+                }
+                const int dl_i = 2 * nl - (int)n_energy, dg_i = (int)n_energy - 2 * ng;
+                double dvol = 0.0, ditf = 0.0;
+                if (L) {
+                    const double v = (double)__atomic_load_n(&vol[L - 1], __ATOMIC_RELAXED);
+                    const double x1 = (v - 1.0) - (double)tv[tl], x0 = v - (double)tv[tl];
+                    dvol += lv[tl] * (x1 * x1) - lv[tl] * (x0 * x0);
+                    const double i0 = (double)__atomic_load_n(&itf[L - 1], __ATOMIC_RELAXED);
+                    const double y1 = (i0 + (double)dl_i) - (double)ti[tl], y0 = i0 - (double)ti[tl];
+                    ditf += li[tl] * (y1 * y1) - li[tl] * (y0 * y0);
+                }
+                if (G) {
+                    const double v = (double)__atomic_load_n(&vol[G - 1], __ATOMIC_RELAXED);
+                    const double x1 = (v + 1.0) - (double)tv[tg], x0 = v - (double)tv[tg];
+                    dvol += lv[tg] * (x1 * x1) - lv[tg] * (x0 * x0);
+                    const double i0 = (double)__atomic_load_n(&itf[G - 1], __ATOMIC_RELAXED);
+                    const double y1 = (i0 + (double)dg_i) - (double)ti[tg], y0 = i0 - (double)ti[tg];
+                    ditf += li[tg] * (y1 * y1) - li[tg] * (y0 * y0);
+                }
+                const double dh = dvol + ditf + dadh;
+                if (!(dh <= 0.0)) {
+                    const uint64_t hu = splitmix_at(key, ctr + 1);
+                    const double u = (double)(hu >> 11) * (1.0 / 9007199254740992.0);
+                    if (!(u < exp(-dh))) continue;
+                }
+                c_acc++;
+                for (uint32_t q = 0; q < n_energy; ++q) {
+                    const uint32_t nb = sig[(int32_t)t + R.energy_off[q]];
+                    if (nb != L) atomicSub(&R.contact[(uint64_t)min(L, nb) * ncell1 + max(L, nb)], 1u);
+                    if (nb != G) atomicAdd(&R.contact[(uint64_t)min(G, nb) * ncell1 + max(G, nb)], 1u);
+                }
+                sig[t] = (uint16_t)G;
+                uint32_t vg_new = 0;
+                if (L) {
+                    atomicSub(&vol[L - 1], 1u);
+                    atomicAdd(&itf[L - 1], (uint32_t)dl_i);
+                    unsigned long long* sp = (unsigned long long*)(R.sum_pos + (uint64_t)(L - 1) * D);
+                    unsigned long long* so = (unsigned long long*)(R.sum_outer + (uint64_t)(L - 1) * D * D);
+                    for (int d = 0; d < D; ++d) atomicAdd(&sp[d], (unsigned long long)(-(int64_t)c[d]));
+                    for (int d = 0; d < D; ++d)
+                        for (int e2 = 0; e2 < D; ++e2)
+                            atomicAdd(&so[d * D + e2],
+                                      (unsigned long long)(-((int64_t)c[d] * (int64_t)c[e2])));
+                }
+                if (G) {
+                    vg_new = atomicAdd(&vol[G - 1], 1u) + 1;
+                    atomicAdd(&itf[G - 1], (uint32_t)dg_i);
+                    int64_t* spg = R.sum_pos + (uint64_t)(G - 1) * D;
+                    unsigned long long* sp = (unsigned long long*)spg;
+                    unsigned long long* so = (unsigned long long*)(R.sum_outer + (uint64_t)(G - 1) * D * D);
+                    const uint32_t vbg = max(vg_new > 0 ? vg_new - 1 : 0u, 1u);
+                    int64_t img[D];
+                    for (int d = 0; d < D; ++d) {
+                        const double cen = (double)__atomic_load_n(&spg[d], __ATOMIC_RELAXED) / (double)vbg;
+                        int64_t best = (int64_t)c[d];
+                        if (periodic) {
+                            double bd = fabs((double)best - cen);
+                            const int64_t lo = (int64_t)c[d] - (int64_t)dims[d];
+                            const int64_t hi = (int64_t)c[d] + (int64_t)dims[d];
+                            const double dlo = fabs((double)lo - cen);
+                            if (dlo < bd) {
+                                best = lo;
+                                bd = dlo;
+                            }
+                            if (fabs((double)hi - cen) < bd) best = hi;
+                        }
+                        img[d] = best;
+                    }
+                    for (int d = 0; d < D; ++d) atomicAdd(&sp[d], (unsigned long long)img[d]);
+                    for (int d = 0; d < D; ++d)
+                        for (int e2 = 0; e2 < D; ++e2)
+                            atomicAdd(&so[d * D + e2], (unsigned long long)(img[d] * img[e2]));
+                }
+                if (periodic) {
+                    uint32_t opt[D][3], nopt[D], total = 1;
+                    for (int d = 0; d < D; ++d) {
+                        nopt[d] = 1;
+                        opt[d][0] = c[d] + 1;
+                        if (c[d] == 0) opt[d][nopt[d]++] = dims[d] + 1;
+                        if (c[d] == dims[d] - 1) opt[d][nopt[d]++] = 0;
+                        total *= nopt[d];
+                    }
+                    for (uint32_t q = 1; q < total; ++q) {
+                        uint32_t r = q, f = 0;
+                        for (int d = D - 1; d >= 0; --d) {
+                            f += opt[d][r % nopt[d]] * stride[d];
+                            r /= nopt[d];
+                        }
+                        sig[f] = (uint16_t)G;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    atomicAdd(&s_cnt[0], c_att);
+    atomicAdd(&s_cnt[1], c_nn);
+    atomicAdd(&s_cnt[2], c_top);
+    atomicAdd(&s_cnt[3], c_en);
+    atomicAdd(&s_cnt[4], c_acc);
+    __syncthreads();
+    for (uint32_t i = tid; i < n_padded; i += kSlThreads) R.sigma[i] = sig[i];
+    for (uint32_t i = tid; i < n_cells; i += kSlThreads) {
+        R.volume[i] = vol[i];
+        R.interface[i] = itf[i];
+    }
+    if (tid == 0) {
+        R.mcs += R.todo;
+        R.todo = 0;
+        R.done = 1;
+        for (int i = 0; i < 5; ++i) R.cnt[i] += s_cnt[i];
+    }
+}
+
+// ===========================================================================
 // 3D simple-point table, generated from the definition on the device: one
 // thread per 64 patterns (simple_point.rs:242-276, 340-354).
-// ---------------------------------------------------------------------------
+// ===========================================================================
 struct SpMasks {
     uint32_t face[26], full[26], origin_face, all;
 };
 
-__device__ __forceinline__ uint32_t sp_grow(uint32_t seed, uint32_t uni, const uint32_t* adj) {
+__device__ __forceinline__ uint32_t sp_grow(uint32_t seed, uint32_t uni_, const uint32_t* adj) {
     uint32_t comp = seed;
     for (;;) {
         uint32_t nxt = comp;
         for (uint32_t f = comp; f; f &= f - 1) nxt |= adj[__builtin_ctz(f)];
-        nxt &= uni;
+        nxt &= uni_;
         if (nxt == comp) return comp;
         comp = nxt;
     }
@@ -316,12 +1081,14 @@ __global__ void __launch_bounds__(256) k_sp3d(SpMasks M, uint64_t* __restrict__ 
     out[w] = bits;
 }
 
-// ---------------------------------------------------------------------------
+// ===========================================================================
 // Host side
-// ---------------------------------------------------------------------------
+// ===========================================================================
 struct Device {
     char* base = nullptr;
     Rep rep{};  // device pointers + static fields
+    bool lds = false;
+    uint32_t lds_bytes = 0;
 };
 
 struct Engine {
@@ -330,7 +1097,16 @@ struct Engine {
     uint64_t* sp3d = nullptr;
     Rep* reps_dev = nullptr;
     size_t reps_cap = 0;
+    uint32_t* idx_dev = nullptr;
+    size_t idx_cap = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // replicas of different LDS footprints run as separate launches on
+    // these streams, concurrently, after each round's stream fill
+    static constexpr int kClassStreams = 4;
+    hipStream_t cls_stream[kClassStreams] = {};
+    hipEvent_t cls_done[kClassStreams] = {}, fill_done = nullptr;
+    JumpTable jt{};
+    uint32_t max_lds = 0;
     std::mutex mu;
 };
 
@@ -344,12 +1120,28 @@ Engine* engine_create(int device) {
     HIP_OK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     HIP_OK(hipEventCreate(&e->ev0));
     HIP_OK(hipEventCreate(&e->ev1));
+    for (int i = 0; i < Engine::kClassStreams; ++i) {
+        HIP_OK(hipStreamCreateWithFlags(&e->cls_stream[i], hipStreamNonBlocking));
+        HIP_OK(hipEventCreateWithFlags(&e->cls_done[i], hipEventDisableTiming));
+    }
+    HIP_OK(hipEventCreateWithFlags(&e->fill_done, hipEventDisableTiming));
     uint8_t t2[256];
     sp_table2d(t2);
     uint32_t w[8] = {0};
     for (int p = 0; p < 256; ++p)
         if (t2[p]) w[p >> 5] |= 1u << (p & 31);
     HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_sp2d), w, sizeof w));
+    xo_jump_selfcheck();
+    for (uint32_t g = 0; g < kSegs; ++g) {
+        JumpPoly q = xo_jump_poly((uint64_t)g * kSegLen);
+        std::memcpy(e->jt.q[g], q.w, 32);
+    }
+    int lds = 0;
+    HIP_OK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    e->max_lds = (uint32_t)lds;
+    for (auto fn : {(const void*)k_lds_engine<2>, (const void*)k_lds_engine<3>,
+                    (const void*)k_sublattice<2>, (const void*)k_sublattice<3>})
+        HIP_OK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds - 256));
     return e;
 }
 
@@ -386,8 +1178,14 @@ void engine_destroy(Engine* e) {
     (void)hipSetDevice(e->device);
     if (e->sp3d) (void)hipFree(e->sp3d);
     if (e->reps_dev) (void)hipFree(e->reps_dev);
+    if (e->idx_dev) (void)hipFree(e->idx_dev);
     if (e->ev0) (void)hipEventDestroy(e->ev0);
     if (e->ev1) (void)hipEventDestroy(e->ev1);
+    for (int i = 0; i < Engine::kClassStreams; ++i) {
+        if (e->cls_stream[i]) (void)hipStreamDestroy(e->cls_stream[i]);
+        if (e->cls_done[i]) (void)hipEventDestroy(e->cls_done[i]);
+    }
+    if (e->fill_done) (void)hipEventDestroy(e->fill_done);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
@@ -407,13 +1205,12 @@ void engine_sp_tables(Engine* e, uint8_t* t2, uint64_t* t3) {
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// Layout of one model's device block.
 struct Layout {
     size_t sigma, type_index, volume, interface, sum_pos, sum_outer, contact, params, targets,
-        total;
+        draws, total;
 };
 
-static Layout layout_of(const Model& m) {
+static Layout layout_of(const Model& m, bool with_draws) {
     Layout L{};
     size_t off = 0;
     auto take = [&](size_t bytes) {
@@ -431,14 +1228,30 @@ static Layout layout_of(const Model& m) {
     L.contact = take((n + 1) * (n + 1) * 4);
     L.params = take((2 * T + (T + 1) * (T + 1)) * 8);
     L.targets = take(2 * T * 4);
+    L.draws = with_draws ? take((size_t)kFill * 8) : 0;
     L.total = off;
     return L;
+}
+
+static bool lds_eligible(const Engine* e, const Model& m, uint32_t* bytes) {
+    if (m.n_cells > 65535 || m.n_padded > (1u << 30)) return false;
+    const LdsLayout Ly =
+        lds_layout((uint32_t)m.n_padded, (uint32_t)m.n_cells, (uint32_t)m.cfg.types.size());
+    *bytes = Ly.total;
+    return Ly.total + 256 <= e->max_lds;  // room for the kernels' static LDS
+}
+
+// LDS bytes a launch reserves for replicas needing `bytes` (grouping key).
+static uint32_t lds_class(const Engine* e, uint32_t bytes) {
+    return std::min((bytes + 4095) / 4096 * 4096, e->max_lds - 256);
 }
 
 static void upload(Engine* e, Model* m) {
     if (m->dev) return;
     if (!m->initialized) throw Error(CPM_ERR_ARG, "model is not initialized");
-    const Layout L = layout_of(*m);
+    uint32_t lds_bytes = 0;
+    const bool lds = lds_eligible(e, *m, &lds_bytes);
+    const Layout L = layout_of(*m, lds);
     auto* d = new Device();
     hipError_t err = hipMalloc(&d->base, L.total);
     if (err != hipSuccess) {
@@ -446,6 +1259,8 @@ static void upload(Engine* e, Model* m) {
         throw Error(CPM_ERR_OOM, std::string("device allocation failed: ") + hipGetErrorString(err));
     }
     m->dev = d;
+    d->lds = lds;
+    d->lds_bytes = lds_bytes;
     Rep& R = d->rep;
     R.D = m->D;
     R.boundary = m->cfg.boundary;
@@ -456,6 +1271,7 @@ static void upload(Engine* e, Model* m) {
         R.strides[k] = k < m->D ? (uint32_t)m->strides[k] : 0;
     }
     R.n_sites = (uint32_t)m->n_sites;
+    R.n_padded = (uint32_t)m->n_padded;
     R.n_copy = m->n_copy;
     R.n_energy = m->n_energy;
     R.n_topo = m->n_topo;
@@ -471,6 +1287,7 @@ static void upload(Engine* e, Model* m) {
     R.contact = (uint32_t*)(d->base + L.contact);
     R.params = (const double*)(d->base + L.params);
     R.targets = (const uint32_t*)(d->base + L.targets);
+    R.draws = lds ? (uint64_t*)(d->base + L.draws) : nullptr;
     const size_t T = R.n_types;
     std::vector<double> params(2 * T + (T + 1) * (T + 1));
     std::vector<uint32_t> targets(2 * T);
@@ -508,9 +1325,7 @@ void device_release(Model* m) {
 
 void sync_to_host(Model* m) {
     if (!m->dev || !m->device_dirty) return;
-    const Layout L = layout_of(*m);
-    const Rep& R = m->dev->rep;
-    (void)R;
+    const Layout L = layout_of(*m, m->dev->lds);
     const size_t n = m->n_cells, D = m->D;
     auto get = [&](void* dst, size_t off, size_t bytes) {
         if (bytes) HIP_OK(hipMemcpy(dst, m->dev->base + off, bytes, hipMemcpyDeviceToHost));
@@ -524,15 +1339,16 @@ void sync_to_host(Model* m) {
     m->device_dirty = false;
 }
 
-void sync_scalars(Model*) {}  // scalars are written back after every launch
+void sync_scalars(Model*) {}  // scalars are written back after every call
 
-static void ensure_reps(Engine* e, size_t n) {
-    if (n <= e->reps_cap) return;
-    if (e->reps_dev) HIP_OK(hipFree(e->reps_dev));
-    e->reps_dev = nullptr;
-    size_t cap = std::max(n, (size_t)64);
-    HIP_OK(hipMalloc(&e->reps_dev, cap * sizeof(Rep)));
-    e->reps_cap = cap;
+template <class T>
+static void ensure_buf(T*& ptr, size_t& cap, size_t n) {
+    if (n <= cap) return;
+    if (ptr) HIP_OK(hipFree(ptr));
+    ptr = nullptr;
+    size_t c = std::max(n, (size_t)64);
+    HIP_OK(hipMalloc(&ptr, c * sizeof(T)));
+    cap = c;
 }
 
 static void add_counters(cpm_counters* c, const Rep& R) {
@@ -545,10 +1361,16 @@ static void add_counters(cpm_counters* c, const Rep& R) {
     c->exp_near_ties += R.cnt[5];
 }
 
+static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint64_t* todo,
+                           cpm_counters* counters);
+
 void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* todo, int mode,
                   cpm_counters* counters) {
-    if (mode != CPM_MODE_EXACT_REPLAY)
-        throw Error(CPM_ERR_ARG, "sublattice mode is not available in this build");
+    if (mode == CPM_MODE_SUBLATTICE) {
+        run_sublattice(e, models, n, todo, counters);
+        return;
+    }
+    if (mode != CPM_MODE_EXACT_REPLAY) throw Error(CPM_ERR_ARG, "unknown mode");
     std::lock_guard<std::mutex> lock(e->mu);
     HIP_OK(hipSetDevice(e->device));
     bool any3d = false;
@@ -557,24 +1379,111 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
         any3d |= models[i]->D == 3;
     }
     if (any3d) ensure_sp3d(e);
-    ensure_reps(e, n);
+    ensure_buf(e->reps_dev, e->reps_cap, n);
+    ensure_buf(e->idx_dev, e->idx_cap, n);
     std::vector<Rep> reps(n);
+    // partition: global-path replicas, and LDS classes keyed by (D, LDS bytes)
+    std::vector<uint32_t> order, global_idx;
+    std::map<std::pair<int, uint32_t>, std::vector<uint32_t>> classes;
     for (size_t i = 0; i < n; ++i) {
         Model* m = models[i];
         Rep R = m->dev->rep;
-        for (int k = 0; k < 4; ++k) R.rng[k] = m->rng.s[k];
+        for (int k = 0; k < 4; ++k) R.rng[k] = R.fill_state[k] = m->rng.s[k];
         R.energy = m->energy;
         R.mcs = m->mcs;
         R.todo = todo[i];
+        R.attempt_in_mcs = 0;
+        R.cursor = 0;
+        R.done = todo[i] == 0;
+        R.filled = 0;
+        R.error = 0;
+        for (int k = 0; k < 6; ++k) R.cnt[k] = 0;
         reps[i] = R;
+        if (!todo[i]) continue;
+        if (m->dev->lds) classes[{m->D, lds_class(e, m->dev->lds_bytes)}].push_back((uint32_t)i);
+        else global_idx.push_back((uint32_t)i);
     }
+    struct Launch {
+        int D;
+        uint32_t lds, first, count;
+    };
+    std::vector<Launch> launches;
+    for (auto& [key, v] : classes) {
+        launches.push_back({key.first, key.second, (uint32_t)order.size(), (uint32_t)v.size()});
+        order.insert(order.end(), v.begin(), v.end());
+    }
+    const uint32_t global_first = (uint32_t)order.size();
+    order.insert(order.end(), global_idx.begin(), global_idx.end());
     HIP_OK(hipMemcpyAsync(e->reps_dev, reps.data(), n * sizeof(Rep), hipMemcpyHostToDevice,
                           e->stream));
+    if (!order.empty())
+        HIP_OK(hipMemcpyAsync(e->idx_dev, order.data(), order.size() * 4, hipMemcpyHostToDevice,
+                              e->stream));
     HIP_OK(hipEventRecord(e->ev0, e->stream));
-    const uint32_t block = 64;
-    hipLaunchKernelGGL(k_exact_replay, dim3((uint32_t)((n + block - 1) / block)), dim3(block), 0,
-                       e->stream, e->reps_dev, (uint32_t)n, e->sp3d);
-    HIP_OK(hipGetLastError());
+    if (!global_idx.empty()) {
+        const uint32_t ng = (uint32_t)global_idx.size();
+        hipLaunchKernelGGL(k_exact_replay, dim3((ng + 63) / 64), dim3(64), 0, e->stream,
+                           e->reps_dev, e->idx_dev + global_first, ng, e->sp3d);
+        HIP_OK(hipGetLastError());
+    }
+    if (!launches.empty()) {
+        // rounds of (fill every unfinished stream, advance every unfinished
+        // replica); a replica is finished once its MCS target is reached
+        std::vector<uint32_t> done(n);
+        uint64_t max_draws = 0;
+        for (auto& l : launches)
+            for (uint32_t j = 0; j < l.count; ++j) {
+                const Model* m = models[order[l.first + j]];
+                max_draws = std::max<uint64_t>(max_draws, todo[order[l.first + j]] * m->n_sites * 8);
+            }
+        uint64_t rounds_guess = max_draws / kFill + 1;
+        // every round consumes at least kFill - kSegLen - 256 draws of each
+        // unfinished replica; bound the loop well above what any run needs
+        const uint64_t max_rounds = 64 + 8 * (max_draws / (kFill - kSegLen - 256) + 1);
+        uint64_t rounds = 0;
+        for (;;) {
+            if (rounds > max_rounds)
+                throw Error(CPM_ERR_DEVICE, "exact-replay rounds did not converge");
+            const uint64_t burst = std::max<uint64_t>(1, std::min<uint64_t>(rounds_guess, 8));
+            rounds += burst;
+            for (uint64_t b = 0; b < burst; ++b) {
+                const uint32_t threads = (uint32_t)n * kSegs;
+                hipLaunchKernelGGL(k_rng_fill, dim3((threads + 255) / 256), dim3(256), 0,
+                                   e->stream, e->reps_dev, (uint32_t)n, e->jt);
+                HIP_OK(hipGetLastError());
+                HIP_OK(hipEventRecord(e->fill_done, e->stream));
+                const int ns = (int)std::min<size_t>(launches.size(), Engine::kClassStreams);
+                for (int k = 0; k < ns; ++k)
+                    HIP_OK(hipStreamWaitEvent(e->cls_stream[k], e->fill_done, 0));
+                for (size_t li = 0; li < launches.size(); ++li) {
+                    const Launch& l = launches[li];
+                    hipStream_t st = e->cls_stream[li % Engine::kClassStreams];
+                    if (l.D == 2)
+                        hipLaunchKernelGGL(k_lds_engine<2>, dim3(l.count), dim3(64), l.lds, st,
+                                           e->reps_dev, e->idx_dev + l.first, e->sp3d);
+                    else
+                        hipLaunchKernelGGL(k_lds_engine<3>, dim3(l.count), dim3(64), l.lds, st,
+                                           e->reps_dev, e->idx_dev + l.first, e->sp3d);
+                    HIP_OK(hipGetLastError());
+                }
+                for (int k = 0; k < ns; ++k) {
+                    HIP_OK(hipEventRecord(e->cls_done[k], e->cls_stream[k]));
+                    HIP_OK(hipStreamWaitEvent(e->stream, e->cls_done[k], 0));
+                }
+            }
+            rounds_guess = rounds_guess > burst ? rounds_guess - burst : 1;
+            // gather done flags
+            bool all = true;
+            HIP_OK(hipMemcpyAsync(reps.data(), e->reps_dev, n * sizeof(Rep), hipMemcpyDeviceToHost,
+                                  e->stream));
+            HIP_OK(hipStreamSynchronize(e->stream));
+            for (size_t i = 0; i < n; ++i) all &= reps[i].done != 0;
+            if (all) break;
+        }
+        hipLaunchKernelGGL(k_rng_finalize, dim3(((uint32_t)n + 63) / 64), dim3(64), 0, e->stream,
+                           e->reps_dev, (uint32_t)n);
+        HIP_OK(hipGetLastError());
+    }
     HIP_OK(hipEventRecord(e->ev1, e->stream));
     HIP_OK(hipMemcpyAsync(reps.data(), e->reps_dev, n * sizeof(Rep), hipMemcpyDeviceToHost,
                           e->stream));
@@ -582,13 +1491,101 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
     float ms = 0.f;
     HIP_OK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
     if (counters) counters->kernel_ms += ms;
+    for (size_t i = 0; i < n; ++i)
+        if (reps[i].error)
+            throw Error(CPM_ERR_DEVICE, "an attempt needed more than 128 stream draws");
     for (size_t i = 0; i < n; ++i) {
         Model* m = models[i];
-        for (int k = 0; k < 4; ++k) m->rng.s[k] = reps[i].rng[k];
-        m->energy = reps[i].energy;
-        m->mcs = reps[i].mcs;
-        if (todo[i]) m->device_dirty = true;
+        if (todo[i]) {
+            for (int k = 0; k < 4; ++k) m->rng.s[k] = reps[i].rng[k];
+            m->energy = reps[i].energy;
+            m->mcs = reps[i].mcs;
+            m->device_dirty = true;
+        }
         add_counters(counters, reps[i]);
+    }
+}
+
+static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint64_t* todo,
+                           cpm_counters* counters) {
+    {
+        std::lock_guard<std::mutex> lock(e->mu);
+        HIP_OK(hipSetDevice(e->device));
+        bool any3d = false;
+        for (size_t i = 0; i < n; ++i) {
+            Model* m = models[i];
+            upload(e, m);
+            if (!m->dev->lds)
+                throw Error(CPM_ERR_ARG,
+                            "sublattice mode needs the lattice resident in LDS; this lattice "
+                            "is too large");
+            if (m->cfg.boundary == CPM_BOUNDARY_PERIODIC)
+                for (int d = 0; d < m->D; ++d)
+                    if (m->dims[d] % 2)
+                        throw Error(CPM_ERR_ARG,
+                                    "sublattice mode needs even side lengths on periodic axes");
+            any3d |= m->D == 3;
+        }
+        if (any3d) ensure_sp3d(e);
+        ensure_buf(e->reps_dev, e->reps_cap, n);
+        ensure_buf(e->idx_dev, e->idx_cap, n);
+        std::vector<Rep> reps(n);
+        std::map<std::pair<int, uint32_t>, std::vector<uint32_t>> classes;
+        for (size_t i = 0; i < n; ++i) {
+            Model* m = models[i];
+            Rep R = m->dev->rep;
+            R.mcs = m->mcs;
+            R.todo = todo[i];
+            R.done = todo[i] == 0;
+            for (int k = 0; k < 6; ++k) R.cnt[k] = 0;
+            if (todo[i]) {
+                // one draw of the replica's own stream keys this call's
+                // counter-based randomness
+                R.fill_state[0] = xo_next(m->rng);
+                classes[{m->D, lds_class(e, m->dev->lds_bytes)}].push_back((uint32_t)i);
+            }
+            reps[i] = R;
+        }
+        std::vector<uint32_t> order;
+        HIP_OK(hipMemcpyAsync(e->reps_dev, reps.data(), n * sizeof(Rep), hipMemcpyHostToDevice,
+                              e->stream));
+        for (auto& [key, v] : classes) order.insert(order.end(), v.begin(), v.end());
+        if (!order.empty())
+            HIP_OK(hipMemcpyAsync(e->idx_dev, order.data(), order.size() * 4,
+                                  hipMemcpyHostToDevice, e->stream));
+        HIP_OK(hipEventRecord(e->ev0, e->stream));
+        uint32_t first = 0;
+        for (auto& [key, v] : classes) {
+            const uint32_t cnt = (uint32_t)v.size();
+            if (key.first == 2)
+                hipLaunchKernelGGL(k_sublattice<2>, dim3(cnt), dim3(kSlThreads), key.second,
+                                   e->stream, e->reps_dev, e->idx_dev + first, e->sp3d);
+            else
+                hipLaunchKernelGGL(k_sublattice<3>, dim3(cnt), dim3(kSlThreads), key.second,
+                                   e->stream, e->reps_dev, e->idx_dev + first, e->sp3d);
+            HIP_OK(hipGetLastError());
+            first += cnt;
+        }
+        HIP_OK(hipEventRecord(e->ev1, e->stream));
+        HIP_OK(hipMemcpyAsync(reps.data(), e->reps_dev, n * sizeof(Rep), hipMemcpyDeviceToHost,
+                              e->stream));
+        HIP_OK(hipStreamSynchronize(e->stream));
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+        if (counters) counters->kernel_ms += ms;
+        for (size_t i = 0; i < n; ++i) {
+            if (!todo[i]) continue;
+            models[i]->mcs = reps[i].mcs;
+            models[i]->device_dirty = true;
+            add_counters(counters, reps[i]);
+        }
+    }
+    // the running energy is re-established from scratch: concurrent moves
+    // price dH against per-cell values other moves may be changing
+    for (size_t i = 0; i < n; ++i) {
+        if (!todo[i]) continue;
+        sync_to_host(models[i]);
+        models[i]->energy = models[i]->global_energy();
     }
 }
 

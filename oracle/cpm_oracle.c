@@ -928,6 +928,30 @@ int oc_init_explicit(ocpm* m, const uint32_t* lattice, uint64_t n_cells,
     return 0;
 }
 
+/* Places an arbitrary lattice (no connectivity or volume checks) and
+   recomputes every derived quantity: the brute-force checker's view of a
+   state produced elsewhere (checker.rs:36-136). */
+void oc_load_state(ocpm* m, const uint32_t* lattice, uint64_t n_cells,
+                   const uint64_t* cell_type_of) {
+    memset(m->cells, 0, sizeof(uint32_t) * m->n_padded);
+    free(m->type_index);
+    free(m->volume);
+    free(m->interface);
+    free(m->sum_pos);
+    free(m->sum_outer);
+    free(m->contact);
+    alloc_cells(m, (size_t)n_cells);
+    size_t idx = 0;
+    FOR_INTERIOR(m, c, {
+        uint32_t v = lattice[idx++];
+        m->cells[flat_of(m, c)] = v;
+        if (v) m->volume[v - 1] += 1;
+    });
+    if (m->boundary == OC_PERIODIC) refresh_halo_full(m);
+    for (size_t i = 0; i < n_cells; ++i) m->type_index[i] = (uint32_t)cell_type_of[i];
+    finish_init(m);
+}
+
 /* initialization.rs:308-327 (canonical_growable_neighbour) */
 static int growable(const ocpm* m, size_t flat, ptrdiff_t off, size_t* out) {
     size_t raw = (size_t)((ptrdiff_t)flat + off);

@@ -81,6 +81,9 @@ int oc_init_explicit(ocpm* m, const uint32_t* lattice, uint64_t n_cells,
                      const uint64_t* cell_type_of);
 /* initialization.rs:202-330 (scatter_and_grow) — returns 0 or nonzero */
 int oc_init_scatter(ocpm* m, const uint64_t* cell_counts);
+/* arbitrary lattice, no checks, derived state recomputed (checker view) */
+void oc_load_state(ocpm* m, const uint32_t* lattice, uint64_t n_cells,
+                   const uint64_t* cell_type_of);
 /* monte_carlo.rs:327-357 */
 void oc_run_mcs(ocpm* m, uint64_t n);
 /* energy.rs:561-565 */

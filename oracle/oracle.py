@@ -88,6 +88,7 @@ def lib():
     L.oc_set_halo_mode.argtypes = [C.c_void_p, C.c_int]
     L.oc_init_explicit.argtypes = [C.c_void_p, u32p, C.c_uint64, u64p]
     L.oc_init_scatter.argtypes = [C.c_void_p, u64p]
+    L.oc_load_state.argtypes = [C.c_void_p, u32p, C.c_uint64, u64p]
     L.oc_run_mcs.argtypes = [C.c_void_p, C.c_uint64]
     L.oc_global_energy.argtypes = [C.c_void_p]
     L.oc_global_energy.restype = C.c_double
@@ -353,6 +354,13 @@ class Model:
         if getattr(self, "h", None):
             lib().oc_model_free(self.h)
             self.h = None
+
+    def load_state(self, lattice, cell_type_of) -> None:
+        """Replace the state by an arbitrary lattice (no checks) and recompute
+        everything derived from it — the brute-force checker's view."""
+        lat = np.ascontiguousarray(np.asarray(lattice, dtype=np.uint32).ravel())
+        cto = np.ascontiguousarray(np.asarray(cell_type_of, dtype=np.uint64))
+        lib().oc_load_state(self.h, _ptr(lat, C.c_uint32), len(cto), _ptr(cto, C.c_uint64))
 
     # --- dynamics
     def run_mcs(self, n: int) -> None:

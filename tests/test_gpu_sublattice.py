@@ -1,0 +1,132 @@
+"""Sublattice-parallel mode: checkerboard colours updated in parallel, so the
+update order differs from the reference's serial loop.  It is validated
+statistically: per-replica summary statistics (volume, interface, shape,
+sorting, energy) over 256 seeds must be indistinguishable from the
+exact-replay engine's (itself bit-identical to the reference), by a
+two-sample Kolmogorov-Smirnov test at p > 0.01; and its bookkeeping must
+stay exactly consistent with a from-scratch recomputation."""
+import numpy as np
+import pytest
+
+import cpm
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SEEDS = 256
+
+
+def sorting_sim(seed, side=48, n_types=2):
+    """2D differential adhesion: medium row 16, type pairs from a fixed
+    draw, V*=30, I*=24, lambda_V=5, lambda_I=1, phi=0.7."""
+    J = [[0.0, 16.0, 16.0], [16.0, 3.0, 11.0], [16.0, 11.0, 5.0]]
+    sim = cpm.CPM(grid=(side, side), boundary="periodic", seed=seed)
+    n = round(0.7 * side * side / 30)
+    for t in range(n_types):
+        sim.register_cell_type(f"t{t}", target_volume=30, target_interface=24,
+                               lambda_volume=5.0, lambda_interface=1.0)
+        sim.add_cells(f"t{t}", n // n_types + (1 if t < n % n_types else 0))
+    sim.set_adhesion(J)
+    sim.initialize(warn_on_default_init=False)
+    return sim
+
+
+def homotypic_fraction(contact, type_index):
+    """notebooks/2d_organoid_demo.ipynb:175-188"""
+    n = contact.shape[0]
+    c = contact[1:, 1:].astype(np.int64)
+    t = np.asarray(type_index)
+    same = (t[:, None] == t[None, :])
+    mask = np.triu(np.ones((n - 1, n - 1), dtype=bool), 1)
+    s = int((c * (same & mask)).sum())
+    d = int((c * (~same & mask)).sum())
+    return s / (s + d) if s + d else float("nan")
+
+
+def stats(res):
+    s = res.samples[-1]
+    return {
+        "mean_volume_dev": float(np.mean(np.abs(s.volume.astype(float) - 30.0))),
+        "mean_interface": float(np.mean(s.interface)),
+        "mean_rg2": float(np.mean(s.radius_of_gyration_sq)),
+        "homotypic": homotypic_fraction(s.contact, res.cell_type_index),
+        "energy_per_cell": s.conservative_energy / len(s.ids),
+    }
+
+
+def run_ensemble(mode, mcs):
+    sims = [sorting_sim(cpm.derive_seed_u64(2026, i)) for i in range(SEEDS)]
+    res = cpm.run_batch_configs(sims, burn_in_mcs=0, readout_mcs=mcs, sampling_interval_mcs=mcs,
+                                mode=mode)
+    return [stats(r) for r in res]
+
+
+def test_sublattice_summary_statistics_match_exact_replay_ks():
+    from scipy.stats import ks_2samp
+
+    mcs = 400
+    exact = run_ensemble("exact_replay", mcs)
+    sub = run_ensemble("sublattice", mcs)
+    report = {}
+    for key in exact[0]:
+        a = np.array([x[key] for x in exact])
+        b = np.array([x[key] for x in sub])
+        report[key] = (float(ks_2samp(a, b).pvalue), float(a.mean()), float(b.mean()))
+    print("KS", report, flush=True)
+    for key, (p, ma, mb) in report.items():
+        assert p > 0.01, f"{key}: KS p={p:.4g} exact mean {ma:.4f} sublattice mean {mb:.4f}"
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_sublattice_bookkeeping_is_consistent(dim):
+    """checker.rs:36-136 on device snapshots: after parallel updates, V, I,
+    contact and the position sums (mod the periodic box) equal a
+    from-scratch recomputation; the energy is recomputed exactly; cells stay
+    non-empty."""
+    if dim == 2:
+        sim = sorting_sim(77, side=40)
+        grid = (40, 40)
+    else:
+        grid = (16, 16, 16)
+        sim = cpm.CPM(grid=grid, boundary="periodic", seed=5)
+        sim.register_cell_type("a", 60, 100, 3.0, 0.5)
+        sim.add_cells("a", 8)
+        sim.set_adhesion([[0.0, 8.0], [8.0, 3.0]])
+        sim.initialize(warn_on_default_init=False)
+    rep = cpm.Replica(sim)
+    cnt = rep.run_mcs(50, mode="sublattice")
+    assert cnt["attempts"] == 50 * int(np.prod(grid))
+    assert cnt["accepted"] > 0
+    snap = rep.snapshot()
+    lat = snap["lattice"]
+    n = len(snap["volume"])
+    vol = np.bincount(lat.ravel(), minlength=n + 1)[1:]
+    assert np.array_equal(vol, snap["volume"])
+    assert snap["volume"].min() >= 1
+    # recompute interface / contact with the oracle's definitions
+    cfg = O.Config(grid=grid, boundary="periodic", seed=0,
+                   cell_types=[O.CellTypeSpec(f"t{i}", 30, 24, 5.0, 1.0) for i in range(2)]
+                   if dim == 2 else [O.CellTypeSpec("a", 60, 100, 3.0, 0.5)],
+                   cell_counts=[1, 1] if dim == 2 else [1],
+                   adhesion=[[0.0, 16.0, 16.0], [16.0, 3.0, 11.0], [16.0, 11.0, 5.0]]
+                   if dim == 2 else [[0.0, 8.0], [8.0, 3.0]])
+    m = O.Model(cfg)
+    m.load_state(lat, snap["type_index"])
+    itf, sp, so, ct = m.recompute()
+    assert np.array_equal(itf, snap["interface"])
+    assert np.array_equal(ct, snap["contact"])
+    L = np.array(grid)
+    assert np.all((sp - snap["sum_position"]) % L == 0)
+    assert snap["energy"] == m.global_energy()
+    assert np.isclose(rep.global_energy(), snap["energy"], rtol=0, atol=0)
+
+
+def test_sublattice_is_reproducible_and_seed_sensitive():
+    a = cpm.Replica(sorting_sim(5, side=32))
+    b = cpm.Replica(sorting_sim(5, side=32))
+    c = cpm.Replica(sorting_sim(6, side=32))
+    for r in (a, b, c):
+        r.run_mcs(30, mode="sublattice")
+    sa, sb, sc = a.snapshot(), b.snapshot(), c.snapshot()
+    assert np.array_equal(sa["lattice"], sb["lattice"]) and sa["energy"] == sb["energy"]
+    assert not np.array_equal(sa["lattice"], sc["lattice"])
