@@ -184,8 +184,8 @@ def main():
         "vs_baseline": None,
         "dtype": "int32+f64",
         "data": "synthetic",
-        "config": {"workload": "run_batch: 4096 theta-replicas, 2D periodic sides "
-                               "{48,64,96,128}, 2 types, phi=0.7, 500 MCS per step",
+        "config": {"workload": f"run_batch: {len(reps)} theta-replicas, 2D periodic sides "
+                               f"{{48,64,96,128}}, 2 types, phi=0.7, {args.interval} MCS per step",
                    "replicas": len(reps), "interval_mcs": args.interval,
                    "parallelism": "1 GPU, replicas batched in one launch"},
         "plain_attempts_per_s": plain_rate,

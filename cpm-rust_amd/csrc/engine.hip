@@ -416,17 +416,79 @@ __device__ __forceinline__ uint32_t bperm(uint32_t v, uint32_t src_lane) {
     return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src_lane << 2), (int)v);
 }
 
-// Exact replay on the LDS path: one wave per replica.  Every lane decodes
+__device__ __forceinline__ uint64_t uni64(uint64_t v) {
+    return ((uint64_t)uni((uint32_t)(v >> 32)) << 32) | uni((uint32_t)v);
+}
+
+// LDS words of the exact-replay engine: cell id in the low 12 bits, the
+// cell's adhesion-matrix index (type + 1; 0 for medium) in the high 4, so
+// one read yields both.
+constexpr uint32_t kIdMask = 0xFFF;
+constexpr uint32_t kMaxPackedCells = 4095;
+
+// Index, in the full-shell (topology) stencil order, of entry q of the
+// energy stencil with NE entries.  Full shell: code 0..3^D-1 with
+// component 0 fastest, origin skipped; von Neumann: axis 0 +1, axis 0 -1,
+// axis 1 +1, ...; eighteen: the full shell without its corners
+// (crates/cpm-core/src/neighborhood.rs:45-115).
+__host__ __device__ constexpr int shell_index(int D, int NE, int q) {
+    const int total = D == 2 ? 9 : 27, origin = total / 2;
+    const bool vn = NE == 2 * D;
+    if (vn) {
+        const int axis = q / 2, sgn = q % 2 == 0 ? 1 : -1;
+        int code = 0, pw = 1;
+        for (int d = 0; d < D; ++d) {
+            code += ((d == axis ? sgn : 0) + 1) * pw;
+            pw *= 3;
+        }
+        return code > origin ? code - 1 : code;
+    }
+    int n = 0;
+    for (int code = 0; code < total; ++code) {
+        if (code == origin) continue;
+        int c = code, nz = 0;
+        for (int d = 0; d < D; ++d) {
+            nz += (c % 3) != 1;
+            c /= 3;
+        }
+        if (NE == 18 && nz > 2) continue;
+        if (n == q) return code > origin ? code - 1 : code;
+        ++n;
+    }
+    return -1;
+}
+
+// Per-lane speculative evaluation of the attempt decoded at this lane's
+// window position, on the current lattice (dynamics.rs:22-69 and
+// energy.rs:521-523): early-out, simple-point and volume checks, the local
+// dH in FP64 in the reference's order, and the Metropolis test with the
+// u draw taken from the window when dH > 0.
+struct LaneEval {
+    uint32_t next;  // window position after the attempt
+    uint32_t Lw, Gw;
+    int32_t dl, dg;
+    uint32_t vL, vG, iL, iG;
+    double dh;
+};
+
+struct EvalMasks {
+    uint64_t ok, null_, acc, top, en, tie;
+};
+
+// Exact replay on the LDS path: one wave per replica.  Each lane decodes
 // the attempt that would start at its own position of the 64-draw window
 // (the first accepted draw of each consumer, found in the 128-draw window
-// by mask search) and reads sigma at its target and source; the wave then
-// walks the actual chain of attempts through those decodes: a null attempt
-// costs a bit test and a lane read.  Non-null attempts are resolved one at
-// a time, in order, with the neighbourhood spread over lanes.
-template <int D>
+// by mask search) and evaluates it completely against the current lattice.
+// The wave then walks the actual chain of attempts through those
+// evaluations (a lane read per attempt) up to the first accepted copy,
+// commits it, re-evaluates, and goes on.  Accepted copies are rare, so an
+// attempt costs about 1/20 of a lane-parallel evaluation.  NE is the
+// energy stencil size; NT = 3^D - 1 the full shell.
+template <int D, int NE>
 __global__ void __launch_bounds__(64) k_lds_engine(Rep* __restrict__ reps,
                                                    const uint32_t* __restrict__ idx,
                                                    const uint64_t* __restrict__ sp3d) {
+    constexpr int NT = D == 2 ? 8 : 26;
     extern __shared__ __align__(16) unsigned char smem[];
     Rep& R = reps[idx[blockIdx.x]];
     if (R.done) return;
@@ -435,8 +497,7 @@ __global__ void __launch_bounds__(64) k_lds_engine(Rep* __restrict__ reps,
     const uint32_t n_padded = R.n_padded, n_sites = R.n_sites;
     const LdsLayout Ly = lds_layout(n_padded, n_cells, T);
     uint16_t* sig = (uint16_t*)(smem + Ly.sigma);
-    uint8_t* typ = smem + Ly.type;  // typ[id], typ[0] unused
-    uint32_t* vol = (uint32_t*)(smem + Ly.vol);
+    uint32_t* vol = (uint32_t*)(smem + Ly.vol);  // vol[id - 1]
     uint32_t* itf = (uint32_t*)(smem + Ly.itf);
     double* J = (double*)(smem + Ly.J);
     double* lv = (double*)(smem + Ly.lv);
@@ -445,9 +506,12 @@ __global__ void __launch_bounds__(64) k_lds_engine(Rep* __restrict__ reps,
     uint32_t* ti = (uint32_t*)(smem + Ly.ti);
 
     // ---- stage the replica into LDS
-    for (uint32_t i = lane; i < n_padded; i += 64) sig[i] = (uint16_t)R.sigma[i];
+    const uint32_t* __restrict__ gtyp = R.type_index;
+    for (uint32_t i = lane; i < n_padded; i += 64) {
+        const uint32_t id = R.sigma[i];
+        sig[i] = (uint16_t)(id ? id | ((gtyp[id - 1] + 1) << 12) : 0u);
+    }
     for (uint32_t i = lane; i < n_cells; i += 64) {
-        typ[i + 1] = (uint8_t)R.type_index[i];
         vol[i] = R.volume[i];
         itf[i] = R.interface[i];
     }
@@ -460,10 +524,9 @@ __global__ void __launch_bounds__(64) k_lds_engine(Rep* __restrict__ reps,
     }
     __syncthreads();
 
-    // ---- per-lane stencil offsets, uniform geometry
-    const uint32_t n_topo = R.n_topo, n_energy = R.n_energy, n_copy = R.n_copy;
-    const int32_t toff = lane < n_topo ? R.topo_off[lane] : 0;
-    const int32_t eoff = lane < n_energy ? R.energy_off[lane] : 0;
+    // ---- uniform geometry
+    const uint32_t n_copy = R.n_copy;
+    const int32_t eoff = lane < NE ? R.energy_off[lane] : 0;  // for the contact updates
     const uint32_t coff = lane < n_copy ? (uint32_t)R.copy_off[lane] : 0u;
     uint32_t range[D + 1], stride[D], dims[D];
     uint64_t zone[D + 1];
@@ -479,20 +542,23 @@ __global__ void __launch_bounds__(64) k_lds_engine(Rep* __restrict__ reps,
     uint32_t t_safe = 0;  // an interior site, for lanes whose decode is void
 #pragma unroll
     for (int d = 0; d < D; ++d) t_safe += stride[d];
-    const uint64_t sp2[4] = {(uint64_t)c_sp2d[0] | (uint64_t)c_sp2d[1] << 32,
-                             (uint64_t)c_sp2d[2] | (uint64_t)c_sp2d[3] << 32,
-                             (uint64_t)c_sp2d[4] | (uint64_t)c_sp2d[5] << 32,
-                             (uint64_t)c_sp2d[6] | (uint64_t)c_sp2d[7] << 32};
+    const uint64_t sp2a = (uint64_t)c_sp2d[0] | (uint64_t)c_sp2d[1] << 32;
+    const uint64_t sp2b = (uint64_t)c_sp2d[2] | (uint64_t)c_sp2d[3] << 32;
+    const uint64_t sp2c = (uint64_t)c_sp2d[4] | (uint64_t)c_sp2d[5] << 32;
+    const uint64_t sp2d = (uint64_t)c_sp2d[6] | (uint64_t)c_sp2d[7] << 32;
     const bool periodic = R.boundary == CPM_BOUNDARY_PERIODIC;
     const uint64_t ncell1 = (uint64_t)n_cells + 1;
-    const uint64_t* __restrict__ draws = R.draws;
+    // global address space: a flat load would also count against the LDS
+    // wait counter, so every LDS wait would stall on the draw prefetch
+    const __attribute__((address_space(1))) uint64_t* __restrict__ draws =
+        (const __attribute__((address_space(1))) uint64_t*)R.draws;
 
     uint64_t todo = R.todo;
     uint32_t ain = R.attempt_in_mcs;
     uint64_t mcs = R.mcs;
     double energy = R.energy;
     uint32_t c_att = 0, c_nn = 0, c_top = 0, c_en = 0, c_acc = 0, c_tie = 0;
-    bool dirty = false, finished = false, error = false;
+    bool dirty = false, error = false;
 
     // ---- the 128-draw register window: halves [base, base+64), [base+64, base+128)
     uint32_t base = R.cursor, p = 0;
@@ -501,51 +567,280 @@ __global__ void __launch_bounds__(64) k_lds_engine(Rep* __restrict__ reps,
     bool h1_valid = false;
     uint64_t w_next = 0;
     // per-lane decode of the attempt starting at this lane's window position
-    uint32_t dec_t = 0, dec_s = 0, dec_end = 0, dec_L = 0, dec_G = 0, dec_c[D];
-    uint64_t VALID = 0, NULLM = 0;
+    uint32_t dec_t = 0, dec_s = 0, dec_end = 0, dec_c[D];
+    bool dec_ok = false;
+    LaneEval ev;
+    EvalMasks M;
+    bool need_decode = true, need_eval = true;
 
-    auto load_window = [&]() {
-        h1_valid = base + 128 <= kFill;
-        prep_half<D>(h0, draws[base + lane], range, zone, m0);
-        prep_half<D>(h1, h1_valid ? draws[base + 64 + lane] : 0ull, range, zone, m1);
-        w_next = base + 192 <= kFill ? draws[base + 128 + lane] : 0ull;
-    };
-    auto reread = [&]() {
-        dec_L = sig[dec_t];
-        dec_G = sig[dec_s];
-        NULLM = __ballot(((VALID >> lane) & 1) && dec_L == dec_G);
-    };
-    auto decode = [&]() {
-        const uint64_t hv = h1_valid ? ~0ull : 0ull;
-        uint32_t pos = lane, t = 0, kv = 0;
-        bool valid = true;
-#pragma unroll
-        for (int c = 0; c <= D; ++c) {
-            const uint32_t q = lane_next_bit(m0[c], m1[c] & hv, pos);
-            valid = valid && q < 128;
-            const uint32_t v0 = bperm(h0.hi[c], q & 63), v1 = bperm(h1.hi[c], q & 63);
-            const uint32_t v = q < 64 ? v0 : v1;
-            if (c < D) {
-                dec_c[c] = v;
-                t += (v + 1) * stride[c];
-            } else {
-                kv = v;
-            }
-            pos = q + 1;
-        }
-        const uint32_t off = bperm(coff, kv);
-        dec_t = valid ? t : t_safe;
-        dec_s = valid ? (uint32_t)((int32_t)t + (int32_t)off) : t_safe;
-        dec_end = pos;
-        VALID = __ballot(valid);
-        reread();
-    };
+#define CPM_DECODE()                                                                         \
+    do {                                                                                     \
+        const uint64_t hv = h1_valid ? ~0ull : 0ull;                                         \
+        uint32_t pos = lane, t = 0, kv = 0;                                                  \
+        bool valid = true;                                                                   \
+        _Pragma("unroll") for (int c = 0; c <= D; ++c) {                                     \
+            const uint32_t q = lane_next_bit(m0[c], m1[c] & hv, pos);                        \
+            valid = valid && q < 128;                                                        \
+            const uint32_t v0 = bperm(h0.hi[c], q & 63), v1 = bperm(h1.hi[c], q & 63);       \
+            const uint32_t v = q < 64 ? v0 : v1;                                             \
+            if (c < D) {                                                                     \
+                dec_c[c] = v;                                                                \
+                t += (v + 1) * stride[c];                                                    \
+            } else {                                                                         \
+                kv = v;                                                                      \
+            }                                                                                \
+            pos = q + 1;                                                                     \
+        }                                                                                    \
+        const uint32_t off = bperm(coff, kv);                                                \
+        dec_t = valid ? t : t_safe;                                                          \
+        dec_s = valid ? (uint32_t)((int32_t)t + (int32_t)off) : t_safe;                      \
+        dec_end = pos;                                                                       \
+        dec_ok = valid;                                                                      \
+    } while (0)
 
-    if (base + 64 > kFill) goto exit_round;  // no full window left in this fill
-    load_window();
-    decode();
+#define CPM_EVALUATE()                                                                       \
+    do {                                                                                     \
+        const uint32_t t = dec_t, e = dec_end;                                               \
+        ev.Lw = sig[t];                                                                      \
+        ev.Gw = sig[dec_s];                                                                  \
+        const uint32_t L = ev.Lw & kIdMask, G = ev.Gw & kIdMask;                             \
+        const uint32_t lidx = ev.Lw >> 12, gidx = ev.Gw >> 12;                               \
+        const bool nn = dec_ok && ev.Lw != ev.Gw;                                            \
+        uint32_t nb[NT];                                                                     \
+        _Pragma("unroll") for (int m = 0; m < NT; ++m) {                                     \
+            const int code = m < NT / 2 ? m : m + 1;                                         \
+            const int o0 = code % 3 - 1, o1 = (code / 3) % 3 - 1, o2 = code / 9 - 1;         \
+            const int32_t a = D == 2 ? o0 * (int32_t)stride[0] + o1                          \
+                                     : o0 * (int32_t)stride[0] + o1 * (int32_t)stride[1 % D] + o2; \
+            nb[m] = sig[(int32_t)t + a];                                                     \
+        }                                                                                    \
+        ev.vL = vol[L ? L - 1 : 0];                                                          \
+        ev.vG = vol[G ? G - 1 : 0];                                                          \
+        ev.iL = itf[L ? L - 1 : 0];                                                          \
+        ev.iG = itf[G ? G - 1 : 0];                                                          \
+        uint32_t pat = 0;                                                                    \
+        _Pragma("unroll") for (int m = 0; m < NT; ++m)                                       \
+            pat |= (uint32_t)((nb[m] & kIdMask) == L) << m;                                  \
+        const bool top = nn && L != 0;                                                       \
+        bool simple;                                                                         \
+        if (D == 2) {                                                                        \
+            const uint64_t w = pat < 128 ? (pat < 64 ? sp2a : sp2b)                          \
+                                         : (pat < 192 ? sp2c : sp2d);                        \
+            simple = (w >> (pat & 63)) & 1ull;                                               \
+        } else {                                                                             \
+            simple = top ? ((sp3d[pat >> 6] >> (pat & 63)) & 1ull) : false;                  \
+        }                                                                                    \
+        const bool go = nn && (L == 0 || (simple && ev.vL > 1));                             \
+        double dadh = 0.0; /* summed in stencil order */                                     \
+        int cl = 0, cg = 0;                                                                  \
+        _Pragma("unroll") for (int q = 0; q < NE; ++q) {                                     \
+            const uint32_t w = nb[shell_index(D, NE, q)];                                    \
+            const uint32_t nbe = w & kIdMask, ni = w >> 12;                                  \
+            const double jl = J[lidx * T1 + ni], jg = J[gidx * T1 + ni];                     \
+            dadh += 1.0 * ((nbe != G ? jg : 0.0) - (nbe != L ? jl : 0.0));                   \
+            cl += nbe == L;                                                                  \
+            cg += nbe == G;                                                                  \
+        }                                                                                    \
+        ev.dl = 2 * cl - NE;                                                                 \
+        ev.dg = NE - 2 * cg;                                                                 \
+        const uint32_t tl = lidx ? lidx - 1 : 0u, tg = gidx ? gidx - 1 : 0u;                 \
+        double dvol = 0.0, ditf = 0.0;                                                       \
+        if (L) {                                                                             \
+            const double v = (double)ev.vL, T0 = (double)tv[tl];                             \
+            const double x1 = (v - 1.0) - T0, x0 = v - T0;                                   \
+            dvol += lv[tl] * (x1 * x1) - lv[tl] * (x0 * x0);                                 \
+            const double i0 = (double)ev.iL, I0 = (double)ti[tl];                            \
+            const double y1 = (i0 + (double)ev.dl) - I0, y0 = i0 - I0;                       \
+            ditf += li[tl] * (y1 * y1) - li[tl] * (y0 * y0);                                 \
+        }                                                                                    \
+        if (G) {                                                                             \
+            const double v = (double)ev.vG, T0 = (double)tv[tg];                             \
+            const double x1 = (v + 1.0) - T0, x0 = v - T0;                                   \
+            dvol += lv[tg] * (x1 * x1) - lv[tg] * (x0 * x0);                                 \
+            const double i0 = (double)ev.iG, I0 = (double)ti[tg];                            \
+            const double y1 = (i0 + (double)ev.dg) - I0, y0 = i0 - I0;                       \
+            ditf += li[tg] * (y1 * y1) - li[tg] * (y0 * y0);                                 \
+        }                                                                                    \
+        ev.dh = dvol + ditf + dadh;                                                          \
+        const bool needu = go && !(ev.dh <= 0.0);                                            \
+        const bool euok = !needu || e < (h1_valid ? 128u : 64u);                             \
+        /* the u draw, rng.gen::<f64>(), at window position e */                             \
+        const uint32_t ue = min(e, 127u), ul = ue & 63;                                      \
+        const uint32_t lo0 = bperm((uint32_t)h0.w, ul), hi0 = bperm((uint32_t)(h0.w >> 32), ul); \
+        const uint32_t lo1 = bperm((uint32_t)h1.w, ul), hi1 = bperm((uint32_t)(h1.w >> 32), ul); \
+        const uint64_t uw = ue < 64 ? ((uint64_t)hi0 << 32 | lo0) : ((uint64_t)hi1 << 32 | lo1); \
+        bool acc = go, tie = false;                                                          \
+        if (needu) {                                                                         \
+            const double u = (double)(uw >> 11) * (1.0 / 9007199254740992.0);                \
+            const double pr = exp(-ev.dh);                                                   \
+            acc = u < pr;                                                                    \
+            tie = fabs(u - pr) <= pr * 0x1p-50;                                              \
+        }                                                                                    \
+        ev.next = e + (needu ? 1u : 0u);                                                     \
+        const bool ok = dec_ok && euok;                                                      \
+        M.ok = uni64(__ballot(ok));                                                          \
+        M.null_ = uni64(__ballot(dec_ok && !nn));                                            \
+        M.acc = uni64(__ballot(ok && acc));                                                  \
+        M.top = uni64(__ballot(top));                                                        \
+        M.en = uni64(__ballot(go));                                                          \
+        M.tie = uni64(__ballot(ok && tie));                                                  \
+    } while (0)
 
+    if (base + 64 > kFill) goto exit_loop;  // no full window left in this fill
+    h1_valid = base + 128 <= kFill;
+    prep_half<D>(h0, draws[base + lane], range, zone, m0);
+    prep_half<D>(h1, h1_valid ? draws[base + 64 + lane] : 0ull, range, zone, m1);
+    w_next = base + 192 <= kFill ? draws[base + 128 + lane] : 0ull;
     while (todo > 0) {
+        if (need_decode) {
+            CPM_DECODE();
+            need_decode = false;
+            need_eval = true;
+        }
+        if (need_eval) {
+            CPM_EVALUATE();
+            need_eval = false;
+        }
+        // Walk the chain through the evaluated attempts up to the first
+        // accepted copy, the window's end or the MCS target, in one step:
+        // lane k finds the k-th attempt position x = next^k(p) by pointer
+        // doubling.  An attempt takes at least D + 1 >= 3 draws, so at most
+        // 22 attempts start in a 64-draw window and 5 doublings suffice.
+        uint32_t jb = ((M.ok >> lane) & 1) ? ev.next : 127u;  // next^(2^b), >= 64 = leaves
+        uint32_t x = p;
+#pragma unroll
+        for (int b = 0; b < 5; ++b) {
+            const uint32_t y = bperm(jb, x & 63);
+            if (((lane >> b) & 1) && x < 64) x = y;
+            if (b < 4) {
+                const uint32_t z = bperm(jb, jb & 63);
+                if (jb < 64) jb = z;
+            }
+        }
+        const uint32_t xs = x & 63;
+        const bool vk = lane < 32 && x < 64 && ((M.ok >> xs) & 1);
+        const uint32_t k_inv = (uint32_t)__builtin_ctzll(uni64(__ballot(!vk)));
+        const uint64_t acc_k = uni64(__ballot(vk && ((M.acc >> xs) & 1)));
+        const uint32_t k_acc = acc_k ? (uint32_t)__builtin_ctzll(acc_k) : 64u;
+        // attempts left before the MCS target: (todo - 1) * n_sites + (n_sites - ain)
+        const uint64_t left = (todo - 1) * (uint64_t)n_sites + (n_sites - ain);
+        uint32_t n_att = min(k_inv, left < 64 ? (uint32_t)left : 64u);
+        const bool hit = k_acc < n_att;
+        if (hit) n_att = k_acc + 1;
+        {
+            const bool w = lane < n_att;
+            c_att += n_att;
+            c_nn += __popcll(uni64(__ballot(w && !((M.null_ >> xs) & 1))));
+            c_top += __popcll(uni64(__ballot(w && ((M.top >> xs) & 1))));
+            c_en += __popcll(uni64(__ballot(w && ((M.en >> xs) & 1))));
+            c_acc += hit ? 1u : 0u;
+            c_tie += __popcll(uni64(__ballot(w && ((M.tie >> xs) & 1))));
+        }
+        // monte_carlo.rs:330-356 — MCS accounting
+        ain += n_att;
+        while (ain >= n_sites) {
+            ain -= n_sites;
+            ++mcs;
+            --todo;
+        }
+        const uint32_t np = rl32(x, n_att);  // the position after the walked attempts
+        if (hit) p = rl32(x, k_acc);
+        if (hit) {
+            dirty = true;
+            const uint32_t t = rl32(dec_t, p);
+            const uint32_t Lw = rl32(ev.Lw, p), Gw = rl32(ev.Gw, p);
+            const uint32_t L = Lw & kIdMask, G = Gw & kIdMask;
+            const int32_t dl_i = (int32_t)rl32((uint32_t)ev.dl, p);
+            const int32_t dg_i = (int32_t)rl32((uint32_t)ev.dg, p);
+            const uint32_t vL = rl32(ev.vL, p), vG = rl32(ev.vG, p);
+            const uint32_t iL = rl32(ev.iL, p), iG = rl32(ev.iG, p);
+            const double dh = rlf64(ev.dh, p);
+            uint32_t cval[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) cval[d] = rl32(dec_c[d], p);
+            // dynamics.rs:73-98 — contact counts, pre-move lattice
+            if (lane < NE) {
+                const uint32_t nbe = sig[(int32_t)t + eoff] & kIdMask;
+                if (nbe != L)
+                    atomicSub(&R.contact[(uint64_t)min(L, nbe) * ncell1 + max(L, nbe)], 1u);
+                if (nbe != G)
+                    atomicAdd(&R.contact[(uint64_t)min(G, nbe) * ncell1 + max(G, nbe)], 1u);
+            }
+            if (lane == 0) {
+                sig[t] = (uint16_t)Gw;
+                if (L) {
+                    vol[L - 1] = vL - 1;
+                    itf[L - 1] = (uint32_t)((int32_t)iL + dl_i);
+                }
+                if (G) {
+                    vol[G - 1] = vG + 1;
+                    itf[G - 1] = (uint32_t)((int32_t)iG + dg_i);
+                }
+                // dynamics.rs:125-183 — unwrapped position sums
+                if (L) {
+                    int64_t* sp = R.sum_pos + (uint64_t)(L - 1) * D;
+                    int64_t* so = R.sum_outer + (uint64_t)(L - 1) * D * D;
+                    for (int d = 0; d < D; ++d) sp[d] -= (int64_t)cval[d];
+                    for (int d = 0; d < D; ++d)
+                        for (int e2 = 0; e2 < D; ++e2)
+                            so[d * D + e2] -= (int64_t)cval[d] * (int64_t)cval[e2];
+                }
+                if (G) {
+                    int64_t* sp = R.sum_pos + (uint64_t)(G - 1) * D;
+                    int64_t* so = R.sum_outer + (uint64_t)(G - 1) * D * D;
+                    const uint32_t v = vG + 1;
+                    const uint32_t vbg = max(v > 0 ? v - 1 : 0u, 1u);
+                    double cen[D];
+                    for (int d = 0; d < D; ++d) cen[d] = (double)sp[d] / (double)vbg;
+                    int64_t img[D];
+                    for (int d = 0; d < D; ++d) {
+                        int64_t best = (int64_t)cval[d];
+                        if (periodic) {
+                            double bd = fabs((double)best - cen[d]);
+                            const int64_t lo = (int64_t)cval[d] - (int64_t)dims[d];
+                            const int64_t hi = (int64_t)cval[d] + (int64_t)dims[d];
+                            const double dlo = fabs((double)lo - cen[d]);
+                            if (dlo < bd) {
+                                best = lo;
+                                bd = dlo;
+                            }
+                            if (fabs((double)hi - cen[d]) < bd) best = hi;
+                        }
+                        img[d] = best;
+                        sp[d] += best;
+                    }
+                    for (int d = 0; d < D; ++d)
+                        for (int e2 = 0; e2 < D; ++e2) so[d * D + e2] += img[d] * img[e2];
+                }
+                // dynamics.rs:206-224 — rewrite t's halo mirrors
+                if (periodic) {
+                    uint32_t opt[D][3], nopt[D], total = 1;
+                    for (int d = 0; d < D; ++d) {
+                        nopt[d] = 1;
+                        opt[d][0] = cval[d] + 1;
+                        if (cval[d] == 0) opt[d][nopt[d]++] = dims[d] + 1;
+                        if (cval[d] == dims[d] - 1) opt[d][nopt[d]++] = 0;
+                        total *= nopt[d];
+                    }
+                    for (uint32_t q = 1; q < total; ++q) {
+                        uint32_t r = q, f = 0;
+                        for (int d = D - 1; d >= 0; --d) {
+                            f += opt[d][r % nopt[d]] * stride[d];
+                            r /= nopt[d];
+                        }
+                        sig[f] = (uint16_t)Gw;
+                    }
+                }
+            }
+            __syncthreads();
+            energy += dh;
+            p = np;
+            // the lattice changed: every later attempt is evaluated again
+            need_eval = true;
+            continue;
+        }
+        p = np;
+        if (todo == 0) break;
         if (p >= 64) {  // slide the window by one half
             if (!h1_valid) break;
             base += 64;
@@ -556,206 +851,34 @@ __global__ void __launch_bounds__(64) k_lds_engine(Rep* __restrict__ reps,
             h1_valid = base + 128 <= kFill;
             prep_half<D>(h1, h1_valid ? w_next : 0ull, range, zone, m1);
             w_next = base + 192 <= kFill ? draws[base + 128 + lane] : 0ull;
-            decode();
+            need_decode = true;
             continue;
         }
-        if (!((VALID >> p) & 1)) {
-            // The attempt at p does not fit in the window: restart the
-            // window at p, or stop the round (the next round's window
-            // begins here).  An attempt needing more than 128 draws even
-            // from a window of its own is reported, never looped on.
-            if (p == 0 && h1_valid) {
-                error = true;
-                break;
-            }
-            if (base + p + 128 > kFill) break;
-            base += p;
-            p = 0;
-            load_window();
-            decode();
-            continue;
+        // The attempt at p does not fit in the window: restart the window
+        // at p, or stop the round (the next round's window begins here).
+        // An attempt needing more than 128 draws even from a window of its
+        // own is reported, never looped on.
+        if (p == 0 && h1_valid) {
+            error = true;
+            break;
         }
-        ++c_att;
-        if ((NULLM >> p) & 1) {
-            p = rl32(dec_end, p);  // early-out: sigma_t == sigma_s
-        } else {
-            ++c_nn;
-            const uint32_t t = rl32(dec_t, p), s = rl32(dec_s, p);
-            const uint32_t L = rl32(dec_L, p), G = rl32(dec_G, p);
-            const uint32_t e = rl32(dec_end, p);
-            (void)s;
-            // neighbourhood reads for topology and energy, issued together
-            const uint32_t nbt = sig[(int32_t)t + toff];
-            const uint32_t nbe = sig[(int32_t)t + eoff];
-            const uint32_t vL = L ? uni(vol[L - 1]) : 0u, vG = G ? uni(vol[G - 1]) : 0u;
-            const uint32_t iL = L ? uni(itf[L - 1]) : 0u, iG = G ? uni(itf[G - 1]) : 0u;
-            const uint32_t tl = L ? uni(typ[L]) : 0u, tg = G ? uni(typ[G]) : 0u;
-            bool go = true;
-            if (L != 0) {
-                const uint32_t pat = (uint32_t)__ballot(lane < n_topo && nbt == L);
-                ++c_top;
-                bool simple;
-                if (D == 2) {
-                    const uint64_t w = pat < 128 ? (pat < 64 ? sp2[0] : sp2[1])
-                                                 : (pat < 192 ? sp2[2] : sp2[3]);
-                    simple = (w >> (pat & 63)) & 1ull;
-                } else {
-                    simple = (sp3d[pat >> 6] >> (pat & 63)) & 1ull;
-                }
-                go = simple && vL > 1;
-            }
-            uint32_t p_next = e;
-            if (go) {
-                ++c_en;
-                // ---- local dH (energy.rs:521-523)
-                const uint64_t mL = __ballot(lane < n_energy && nbe == L);
-                const uint64_t mG = __ballot(lane < n_energy && nbe == G);
-                const int dl_i = 2 * __popcll(mL) - (int)n_energy;
-                const int dg_i = (int)n_energy - 2 * __popcll(mG);
-                const uint32_t lidx = L ? tl + 1 : 0u, gidx = G ? tg + 1 : 0u;
-                const uint32_t ni = nbe ? (uint32_t)typ[nbe] + 1 : 0u;
-                const double before = nbe != L ? J[lidx * T1 + ni] : 0.0;
-                const double after = nbe != G ? J[gidx * T1 + ni] : 0.0;
-                const double term = 1.0 * (after - before);
-                double dvol = 0.0, ditf = 0.0;
-                if (L) {
-                    const double v = (double)vL, T0 = (double)tv[tl];
-                    const double x1 = (v - 1.0) - T0, x0 = v - T0;
-                    dvol += lv[tl] * (x1 * x1) - lv[tl] * (x0 * x0);
-                    const double i0 = (double)iL, I0 = (double)ti[tl];
-                    const double y1 = (i0 + (double)dl_i) - I0, y0 = i0 - I0;
-                    ditf += li[tl] * (y1 * y1) - li[tl] * (y0 * y0);
-                }
-                if (G) {
-                    const double v = (double)vG, T0 = (double)tv[tg];
-                    const double x1 = (v + 1.0) - T0, x0 = v - T0;
-                    dvol += lv[tg] * (x1 * x1) - lv[tg] * (x0 * x0);
-                    const double i0 = (double)iG, I0 = (double)ti[tg];
-                    const double y1 = (i0 + (double)dg_i) - I0, y0 = i0 - I0;
-                    ditf += li[tg] * (y1 * y1) - li[tg] * (y0 * y0);
-                }
-                double dadh = 0.0;  // summed in stencil order
-                for (uint32_t q = 0; q < n_energy; ++q) dadh += rlf64(term, q);
-                const double dh = dvol + ditf + dadh;
-                bool accept = dh <= 0.0;
-                if (!accept) {
-                    // one more draw: rng.gen::<f64>()
-                    if (e >= 128 || (e >= 64 && !h1_valid)) {
-                        // the window ends here: redo the whole attempt next round
-                        if (p == 0 && h1_valid) error = true;
-                        --c_att;
-                        --c_nn;
-                        if (L != 0) --c_top;
-                        --c_en;
-                        break;
-                    }
-                    const uint64_t w = e < 64 ? rl64(h0.w, e) : rl64(h1.w, e - 64);
-                    p_next = e + 1;
-                    const double u = (double)(w >> 11) * (1.0 / 9007199254740992.0);
-                    const double pr = exp(-dh);
-                    accept = u < pr;
-                    if (fabs(u - pr) <= pr * 0x1p-50) ++c_tie;
-                }
-                if (accept) {
-                    ++c_acc;
-                    dirty = true;
-                    uint32_t cval[D];
-#pragma unroll
-                    for (int d = 0; d < D; ++d) cval[d] = rl32(dec_c[d], p);
-                    // dynamics.rs:73-98 — contact counts, pre-move lattice
-                    if (lane < n_energy) {
-                        if (nbe != L)
-                            atomicSub(&R.contact[(uint64_t)min(L, nbe) * ncell1 + max(L, nbe)], 1u);
-                        if (nbe != G)
-                            atomicAdd(&R.contact[(uint64_t)min(G, nbe) * ncell1 + max(G, nbe)], 1u);
-                    }
-                    if (lane == 0) {
-                        sig[t] = (uint16_t)G;
-                        if (L) {
-                            vol[L - 1] = vL - 1;
-                            itf[L - 1] = (uint32_t)((int32_t)iL + dl_i);
-                        }
-                        if (G) {
-                            vol[G - 1] = vG + 1;
-                            itf[G - 1] = (uint32_t)((int32_t)iG + dg_i);
-                        }
-                        // dynamics.rs:125-183 — unwrapped position sums
-                        if (L) {
-                            int64_t* sp = R.sum_pos + (uint64_t)(L - 1) * D;
-                            int64_t* so = R.sum_outer + (uint64_t)(L - 1) * D * D;
-                            for (int d = 0; d < D; ++d) sp[d] -= (int64_t)cval[d];
-                            for (int d = 0; d < D; ++d)
-                                for (int e2 = 0; e2 < D; ++e2)
-                                    so[d * D + e2] -= (int64_t)cval[d] * (int64_t)cval[e2];
-                        }
-                        if (G) {
-                            int64_t* sp = R.sum_pos + (uint64_t)(G - 1) * D;
-                            int64_t* so = R.sum_outer + (uint64_t)(G - 1) * D * D;
-                            const uint32_t v = vG + 1;
-                            const uint32_t vbg = max(v > 0 ? v - 1 : 0u, 1u);
-                            double cen[D];
-                            for (int d = 0; d < D; ++d) cen[d] = (double)sp[d] / (double)vbg;
-                            int64_t img[D];
-                            for (int d = 0; d < D; ++d) {
-                                int64_t best = (int64_t)cval[d];
-                                if (periodic) {
-                                    double bd = fabs((double)best - cen[d]);
-                                    const int64_t lo = (int64_t)cval[d] - (int64_t)dims[d];
-                                    const int64_t hi = (int64_t)cval[d] + (int64_t)dims[d];
-                                    const double dlo = fabs((double)lo - cen[d]);
-                                    if (dlo < bd) {
-                                        best = lo;
-                                        bd = dlo;
-                                    }
-                                    if (fabs((double)hi - cen[d]) < bd) best = hi;
-                                }
-                                img[d] = best;
-                                sp[d] += best;
-                            }
-                            for (int d = 0; d < D; ++d)
-                                for (int e2 = 0; e2 < D; ++e2) so[d * D + e2] += img[d] * img[e2];
-                        }
-                        // dynamics.rs:206-224 — rewrite t's halo mirrors
-                        if (periodic) {
-                            uint32_t opt[D][3], nopt[D], total = 1;
-                            for (int d = 0; d < D; ++d) {
-                                nopt[d] = 1;
-                                opt[d][0] = cval[d] + 1;
-                                if (cval[d] == 0) opt[d][nopt[d]++] = dims[d] + 1;
-                                if (cval[d] == dims[d] - 1) opt[d][nopt[d]++] = 0;
-                                total *= nopt[d];
-                            }
-                            for (uint32_t q = 1; q < total; ++q) {
-                                uint32_t r = q, f = 0;
-                                for (int d = D - 1; d >= 0; --d) {
-                                    f += opt[d][r % nopt[d]] * stride[d];
-                                    r /= nopt[d];
-                                }
-                                sig[f] = (uint16_t)G;
-                            }
-                        }
-                    }
-                    __syncthreads();
-                    energy += dh;
-                    reread();  // the lattice changed: refresh the early-out flags
-                }
-            }
-            p = p_next;
-        }
-        // monte_carlo.rs:330-356 — MCS accounting
-        if (++ain == n_sites) {
-            ain = 0;
-            ++mcs;
-            --todo;
-        }
+        if (base + p + 128 > kFill) break;
+        base += p;
+        p = 0;
+        h1_valid = true;
+        prep_half<D>(h0, draws[base + lane], range, zone, m0);
+        prep_half<D>(h1, draws[base + 64 + lane], range, zone, m1);
+        w_next = base + 192 <= kFill ? draws[base + 128 + lane] : 0ull;
+        need_decode = true;
     }
-    if (todo == 0) finished = true;
     base += p;  // absolute cursor of the next unconsumed draw
 
-exit_round:
+exit_loop:
+#undef CPM_DECODE
+#undef CPM_EVALUATE
     __syncthreads();
     if (dirty) {
-        for (uint32_t i = lane; i < n_padded; i += 64) R.sigma[i] = sig[i];
+        for (uint32_t i = lane; i < n_padded; i += 64) R.sigma[i] = sig[i] & kIdMask;
         for (uint32_t i = lane; i < n_cells; i += 64) {
             R.volume[i] = vol[i];
             R.interface[i] = itf[i];
@@ -775,7 +898,7 @@ exit_round:
         if (error) {
             R.error = 1;
             R.done = 1;
-        } else if (finished) {
+        } else if (todo == 0) {
             R.cursor = base;
             R.done = 1;
         } else {
@@ -1139,8 +1262,10 @@ Engine* engine_create(int device) {
     int lds = 0;
     HIP_OK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
     e->max_lds = (uint32_t)lds;
-    for (auto fn : {(const void*)k_lds_engine<2>, (const void*)k_lds_engine<3>,
-                    (const void*)k_sublattice<2>, (const void*)k_sublattice<3>})
+    for (auto fn : {(const void*)k_lds_engine<2, 4>, (const void*)k_lds_engine<2, 8>,
+                    (const void*)k_lds_engine<3, 6>, (const void*)k_lds_engine<3, 18>,
+                    (const void*)k_lds_engine<3, 26>, (const void*)k_sublattice<2>,
+                    (const void*)k_sublattice<3>})
         HIP_OK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds - 256));
     return e;
 }
@@ -1234,7 +1359,9 @@ static Layout layout_of(const Model& m, bool with_draws) {
 }
 
 static bool lds_eligible(const Engine* e, const Model& m, uint32_t* bytes) {
-    if (m.n_cells > 65535 || m.n_padded > (1u << 30)) return false;
+    // packed LDS words: 12-bit ids, 4-bit adhesion indices
+    if (m.n_cells > kMaxPackedCells || m.n_padded > (1u << 30)) return false;
+    if (m.cfg.types.size() > 15) return false;
     const LdsLayout Ly =
         lds_layout((uint32_t)m.n_padded, (uint32_t)m.n_cells, (uint32_t)m.cfg.types.size());
     *bytes = Ly.total;
@@ -1364,6 +1491,20 @@ static void add_counters(cpm_counters* c, const Rep& R) {
 static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint64_t* todo,
                            cpm_counters* counters);
 
+// key = 100 * D + energy stencil size
+static void launch_lds_engine(int key, uint32_t lds, uint32_t count, hipStream_t st, Rep* reps,
+                              const uint32_t* idx, const uint64_t* sp3d) {
+    const dim3 g(count), b(64);
+    switch (key) {
+        case 204: hipLaunchKernelGGL((k_lds_engine<2, 4>), g, b, lds, st, reps, idx, sp3d); break;
+        case 208: hipLaunchKernelGGL((k_lds_engine<2, 8>), g, b, lds, st, reps, idx, sp3d); break;
+        case 306: hipLaunchKernelGGL((k_lds_engine<3, 6>), g, b, lds, st, reps, idx, sp3d); break;
+        case 318: hipLaunchKernelGGL((k_lds_engine<3, 18>), g, b, lds, st, reps, idx, sp3d); break;
+        case 326: hipLaunchKernelGGL((k_lds_engine<3, 26>), g, b, lds, st, reps, idx, sp3d); break;
+        default: throw Error(CPM_ERR_DEVICE, "no exact-replay kernel for this stencil");
+    }
+}
+
 void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* todo, int mode,
                   cpm_counters* counters) {
     if (mode == CPM_MODE_SUBLATTICE) {
@@ -1400,8 +1541,10 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
         for (int k = 0; k < 6; ++k) R.cnt[k] = 0;
         reps[i] = R;
         if (!todo[i]) continue;
-        if (m->dev->lds) classes[{m->D, lds_class(e, m->dev->lds_bytes)}].push_back((uint32_t)i);
-        else global_idx.push_back((uint32_t)i);
+        if (m->dev->lds)
+            classes[{m->D * 100 + m->n_energy, lds_class(e, m->dev->lds_bytes)}].push_back((uint32_t)i);
+        else
+            global_idx.push_back((uint32_t)i);
     }
     struct Launch {
         int D;
@@ -1458,12 +1601,8 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
                 for (size_t li = 0; li < launches.size(); ++li) {
                     const Launch& l = launches[li];
                     hipStream_t st = e->cls_stream[li % Engine::kClassStreams];
-                    if (l.D == 2)
-                        hipLaunchKernelGGL(k_lds_engine<2>, dim3(l.count), dim3(64), l.lds, st,
-                                           e->reps_dev, e->idx_dev + l.first, e->sp3d);
-                    else
-                        hipLaunchKernelGGL(k_lds_engine<3>, dim3(l.count), dim3(64), l.lds, st,
-                                           e->reps_dev, e->idx_dev + l.first, e->sp3d);
+                    launch_lds_engine(l.D, l.lds, l.count, st, e->reps_dev, e->idx_dev + l.first,
+                                      e->sp3d);
                     HIP_OK(hipGetLastError());
                 }
                 for (int k = 0; k < ns; ++k) {
