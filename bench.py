@@ -28,6 +28,8 @@ sys.path.insert(0, os.path.join(ROOT, "cpm-rust_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+DOMINANT_KERNEL = "k_lds_engine<2, 8>"  # exact replay, 2D, Moore energy stencil
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_lds_engine_2d_run_batch4096_pmc.txt")
 SIDES = (48, 64, 96, 128)
 INTERVAL = 500
 MASTER_SEED = 42
@@ -69,6 +71,44 @@ def algorithmic_bytes(c: dict) -> float:
     356 per accepted copy (sigma write, V/I, position sums, 16 contact
     read-modify-writes, energy)."""
     return 8.0 * c["attempts"] + 28.0 * c["topology"] + 56.0 * c["energy"] + 356.0 * c["accepted"]
+
+
+def measured_traffic():
+    """HBM bytes per launch of the dominant kernel, from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE passes, each scaled
+    by the same counter's reading on a device copy of known size); None if
+    the summary is absent."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration(mib: int = 256, reps: int = 4):
+    """A plain device-to-device copy of known size (reads and writes `mib`
+    MiB per launch, HIP's own copy kernel), run beside the engine so a PMC
+    pass can scale FETCH_SIZE and WRITE_SIZE to bytes.  Calls the HIP
+    runtime directly, so no other framework's kernels enter the profile."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    for ptr in (a, b):
+        if hip.hipMalloc(C.byref(ptr), C.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed in the copy calibration")
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    for _ in range(reps):
+        if hip.hipMemcpy(b, a, C.c_size_t(n), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed in the copy calibration")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(f"[bench] copy calibration: {reps} device copies of {mib} MiB", flush=True)
 
 
 def cpu_baseline(params, seconds: float = 12.0):
@@ -116,6 +156,8 @@ def main():
     ap.add_argument("--interval", type=int, default=INTERVAL)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-mcs", type=int, default=20)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 256 MiB device copy (PMC byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the engine shards replicas across GPUs only by "
@@ -142,7 +184,8 @@ def main():
         print(f"[bench] warmup {w}: {c['kernel_ms']:.0f} ms", flush=True)
 
     totals = {k: 0.0 for k in ("attempts", "non_null", "topology", "energy", "accepted",
-                               "exp_near_ties", "kernel_ms")}
+                               "exp_near_ties", "kernel_ms", "replay_launches",
+                               "replay_launch_ms")}
     t0 = time.perf_counter()
     for s in range(args.steps):
         c = cpm.Replica.run_mcs_many(reps, args.interval)
@@ -152,9 +195,12 @@ def main():
               f" G attempts/s", flush=True)
     wall = time.perf_counter() - t0
     value = totals["attempts"] / wall
-    avg_launch_s = totals["kernel_ms"] / 1e3 / args.steps
-    bytes_per_launch = algorithmic_bytes(totals) / args.steps
-    achieved = bytes_per_launch / avg_launch_s / 1e9
+    # roofline of the dominant kernel: algorithmic bytes per launch over its
+    # average launch duration (HIP events around each launch, on its stream)
+    launches = max(1.0, totals["replay_launches"])
+    avg_launch_s = totals["replay_launch_ms"] / 1e3 / launches
+    bytes_per_launch = algorithmic_bytes(totals) / launches
+    achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
 
     # plain one-replica-per-launch kernel on a 64x64 replica
     p64 = next(p for p in params if p["side"] == 64)
@@ -170,6 +216,8 @@ def main():
     step_s = wall / args.steps
     sims_per_hour = len(reps) * 3600.0 / (t_init + 20 * step_s)
 
+    if args.copy_calibration:
+        copy_calibration()
     cpu = None if args.no_cpu else cpu_baseline(params)
     line = {
         "metric": "copy attempts/s on 1 GPU (batched exact-replay run_batch)",
@@ -193,7 +241,10 @@ def main():
         "sims_per_hour_projected": sims_per_hour,
         "counters": {k: totals[k] for k in totals},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "kernel": DOMINANT_KERNEL, "launches": int(launches),
+                     "avg_launch_ms": avg_launch_s * 1e3,
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -1230,6 +1230,7 @@ struct Engine {
     hipEvent_t cls_done[kClassStreams] = {}, fill_done = nullptr;
     JumpTable jt{};
     uint32_t max_lds = 0;
+    std::vector<hipEvent_t> launch_ev;  // start/stop pairs around replay launches
     std::mutex mu;
 };
 
@@ -1311,6 +1312,7 @@ void engine_destroy(Engine* e) {
         if (e->cls_done[i]) (void)hipEventDestroy(e->cls_done[i]);
     }
     if (e->fill_done) (void)hipEventDestroy(e->fill_done);
+    for (hipEvent_t ev : e->launch_ev) (void)hipEventDestroy(ev);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
@@ -1388,6 +1390,23 @@ static void upload(Engine* e, Model* m) {
     m->dev = d;
     d->lds = lds;
     d->lds_bytes = lds_bytes;
+    if (lds) {
+        // the LDS kernel addresses the full shell from the strides and maps
+        // the energy stencil into it at compile time
+        const int NT = m->D == 2 ? 8 : 26;
+        bool same = m->n_topo == NT;
+        for (int k = 0; same && k < NT; ++k) {
+            const int code = k < NT / 2 ? k : k + 1;
+            const int64_t o0 = code % 3 - 1, o1 = (code / 3) % 3 - 1, o2 = code / 9 - 1;
+            const int64_t a = m->D == 2 ? o0 * (int64_t)m->strides[0] + o1
+                                        : o0 * (int64_t)m->strides[0] +
+                                              o1 * (int64_t)m->strides[1] + o2;
+            same = a == m->topo_off[k];
+        }
+        for (int q = 0; same && q < m->n_energy; ++q)
+            same = m->energy_off[q] == m->topo_off[shell_index(m->D, m->n_energy, q)];
+        if (!same) throw Error(CPM_ERR_DEVICE, "stencil order differs from the LDS kernel's");
+    }
     Rep& R = d->rep;
     R.D = m->D;
     R.boundary = m->cfg.boundary;
@@ -1569,6 +1588,8 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
                            e->reps_dev, e->idx_dev + global_first, ng, e->sp3d);
         HIP_OK(hipGetLastError());
     }
+    double replay_ms = 0.0;
+    uint64_t replay_launches = 0;
     if (!launches.empty()) {
         // rounds of (fill every unfinished stream, advance every unfinished
         // replica); a replica is finished once its MCS target is reached
@@ -1589,6 +1610,13 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
                 throw Error(CPM_ERR_DEVICE, "exact-replay rounds did not converge");
             const uint64_t burst = std::max<uint64_t>(1, std::min<uint64_t>(rounds_guess, 8));
             rounds += burst;
+            const size_t n_ev = 2 * burst * launches.size();
+            while (e->launch_ev.size() < n_ev) {
+                hipEvent_t ev;
+                HIP_OK(hipEventCreate(&ev));
+                e->launch_ev.push_back(ev);
+            }
+            size_t ev_i = 0;
             for (uint64_t b = 0; b < burst; ++b) {
                 const uint32_t threads = (uint32_t)n * kSegs;
                 hipLaunchKernelGGL(k_rng_fill, dim3((threads + 255) / 256), dim3(256), 0,
@@ -1601,9 +1629,12 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
                 for (size_t li = 0; li < launches.size(); ++li) {
                     const Launch& l = launches[li];
                     hipStream_t st = e->cls_stream[li % Engine::kClassStreams];
+                    HIP_OK(hipEventRecord(e->launch_ev[ev_i], st));
                     launch_lds_engine(l.D, l.lds, l.count, st, e->reps_dev, e->idx_dev + l.first,
                                       e->sp3d);
                     HIP_OK(hipGetLastError());
+                    HIP_OK(hipEventRecord(e->launch_ev[ev_i + 1], st));
+                    ev_i += 2;
                 }
                 for (int k = 0; k < ns; ++k) {
                     HIP_OK(hipEventRecord(e->cls_done[k], e->cls_stream[k]));
@@ -1616,6 +1647,12 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
             HIP_OK(hipMemcpyAsync(reps.data(), e->reps_dev, n * sizeof(Rep), hipMemcpyDeviceToHost,
                                   e->stream));
             HIP_OK(hipStreamSynchronize(e->stream));
+            for (size_t i = 0; i < ev_i; i += 2) {
+                float lms = 0.f;
+                HIP_OK(hipEventElapsedTime(&lms, e->launch_ev[i], e->launch_ev[i + 1]));
+                replay_ms += lms;
+                ++replay_launches;
+            }
             for (size_t i = 0; i < n; ++i) all &= reps[i].done != 0;
             if (all) break;
         }
@@ -1629,7 +1666,11 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
     HIP_OK(hipStreamSynchronize(e->stream));
     float ms = 0.f;
     HIP_OK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
-    if (counters) counters->kernel_ms += ms;
+    if (counters) {
+        counters->kernel_ms += ms;
+        counters->replay_launch_ms += replay_ms;
+        counters->replay_launches += replay_launches;
+    }
     for (size_t i = 0; i < n; ++i)
         if (reps[i].error)
             throw Error(CPM_ERR_DEVICE, "an attempt needed more than 128 stream draws");

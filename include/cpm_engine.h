@@ -105,6 +105,11 @@ typedef struct {
 typedef struct {
     uint64_t attempts, non_null, topology, energy, accepted, exp_near_ties;
     double kernel_ms; /* device time of the launches, HIP events */
+    /* the exact-replay LDS kernel alone: its launches and their summed
+       durations, from HIP events recorded around each launch on the
+       stream it runs on */
+    uint64_t replay_launches;
+    double replay_launch_ms;
 } cpm_counters;
 
 typedef struct cpm_engine cpm_engine;
