@@ -917,12 +917,23 @@ exit_loop:
 // only sigma[t]; sites of one colour (c_d mod 2 on every axis) are at
 // Chebyshev distance >= 2, so within a colour phase no site reads what
 // another writes (periodic sides are even, so the wrap keeps the
-// colouring).  Same-colour removals are never 8/26-adjacent, so each stays
-// a simple point after the others: topology checks stay exact.  Per-cell V
-// and I are read as they stand (possibly mid-phase) and updated with
-// atomics.  Each site is the target of exactly one attempt per MCS; the
-// colour order is redrawn every MCS.  Random numbers come from SplitMix64
-// indexed by (replica key, mcs, colour, site, draw).
+// colouring), and topology checks stay exact.
+//
+// Schedule: an MCS is kSlSub sub-sweeps; a sub-sweep visits the 2^D
+// colours in a random order, and each site of a colour is the target of an
+// attempt with probability 1/kSlSub, so a site is attempted Binomial(kSlSub,
+// 1/kSlSub) times per MCS (the reference's uniform target choice gives
+// about Poisson(1)), n_sites attempts per MCS on average.
+//
+// Per-cell V and I are exact: attempts of a colour phase run in batches of
+// one site per thread; within a batch, the attempts that reach the volume
+// check (non-null, simple point) commit in descending order of a random
+// priority among those sharing a cell — each round, an attempt proceeds if
+// it holds the highest priority on both of its cells (an LDS atomicMax per
+// cell), so every decision sees V and I exactly as a sequential pass in
+// that order would.  Outcomes depend only on the counter-based random
+// numbers (SplitMix64 of replica key, MCS, sub-sweep, colour, site and
+// draw), not on thread timing.
 // ===========================================================================
 __device__ __forceinline__ uint64_t splitmix_at(uint64_t key, uint64_t ctr) {
     uint64_t z = key + ctr * 0x9e3779b97f4a7c15ULL;
@@ -932,6 +943,14 @@ __device__ __forceinline__ uint64_t splitmix_at(uint64_t key, uint64_t ctr) {
 }
 
 constexpr uint32_t kSlThreads = 256;
+constexpr uint32_t kSlSub = 4;  // sub-sweeps per MCS (a power of two)
+
+// Dynamic LDS of the sublattice kernel: the exact-replay layout plus one
+// 64-bit priority word per cell.
+__host__ __device__ inline uint32_t sl_prio_offset(const LdsLayout& Ly) { return Ly.total; }
+__host__ __device__ inline uint32_t sl_lds_bytes(const LdsLayout& Ly, uint32_t n_cells) {
+    return Ly.total + (n_cells * 8 + 15) / 16 * 16;
+}
 
 template <int D>
 __global__ void __launch_bounds__(kSlThreads) k_sublattice(Rep* __restrict__ reps,
@@ -944,8 +963,7 @@ __global__ void __launch_bounds__(kSlThreads) k_sublattice(Rep* __restrict__ rep
     const uint32_t T = R.n_types, T1 = T + 1, n_cells = R.n_cells;
     const uint32_t n_padded = R.n_padded;
     const LdsLayout Ly = lds_layout(n_padded, n_cells, T);
-    uint16_t* sig = (uint16_t*)(smem + Ly.sigma);
-    uint8_t* typ = smem + Ly.type;
+    uint16_t* sig = (uint16_t*)(smem + Ly.sigma);  // packed: id | (type + 1) << 12
     uint32_t* vol = (uint32_t*)(smem + Ly.vol);
     uint32_t* itf = (uint32_t*)(smem + Ly.itf);
     double* J = (double*)(smem + Ly.J);
@@ -953,11 +971,16 @@ __global__ void __launch_bounds__(kSlThreads) k_sublattice(Rep* __restrict__ rep
     double* li = (double*)(smem + Ly.li);
     uint32_t* tv = (uint32_t*)(smem + Ly.tv);
     uint32_t* ti = (uint32_t*)(smem + Ly.ti);
-    for (uint32_t i = tid; i < n_padded; i += kSlThreads) sig[i] = (uint16_t)R.sigma[i];
+    unsigned long long* prio = (unsigned long long*)(smem + sl_prio_offset(Ly));
+    const uint32_t* __restrict__ gtyp = R.type_index;
+    for (uint32_t i = tid; i < n_padded; i += kSlThreads) {
+        const uint32_t id = R.sigma[i];
+        sig[i] = (uint16_t)(id ? id | ((gtyp[id - 1] + 1) << 12) : 0u);
+    }
     for (uint32_t i = tid; i < n_cells; i += kSlThreads) {
-        typ[i + 1] = (uint8_t)R.type_index[i];
         vol[i] = R.volume[i];
         itf[i] = R.interface[i];
+        prio[i] = 0;
     }
     for (uint32_t i = tid; i < T1 * T1; i += kSlThreads) J[i] = R.params[2 * T + i];
     for (uint32_t i = tid; i < T; i += kSlThreads) {
@@ -984,158 +1007,212 @@ __global__ void __launch_bounds__(kSlThreads) k_sublattice(Rep* __restrict__ rep
 
     for (uint64_t m = 0; m < R.todo; ++m) {
         const uint64_t mcs = R.mcs + m;
-        // colour order for this MCS: Fisher-Yates from one hash
-        uint32_t order[NC];
-        for (uint32_t c = 0; c < NC; ++c) order[c] = c;
-        uint64_t hperm = splitmix_at(key, mcs * 0x100000001ull + 0xC01);
-        for (uint32_t c = NC - 1; c > 0; --c) {
-            const uint32_t j = (uint32_t)((hperm & 0xffff) % (c + 1));
-            hperm >>= 8;
-            const uint32_t tmp = order[c];
-            order[c] = order[j];
-            order[j] = tmp;
-        }
-        for (uint32_t ci = 0; ci < NC; ++ci) {
-            const uint32_t col = order[ci];
-            uint32_t half[D], nsite = 1;
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                const uint32_t cd = (col >> d) & 1u;
-                half[d] = dims[d] > cd ? (dims[d] - cd + 1) / 2 : 0;
-                nsite *= half[d];
+        for (uint32_t sub = 0; sub < kSlSub; ++sub) {
+            const uint64_t phase0 = (mcs * kSlSub + sub) * NC;
+            // colour order of this sub-sweep: Fisher-Yates from one hash
+            uint32_t order[NC];
+            for (uint32_t c = 0; c < NC; ++c) order[c] = c;
+            uint64_t hperm = splitmix_at(key, ~phase0);
+            for (uint32_t c = NC - 1; c > 0; --c) {
+                const uint32_t j = (uint32_t)((hperm & 0xffff) % (c + 1));
+                hperm >>= 16;
+                const uint32_t tmp = order[c];
+                order[c] = order[j];
+                order[j] = tmp;
             }
-            for (uint32_t j = tid; j < nsite; j += kSlThreads) {
-                uint32_t rem = j, c[D], t = 0;
+            for (uint32_t ci = 0; ci < NC; ++ci) {
+                const uint32_t col = order[ci];
+                const uint64_t phase = phase0 + col;
+                uint32_t half[D], nsite = 1;
 #pragma unroll
-                for (int d = D - 1; d >= 0; --d) {
-                    c[d] = 2 * (rem % half[d]) + ((col >> d) & 1u);
-                    rem /= half[d];
-                    t += (c[d] + 1) * stride[d];
+                for (int d = 0; d < D; ++d) {
+                    const uint32_t cd = (col >> d) & 1u;
+                    half[d] = dims[d] > cd ? (dims[d] - cd + 1) / 2 : 0;
+                    nsite *= half[d];
                 }
-                const uint64_t ctr = ((mcs * NC + col) * (uint64_t)R.n_sites + j) * 2;
-                const uint64_t hk = splitmix_at(key, ctr);
-                const uint32_t k = (uint32_t)(((hk >> 32) * (uint64_t)n_copy) >> 32);
-                const uint32_t s = (uint32_t)((int32_t)t + R.copy_off[k]);
-                c_att++;
-                const uint32_t L = sig[t], G = sig[s];
-                if (L == G) continue;
-                c_nn++;
-                if (L != 0) {
-                    uint32_t pat = 0;
-                    for (uint32_t i = 0; i < n_topo; ++i)
-                        pat |= (uint32_t)(sig[(int32_t)t + R.topo_off[i]] == L) << i;
-                    c_top++;
-                    const bool simple = D == 2 ? ((c_sp2d[pat >> 5] >> (pat & 31)) & 1u)
-                                               : ((sp3d[pat >> 6] >> (pat & 63)) & 1ull);
-                    if (!simple) continue;
-                    if (__atomic_load_n(&vol[L - 1], __ATOMIC_RELAXED) <= 1) continue;
-                }
-                c_en++;
-                const uint32_t tl = L ? typ[L] : 0u, tg = G ? typ[G] : 0u;
-                const uint32_t lidx = L ? tl + 1 : 0u, gidx = G ? tg + 1 : 0u;
-                int nl = 0, ng = 0;
-                double dadh = 0.0;
-                for (uint32_t q = 0; q < n_energy; ++q) {
-                    const uint32_t nb = sig[(int32_t)t + R.energy_off[q]];
-                    nl += nb == L;
-                    ng += nb == G;
-                    const uint32_t ni = nb ? (uint32_t)typ[nb] + 1 : 0u;
-                    const double before = nb != L ? J[lidx * T1 + ni] : 0.0;
-                    const double after = nb != G ? J[gidx * T1 + ni] : 0.0;
-                    dadh += 1.0 * (after - before);
-                }
-                const int dl_i = 2 * nl - (int)n_energy, dg_i = (int)n_energy - 2 * ng;
-                double dvol = 0.0, ditf = 0.0;
-                if (L) {
-                    const double v = (double)__atomic_load_n(&vol[L - 1], __ATOMIC_RELAXED);
-                    const double x1 = (v - 1.0) - (double)tv[tl], x0 = v - (double)tv[tl];
-                    dvol += lv[tl] * (x1 * x1) - lv[tl] * (x0 * x0);
-                    const double i0 = (double)__atomic_load_n(&itf[L - 1], __ATOMIC_RELAXED);
-                    const double y1 = (i0 + (double)dl_i) - (double)ti[tl], y0 = i0 - (double)ti[tl];
-                    ditf += li[tl] * (y1 * y1) - li[tl] * (y0 * y0);
-                }
-                if (G) {
-                    const double v = (double)__atomic_load_n(&vol[G - 1], __ATOMIC_RELAXED);
-                    const double x1 = (v + 1.0) - (double)tv[tg], x0 = v - (double)tv[tg];
-                    dvol += lv[tg] * (x1 * x1) - lv[tg] * (x0 * x0);
-                    const double i0 = (double)__atomic_load_n(&itf[G - 1], __ATOMIC_RELAXED);
-                    const double y1 = (i0 + (double)dg_i) - (double)ti[tg], y0 = i0 - (double)ti[tg];
-                    ditf += li[tg] * (y1 * y1) - li[tg] * (y0 * y0);
-                }
-                const double dh = dvol + ditf + dadh;
-                if (!(dh <= 0.0)) {
-                    const uint64_t hu = splitmix_at(key, ctr + 1);
-                    const double u = (double)(hu >> 11) * (1.0 / 9007199254740992.0);
-                    if (!(u < exp(-dh))) continue;
-                }
-                c_acc++;
-                for (uint32_t q = 0; q < n_energy; ++q) {
-                    const uint32_t nb = sig[(int32_t)t + R.energy_off[q]];
-                    if (nb != L) atomicSub(&R.contact[(uint64_t)min(L, nb) * ncell1 + max(L, nb)], 1u);
-                    if (nb != G) atomicAdd(&R.contact[(uint64_t)min(G, nb) * ncell1 + max(G, nb)], 1u);
-                }
-                sig[t] = (uint16_t)G;
-                uint32_t vg_new = 0;
-                if (L) {
-                    atomicSub(&vol[L - 1], 1u);
-                    atomicAdd(&itf[L - 1], (uint32_t)dl_i);
-                    unsigned long long* sp = (unsigned long long*)(R.sum_pos + (uint64_t)(L - 1) * D);
-                    unsigned long long* so = (unsigned long long*)(R.sum_outer + (uint64_t)(L - 1) * D * D);
-                    for (int d = 0; d < D; ++d) atomicAdd(&sp[d], (unsigned long long)(-(int64_t)c[d]));
-                    for (int d = 0; d < D; ++d)
-                        for (int e2 = 0; e2 < D; ++e2)
-                            atomicAdd(&so[d * D + e2],
-                                      (unsigned long long)(-((int64_t)c[d] * (int64_t)c[e2])));
-                }
-                if (G) {
-                    vg_new = atomicAdd(&vol[G - 1], 1u) + 1;
-                    atomicAdd(&itf[G - 1], (uint32_t)dg_i);
-                    int64_t* spg = R.sum_pos + (uint64_t)(G - 1) * D;
-                    unsigned long long* sp = (unsigned long long*)spg;
-                    unsigned long long* so = (unsigned long long*)(R.sum_outer + (uint64_t)(G - 1) * D * D);
-                    const uint32_t vbg = max(vg_new > 0 ? vg_new - 1 : 0u, 1u);
-                    int64_t img[D];
-                    for (int d = 0; d < D; ++d) {
-                        const double cen = (double)__atomic_load_n(&spg[d], __ATOMIC_RELAXED) / (double)vbg;
-                        int64_t best = (int64_t)c[d];
-                        if (periodic) {
-                            double bd = fabs((double)best - cen);
-                            const int64_t lo = (int64_t)c[d] - (int64_t)dims[d];
-                            const int64_t hi = (int64_t)c[d] + (int64_t)dims[d];
-                            const double dlo = fabs((double)lo - cen);
-                            if (dlo < bd) {
-                                best = lo;
-                                bd = dlo;
-                            }
-                            if (fabs((double)hi - cen) < bd) best = hi;
-                        }
-                        img[d] = best;
-                    }
-                    for (int d = 0; d < D; ++d) atomicAdd(&sp[d], (unsigned long long)img[d]);
-                    for (int d = 0; d < D; ++d)
-                        for (int e2 = 0; e2 < D; ++e2)
-                            atomicAdd(&so[d * D + e2], (unsigned long long)(img[d] * img[e2]));
-                }
-                if (periodic) {
-                    uint32_t opt[D][3], nopt[D], total = 1;
-                    for (int d = 0; d < D; ++d) {
-                        nopt[d] = 1;
-                        opt[d][0] = c[d] + 1;
-                        if (c[d] == 0) opt[d][nopt[d]++] = dims[d] + 1;
-                        if (c[d] == dims[d] - 1) opt[d][nopt[d]++] = 0;
-                        total *= nopt[d];
-                    }
-                    for (uint32_t q = 1; q < total; ++q) {
-                        uint32_t r = q, f = 0;
+                if (nsite == 0) continue;
+                // batches start at a random rotation of the colour's sites
+                const uint32_t rot = (uint32_t)(splitmix_at(key ^ 0x5bd1e995ull, phase) % nsite);
+                for (uint32_t b0 = 0; b0 < nsite; b0 += kSlThreads) {
+                    const uint32_t jb = b0 + tid;
+                    uint32_t j = 0, c[D], t = 0, s = 0, Lw = 0, Gw = 0;
+                    unsigned long long pr = 0;
+                    uint64_t ctr = 0;
+                    bool pending = false;
+                    if (jb < nsite) {
+                        j = jb + rot;
+                        if (j >= nsite) j -= nsite;
+                        uint32_t rem = j;
+#pragma unroll
                         for (int d = D - 1; d >= 0; --d) {
-                            f += opt[d][r % nopt[d]] * stride[d];
-                            r /= nopt[d];
+                            c[d] = 2 * (rem % half[d]) + ((col >> d) & 1u);
+                            rem /= half[d];
+                            t += (c[d] + 1) * stride[d];
                         }
-                        sig[f] = (uint16_t)G;
+                        ctr = (phase * (uint64_t)R.n_sites + j) * 4;
+                        const uint64_t h0 = splitmix_at(key, ctr);
+                        if ((h0 & (kSlSub - 1)) == 0) {  // this site is attempted
+                            const uint32_t k = (uint32_t)(((h0 >> 32) * (uint64_t)n_copy) >> 32);
+                            s = (uint32_t)((int32_t)t + R.copy_off[k]);
+                            c_att++;
+                            Lw = sig[t];
+                            Gw = sig[s];
+                            if (Lw != Gw) {
+                                c_nn++;
+                                const uint32_t L = Lw & kIdMask;
+                                bool simple = true;
+                                if (L != 0) {
+                                    uint32_t pat = 0;
+                                    for (uint32_t i = 0; i < n_topo; ++i)
+                                        pat |= (uint32_t)((sig[(int32_t)t + R.topo_off[i]] & kIdMask) == L) << i;
+                                    c_top++;
+                                    simple = D == 2 ? ((c_sp2d[pat >> 5] >> (pat & 31)) & 1u)
+                                                    : ((sp3d[pat >> 6] >> (pat & 63)) & 1ull);
+                                }
+                                if (simple) {
+                                    pending = true;
+                                    // unique within the phase: site index in the low 24 bits
+                                    pr = 0x8000000000000000ull |
+                                         (splitmix_at(key, ctr + 1) & 0x7fffffffff000000ull) | j;
+                                }
+                            }
+                        }
+                    }
+                    // commit rounds: the highest priority on both cells goes first
+                    for (;;) {
+                        if (!__syncthreads_or(pending)) break;
+                        const uint32_t L = Lw & kIdMask, G = Gw & kIdMask;
+                        if (pending) {
+                            if (L) atomicMax(&prio[L - 1], pr);
+                            if (G) atomicMax(&prio[G - 1], pr);
+                        }
+                        __syncthreads();
+                        const bool win = pending && (!L || prio[L - 1] == pr) &&
+                                         (!G || prio[G - 1] == pr);
+                        __syncthreads();
+                        if (pending) {
+                            if (L) prio[L - 1] = 0;
+                            if (G) prio[G - 1] = 0;
+                        }
+                        if (win) {
+                            pending = false;
+                            const uint32_t vL = L ? vol[L - 1] : 0u;
+                            if (L == 0 || vL > 1) {
+                                c_en++;
+                                const uint32_t lidx = Lw >> 12, gidx = Gw >> 12;
+                                const uint32_t tl = lidx ? lidx - 1 : 0u, tg = gidx ? gidx - 1 : 0u;
+                                int nl = 0, ng = 0;
+                                double dadh = 0.0;  // energy.rs:242-264, stencil order
+                                for (uint32_t q = 0; q < n_energy; ++q) {
+                                    const uint32_t w = sig[(int32_t)t + R.energy_off[q]];
+                                    const uint32_t nb = w & kIdMask, ni = w >> 12;
+                                    nl += nb == L;
+                                    ng += nb == G;
+                                    const double before = nb != L ? J[lidx * T1 + ni] : 0.0;
+                                    const double after = nb != G ? J[gidx * T1 + ni] : 0.0;
+                                    dadh += 1.0 * (after - before);
+                                }
+                                const int dl_i = 2 * nl - (int)n_energy, dg_i = (int)n_energy - 2 * ng;
+                                double dvol = 0.0, ditf = 0.0;
+                                if (L) {
+                                    const double v = (double)vL;
+                                    const double x1 = (v - 1.0) - (double)tv[tl], x0 = v - (double)tv[tl];
+                                    dvol += lv[tl] * (x1 * x1) - lv[tl] * (x0 * x0);
+                                    const double i0 = (double)itf[L - 1];
+                                    const double y1 = (i0 + (double)dl_i) - (double)ti[tl], y0 = i0 - (double)ti[tl];
+                                    ditf += li[tl] * (y1 * y1) - li[tl] * (y0 * y0);
+                                }
+                                if (G) {
+                                    const double v = (double)vol[G - 1];
+                                    const double x1 = (v + 1.0) - (double)tv[tg], x0 = v - (double)tv[tg];
+                                    dvol += lv[tg] * (x1 * x1) - lv[tg] * (x0 * x0);
+                                    const double i0 = (double)itf[G - 1];
+                                    const double y1 = (i0 + (double)dg_i) - (double)ti[tg], y0 = i0 - (double)ti[tg];
+                                    ditf += li[tg] * (y1 * y1) - li[tg] * (y0 * y0);
+                                }
+                                const double dh = dvol + ditf + dadh;
+                                bool accept = dh <= 0.0;
+                                if (!accept) {
+                                    const uint64_t hu = splitmix_at(key, ctr + 2);
+                                    const double u = (double)(hu >> 11) * (1.0 / 9007199254740992.0);
+                                    accept = u < exp(-dh);
+                                }
+                                if (accept) {
+                                    c_acc++;
+                                    // dynamics.rs:73-98: contacts on the pre-move lattice
+                                    for (uint32_t q = 0; q < n_energy; ++q) {
+                                        const uint32_t nb = sig[(int32_t)t + R.energy_off[q]] & kIdMask;
+                                        if (nb != L)
+                                            atomicSub(&R.contact[(uint64_t)min(L, nb) * ncell1 + max(L, nb)], 1u);
+                                        if (nb != G)
+                                            atomicAdd(&R.contact[(uint64_t)min(G, nb) * ncell1 + max(G, nb)], 1u);
+                                    }
+                                    sig[t] = (uint16_t)Gw;
+                                    // this round owns L's and G's V, I and position sums
+                                    if (L) {
+                                        vol[L - 1] = vL - 1;
+                                        itf[L - 1] = (uint32_t)((int32_t)itf[L - 1] + dl_i);
+                                        int64_t* sp = R.sum_pos + (uint64_t)(L - 1) * D;
+                                        int64_t* so = R.sum_outer + (uint64_t)(L - 1) * D * D;
+                                        for (int d = 0; d < D; ++d) sp[d] -= (int64_t)c[d];
+                                        for (int d = 0; d < D; ++d)
+                                            for (int e2 = 0; e2 < D; ++e2)
+                                                so[d * D + e2] -= (int64_t)c[d] * (int64_t)c[e2];
+                                    }
+                                    if (G) {
+                                        const uint32_t vg_new = vol[G - 1] + 1;
+                                        vol[G - 1] = vg_new;
+                                        itf[G - 1] = (uint32_t)((int32_t)itf[G - 1] + dg_i);
+                                        int64_t* sp = R.sum_pos + (uint64_t)(G - 1) * D;
+                                        int64_t* so = R.sum_outer + (uint64_t)(G - 1) * D * D;
+                                        const uint32_t vbg = max(vg_new - 1, 1u);
+                                        int64_t img[D];
+                                        for (int d = 0; d < D; ++d) {
+                                            const double cen = (double)sp[d] / (double)vbg;
+                                            int64_t best = (int64_t)c[d];
+                                            if (periodic) {
+                                                double bd = fabs((double)best - cen);
+                                                const int64_t lo = (int64_t)c[d] - (int64_t)dims[d];
+                                                const int64_t hi = (int64_t)c[d] + (int64_t)dims[d];
+                                                const double dlo = fabs((double)lo - cen);
+                                                if (dlo < bd) {
+                                                    best = lo;
+                                                    bd = dlo;
+                                                }
+                                                if (fabs((double)hi - cen) < bd) best = hi;
+                                            }
+                                            img[d] = best;
+                                        }
+                                        for (int d = 0; d < D; ++d) sp[d] += img[d];
+                                        for (int d = 0; d < D; ++d)
+                                            for (int e2 = 0; e2 < D; ++e2) so[d * D + e2] += img[d] * img[e2];
+                                    }
+                                    if (periodic) {  // t's halo mirrors
+                                        uint32_t opt[D][3], nopt[D], total = 1;
+                                        for (int d = 0; d < D; ++d) {
+                                            nopt[d] = 1;
+                                            opt[d][0] = c[d] + 1;
+                                            if (c[d] == 0) opt[d][nopt[d]++] = dims[d] + 1;
+                                            if (c[d] == dims[d] - 1) opt[d][nopt[d]++] = 0;
+                                            total *= nopt[d];
+                                        }
+                                        for (uint32_t q = 1; q < total; ++q) {
+                                            uint32_t r = q, f = 0;
+                                            for (int d = D - 1; d >= 0; --d) {
+                                                f += opt[d][r % nopt[d]] * stride[d];
+                                                r /= nopt[d];
+                                            }
+                                            sig[f] = (uint16_t)Gw;
+                                        }
+                                    }
+                                }
+                            }
+                        }
+                        __threadfence_block();
                     }
                 }
+                __syncthreads();
             }
-            __syncthreads();
         }
     }
     atomicAdd(&s_cnt[0], c_att);
@@ -1144,7 +1221,7 @@ __global__ void __launch_bounds__(kSlThreads) k_sublattice(Rep* __restrict__ rep
     atomicAdd(&s_cnt[3], c_en);
     atomicAdd(&s_cnt[4], c_acc);
     __syncthreads();
-    for (uint32_t i = tid; i < n_padded; i += kSlThreads) R.sigma[i] = sig[i];
+    for (uint32_t i = tid; i < n_padded; i += kSlThreads) R.sigma[i] = sig[i] & kIdMask;
     for (uint32_t i = tid; i < n_cells; i += kSlThreads) {
         R.volume[i] = vol[i];
         R.interface[i] = itf[i];
@@ -1214,6 +1291,10 @@ struct Device {
     uint32_t lds_bytes = 0;
 };
 
+// LDS kept free of the dynamic allocation for the kernels' static LDS
+// (the sublattice kernel's counters and the compiler's own use)
+constexpr uint32_t kLdsStaticReserve = 1024;
+
 struct Engine {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -1267,7 +1348,8 @@ Engine* engine_create(int device) {
                     (const void*)k_lds_engine<3, 6>, (const void*)k_lds_engine<3, 18>,
                     (const void*)k_lds_engine<3, 26>, (const void*)k_sublattice<2>,
                     (const void*)k_sublattice<3>})
-        HIP_OK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds - 256));
+        HIP_OK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   lds - kLdsStaticReserve));
     return e;
 }
 
@@ -1367,12 +1449,12 @@ static bool lds_eligible(const Engine* e, const Model& m, uint32_t* bytes) {
     const LdsLayout Ly =
         lds_layout((uint32_t)m.n_padded, (uint32_t)m.n_cells, (uint32_t)m.cfg.types.size());
     *bytes = Ly.total;
-    return Ly.total + 256 <= e->max_lds;  // room for the kernels' static LDS
+    return Ly.total + kLdsStaticReserve <= e->max_lds;
 }
 
 // LDS bytes a launch reserves for replicas needing `bytes` (grouping key).
 static uint32_t lds_class(const Engine* e, uint32_t bytes) {
-    return std::min((bytes + 4095) / 4096 * 4096, e->max_lds - 256);
+    return std::min((bytes + 4095) / 4096 * 4096, e->max_lds - kLdsStaticReserve);
 }
 
 static void upload(Engine* e, Model* m) {
@@ -1695,7 +1777,9 @@ static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint
         for (size_t i = 0; i < n; ++i) {
             Model* m = models[i];
             upload(e, m);
-            if (!m->dev->lds)
+            const LdsLayout Ly = lds_layout((uint32_t)m->n_padded, (uint32_t)m->n_cells,
+                                            (uint32_t)m->cfg.types.size());
+            if (!m->dev->lds || sl_lds_bytes(Ly, (uint32_t)m->n_cells) + kLdsStaticReserve > e->max_lds)
                 throw Error(CPM_ERR_ARG,
                             "sublattice mode needs the lattice resident in LDS; this lattice "
                             "is too large");
@@ -1722,7 +1806,10 @@ static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint
                 // one draw of the replica's own stream keys this call's
                 // counter-based randomness
                 R.fill_state[0] = xo_next(m->rng);
-                classes[{m->D, lds_class(e, m->dev->lds_bytes)}].push_back((uint32_t)i);
+                const LdsLayout Ly = lds_layout((uint32_t)m->n_padded, (uint32_t)m->n_cells,
+                                                (uint32_t)m->cfg.types.size());
+                classes[{m->D, lds_class(e, sl_lds_bytes(Ly, (uint32_t)m->n_cells))}].push_back(
+                    (uint32_t)i);
             }
             reps[i] = R;
         }
