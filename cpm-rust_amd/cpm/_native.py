@@ -27,6 +27,7 @@ NB_DEFAULT = -1
 ACCEPT = {"metropolis": 0, "metropolis_hastings": 1}
 PROPOSAL = {"uniform": 0, "edge_list": 1}
 MODE = {"exact_replay": 0, "sublattice": 1}
+CPM_REQ_QUEUED, CPM_REQ_ACTIVE, CPM_REQ_DONE, CPM_REQ_FAILED = range(4)
 
 u32p = C.POINTER(C.c_uint32)
 u64p = C.POINTER(C.c_uint64)
@@ -57,6 +58,28 @@ class CpmCounters(C.Structure):
                 ("energy", C.c_uint64), ("accepted", C.c_uint64),
                 ("exp_near_ties", C.c_uint64), ("kernel_ms", C.c_double),
                 ("replay_launches", C.c_uint64), ("replay_launch_ms", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class CpmServerOptions(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_active", C.c_uint32),
+                ("quantum_attempts", C.c_uint64)]
+
+
+class CpmRequestInfo(C.Structure):
+    _fields_ = [("state", C.c_int32), ("submit_s", C.c_double), ("admit_s", C.c_double),
+                ("done_s", C.c_double), ("footprint_bytes", C.c_uint64),
+                ("mcs_done", C.c_uint64), ("mcs_total", C.c_uint64)]
+
+
+class CpmServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("queued", C.c_uint64), ("active", C.c_uint64),
+                ("hbm_budget_bytes", C.c_uint64), ("hbm_in_use_bytes", C.c_uint64),
+                ("rounds", C.c_uint64), ("attempts", C.c_uint64), ("busy_s", C.c_double),
+                ("uptime_s", C.c_double)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -155,6 +178,17 @@ def lib():
         L.cpm_derive_seed_u64.restype = C.c_uint64
         L.cpm_engine_sp_tables.argtypes = [vp, u8p, u64p]
         L.cpm_engine_sp_tables.restype = st
+        L.cpm_serve_create.argtypes = [vp, C.POINTER(CpmServerOptions), C.POINTER(vp)]
+        L.cpm_serve_create.restype = st
+        L.cpm_serve_destroy.argtypes = [vp]
+        L.cpm_serve_submit.argtypes = [vp, C.POINTER(CpmConfig), C.c_int32, C.c_int32,
+                                       C.POINTER(C.c_uint64), C.POINTER(C.c_int32)]
+        L.cpm_serve_submit.restype = st
+        L.cpm_serve_poll.argtypes = [vp, C.c_uint64, C.c_double, C.POINTER(CpmRequestInfo),
+                                     C.POINTER(vp)]
+        L.cpm_serve_poll.restype = st
+        L.cpm_serve_stats.argtypes = [vp, C.POINTER(CpmServerStats)]
+        L.cpm_serve_stats.restype = st
         _lib = L
         return L
 

@@ -5,13 +5,16 @@
 #include <algorithm>
 #include <atomic>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <thread>
 #include <vector>
 
 #include "../../include/cpm_engine.h"
 #include "device.hpp"
+#include "driver.hpp"
 #include "host.hpp"
+#include "serve.hpp"
 
 using namespace cpm;
 
@@ -23,6 +26,9 @@ struct cpm_model {
 };
 struct cpm_result {
     Result r;
+};
+struct cpm_server {
+    Server* s;
 };
 
 namespace {
@@ -47,55 +53,6 @@ cpm_status guarded(F&& f) {
 
 void need(bool ok, const char* what) {
     if (!ok) throw Error(CPM_ERR_ARG, std::string("invalid argument: ") + what);
-}
-
-// Runs f(i) for i in [0, n) over the host's cores.
-template <class F>
-void parallel_for(size_t n, F&& f) {
-    size_t nt = std::min<size_t>(std::max(1u, std::thread::hardware_concurrency()), 16);
-    nt = std::min(nt, n);
-    if (nt <= 1) {
-        for (size_t i = 0; i < n; ++i) f(i);
-        return;
-    }
-    std::atomic<size_t> next{0};
-    std::vector<std::exception_ptr> errs(nt);
-    std::vector<std::thread> ts;
-    for (size_t t = 0; t < nt; ++t)
-        ts.emplace_back([&, t] {
-            try {
-                for (size_t i; (i = next.fetch_add(1)) < n;) f(i);
-            } catch (...) {
-                errs[t] = std::current_exception();
-            }
-        });
-    for (auto& t : ts) t.join();
-    for (auto& e : errs)
-        if (e) std::rethrow_exception(e);
-}
-
-// One step of a replica's driver schedule: advance `mcs`, then either push
-// the energy onto the burn-in trace or take a readout sample.
-struct Step {
-    uint64_t mcs;
-    bool trace, sample;
-};
-
-// output.rs:441-468
-std::vector<Step> schedule(const Config& c) {
-    std::vector<Step> s;
-    const uint64_t burn = c.burn_in, interval = c.interval;
-    const uint64_t window = (uint64_t)((double)burn * 0.25);
-    const uint64_t start = burn > window ? burn - window : 0;
-    uint64_t burned = 0;
-    while (burned < burn) {
-        uint64_t chunk = std::min(interval, burn - burned);
-        burned += chunk;
-        s.push_back({chunk, burned >= start, false});
-    }
-    const uint64_t n = std::max<uint64_t>(c.readout / std::max<uint64_t>(interval, 1), 1);
-    for (uint64_t i = 0; i < n; ++i) s.push_back({interval, false, true});
-    return s;
 }
 
 // Drives many initialised models through their run_with schedules in
@@ -410,4 +367,75 @@ cpm_status cpm_engine_sp_tables(cpm_engine* engine, uint8_t* table2d, uint64_t* 
     });
 }
 
+
+cpm_status cpm_serve_create(cpm_engine* engine, const cpm_server_options* options,
+                            cpm_server** out) {
+    return guarded([&] {
+        need(engine && out, "engine/out");
+        ServeOptions o;
+        if (options) {
+            o.hbm_budget = options->hbm_budget_bytes;
+            o.max_active = options->max_active;
+            o.quantum = options->quantum_attempts;
+        }
+        *out = new cpm_server{serve_create(engine->e, o)};
+    });
+}
+
+void cpm_serve_destroy(cpm_server* server) {
+    if (!server) return;
+    serve_destroy(server->s);
+    delete server;
+}
+
+cpm_status cpm_serve_submit(cpm_server* server, const cpm_config* config,
+                            int32_t include_lattice, int32_t mode, uint64_t* request_id,
+                            int32_t* used_default) {
+    return guarded([&] {
+        need(server && config && request_id, "server/config/request_id");
+        auto m = std::make_unique<Model>(resolve(*config));
+        m->initialize();
+        if (used_default) *used_default = m->used_default ? 1 : 0;
+        *request_id = serve_submit(server->s, m.release(), include_lattice != 0, mode);
+    });
+}
+
+cpm_status cpm_serve_poll(cpm_server* server, uint64_t request_id, double timeout_s,
+                          cpm_request_info* info, cpm_result** result) {
+    return guarded([&] {
+        need(server, "server");
+        if (result) *result = nullptr;
+        RequestInfo ri;
+        Result r;
+        const bool done = serve_poll(server->s, request_id, timeout_s, ri, &r);
+        if (info) {
+            info->state = ri.state;
+            info->submit_s = ri.submit_s;
+            info->admit_s = ri.admit_s;
+            info->done_s = ri.done_s;
+            info->footprint_bytes = ri.footprint;
+            info->mcs_done = ri.mcs_done;
+            info->mcs_total = ri.mcs_total;
+        }
+        if (done && result) *result = new cpm_result{std::move(r)};
+    });
+}
+
+cpm_status cpm_serve_stats(cpm_server* server, cpm_server_stats* out) {
+    return guarded([&] {
+        need(server && out, "server/out");
+        const ServerStats st = serve_stats(server->s);
+        out->submitted = st.submitted;
+        out->completed = st.completed;
+        out->failed = st.failed;
+        out->queued = st.queued;
+        out->active = st.active;
+        out->hbm_budget_bytes = st.hbm_budget;
+        out->hbm_in_use_bytes = st.hbm_in_use;
+        out->rounds = st.rounds;
+        out->attempts = st.attempts;
+        out->busy_s = st.busy_s;
+        out->uptime_s = st.uptime_s;
+    });
+}
 }  // extern "C"

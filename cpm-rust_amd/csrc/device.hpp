@@ -32,4 +32,12 @@ void device_release(Model* m);
 
 void engine_sp_tables(Engine* e, uint8_t* t2, uint64_t* t3);
 
+// Device bytes a model occupies once uploaded.
+size_t device_footprint(Engine* e, const Model& m);
+// Whether sublattice mode can run the model (LDS-resident lattice, even
+// periodic sides).
+bool sublattice_supported(Engine* e, const Model& m);
+// Free device memory now.
+size_t device_free_bytes(Engine* e);
+
 }  // namespace cpm

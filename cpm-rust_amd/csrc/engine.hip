@@ -1544,6 +1544,30 @@ static void upload(Engine* e, Model* m) {
     m->device_dirty = false;
 }
 
+size_t device_footprint(Engine* e, const Model& m) {
+    uint32_t bytes = 0;
+    return layout_of(m, lds_eligible(e, m, &bytes)).total;
+}
+
+bool sublattice_supported(Engine* e, const Model& m) {
+    uint32_t bytes = 0;
+    if (!lds_eligible(e, m, &bytes)) return false;
+    const LdsLayout Ly = lds_layout((uint32_t)m.n_padded, (uint32_t)m.n_cells,
+                                    (uint32_t)m.cfg.types.size());
+    if (sl_lds_bytes(Ly, (uint32_t)m.n_cells) + kLdsStaticReserve > e->max_lds) return false;
+    if (m.cfg.boundary == CPM_BOUNDARY_PERIODIC)
+        for (int d = 0; d < m.D; ++d)
+            if (m.dims[d] % 2) return false;
+    return true;
+}
+
+size_t device_free_bytes(Engine* e) {
+    HIP_OK(hipSetDevice(e->device));
+    size_t free_b = 0, total_b = 0;
+    HIP_OK(hipMemGetInfo(&free_b, &total_b));
+    return free_b;
+}
+
 void device_release(Model* m) {
     if (!m || !m->dev) return;
     (void)hipFree(m->dev->base);
@@ -1803,9 +1827,13 @@ static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint
             R.done = todo[i] == 0;
             for (int k = 0; k < 6; ++k) R.cnt[k] = 0;
             if (todo[i]) {
-                // one draw of the replica's own stream keys this call's
-                // counter-based randomness
-                R.fill_state[0] = xo_next(m->rng);
+                // one draw of the replica's own stream keys its
+                // counter-based randomness for every later call
+                if (!m->has_sublattice_key) {
+                    m->sublattice_key = xo_next(m->rng);
+                    m->has_sublattice_key = true;
+                }
+                R.fill_state[0] = m->sublattice_key;
                 const LdsLayout Ly = lds_layout((uint32_t)m->n_padded, (uint32_t)m->n_cells,
                                                 (uint32_t)m->cfg.types.size());
                 classes[{m->D, lds_class(e, sl_lds_bytes(Ly, (uint32_t)m->n_cells))}].push_back(

@@ -70,6 +70,11 @@ struct Model {
     Xoshiro rng{};
     uint64_t mcs = 0;
     bool initialized = false, used_default = false;
+    // sublattice mode: key of its counter-based randomness, drawn once from
+    // `rng` on first use; draws are indexed by absolute MCS, so a run gives
+    // the same result however its MCS are split into calls
+    uint64_t sublattice_key = 0;
+    bool has_sublattice_key = false;
 
     // the device copy is authoritative while `device_dirty` (host arrays
     // stale) — see engine.hip

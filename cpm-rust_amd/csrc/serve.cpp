@@ -1,0 +1,268 @@
+// Serving loop (see serve.hpp).  One worker thread owns the device work:
+//
+//   admit:  queued requests, in arrival order, while the HBM budget and the
+//           active-request limit allow;
+//   round:  every active request advances by min(quantum / n_sites, MCS
+//           left in its current schedule step) MCS, all of them in one
+//           batched engine call per mode;
+//   finish: requests whose step completed take their trace point or
+//           readout sample; requests whose schedule is complete get their
+//           RunResult, release their device memory and wake any waiter.
+//
+// A request's result is independent of the batch it runs in and of how
+// its MCS are split into rounds: exact replay reproduces the reference's
+// serial stream whatever the chunking, and sublattice randomness is keyed
+// per model and indexed by absolute MCS.
+#include "serve.hpp"
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+#include "driver.hpp"
+
+namespace cpm {
+
+namespace {
+
+struct Request {
+    uint64_t id = 0;
+    std::unique_ptr<Model> m;
+    int mode = CPM_MODE_EXACT_REPLAY;
+    bool include_lattice = false;
+    std::vector<Step> sch;
+    size_t step = 0;
+    uint64_t in_step = 0;  // MCS done within sch[step]
+    std::vector<double> trace;
+    Result result;
+    RequestInfo info;
+};
+
+using Clock = std::chrono::steady_clock;
+
+}  // namespace
+
+struct Server {
+    Engine* e = nullptr;
+    ServeOptions opt;
+    Clock::time_point t0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request*> queue;
+    std::vector<Request*> active;  // touched by the worker only
+    std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
+    uint64_t next_id = 1;
+    uint64_t in_use = 0;
+    bool stop = false;
+    ServerStats st;
+    std::thread worker;
+
+    double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+    void loop();
+    void round();
+};
+
+void Server::loop() {
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_work.wait(lk, [&] { return stop || !queue.empty() || !active.empty(); });
+            if (stop) return;
+            // admission in arrival order; a request that does not fit waits
+            // for memory to free up (it fits the budget on its own)
+            while (!queue.empty() && active.size() < opt.max_active) {
+                Request* r = queue.front();
+                if (in_use + r->info.footprint > opt.hbm_budget) break;
+                queue.pop_front();
+                in_use += r->info.footprint;
+                r->info.state = CPM_REQ_ACTIVE;
+                r->info.admit_s = now();
+                active.push_back(r);
+            }
+            st.queued = queue.size();
+            st.active = active.size();
+            st.hbm_in_use = in_use;
+        }
+        if (!active.empty()) round();
+    }
+}
+
+void Server::round() {
+    const double t_start = now();
+    const size_t n = active.size();
+    std::vector<uint64_t> todo(n);
+    for (size_t i = 0; i < n; ++i) {
+        Request* r = active[i];
+        const uint64_t per = std::max<uint64_t>(1, opt.quantum / std::max<uint64_t>(1, r->m->n_sites));
+        todo[i] = std::min(per, r->sch[r->step].mcs - r->in_step);
+    }
+    cpm_counters cnt{};
+    std::string failure;
+    try {
+        for (int mode : {CPM_MODE_EXACT_REPLAY, CPM_MODE_SUBLATTICE}) {
+            std::vector<Model*> ms;
+            std::vector<uint64_t> td;
+            for (size_t i = 0; i < n; ++i)
+                if (active[i]->mode == mode) {
+                    ms.push_back(active[i]->m.get());
+                    td.push_back(todo[i]);
+                }
+            if (!ms.empty()) run_mcs_each(e, ms.data(), ms.size(), td.data(), mode, &cnt);
+        }
+    } catch (const std::exception& ex) {
+        failure = ex.what();
+    }
+    std::vector<Request*> finished, sampled;
+    if (failure.empty()) {
+        for (size_t i = 0; i < n; ++i) {
+            Request* r = active[i];
+            r->in_step += todo[i];
+            r->info.mcs_done += todo[i];
+            if (r->in_step < r->sch[r->step].mcs) continue;
+            const Step& s = r->sch[r->step];
+            if (s.trace) r->trace.push_back(r->m->energy);
+            if (s.sample) {
+                sync_to_host(r->m.get());
+                sampled.push_back(r);
+            }
+            r->in_step = 0;
+            if (++r->step == r->sch.size()) finished.push_back(r);
+        }
+        try {
+            parallel_for(sampled.size(), [&](size_t j) {
+                Request* r = sampled[j];
+                r->result.samples.push_back(take_sample(*r->m, r->include_lattice));
+            });
+            parallel_for(finished.size(), [&](size_t j) {
+                Request* r = finished[j];
+                sync_to_host(r->m.get());
+                finish_result(*r->m, r->trace, r->result);
+                device_release(r->m.get());
+            });
+        } catch (const std::exception& ex) {
+            failure = ex.what();
+        }
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    const double t_end = now();
+    st.rounds += 1;
+    st.attempts += cnt.attempts;
+    st.busy_s += t_end - t_start;
+    if (!failure.empty()) finished = active;  // every active request fails
+    for (Request* r : finished) {
+        if (failure.empty()) {
+            r->info.state = CPM_REQ_DONE;
+            st.completed += 1;
+        } else {
+            r->info.state = CPM_REQ_FAILED;
+            r->info.error = failure;
+            device_release(r->m.get());
+            st.failed += 1;
+        }
+        r->info.done_s = t_end;
+        r->m.reset();
+        in_use -= r->info.footprint;
+    }
+    std::vector<Request*> still;
+    for (Request* r : active)
+        if (r->info.state == CPM_REQ_ACTIVE) still.push_back(r);
+    active.swap(still);
+    st.active = active.size();
+    st.hbm_in_use = in_use;
+    if (!finished.empty()) cv_done.notify_all();
+}
+
+Server* serve_create(Engine* e, const ServeOptions& o) {
+    auto* s = new Server();
+    s->e = e;
+    s->opt = o;
+    if (!s->opt.hbm_budget) s->opt.hbm_budget = (uint64_t)(0.8 * (double)device_free_bytes(e));
+    if (!s->opt.max_active) s->opt.max_active = 4096;
+    if (!s->opt.quantum) s->opt.quantum = 1ull << 22;
+    s->st.hbm_budget = s->opt.hbm_budget;
+    s->t0 = Clock::now();
+    s->worker = std::thread([s] { s->loop(); });
+    return s;
+}
+
+void serve_destroy(Server* s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    s->cv_done.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    for (Request* r : s->active) device_release(r->m.get());
+    delete s;
+}
+
+uint64_t serve_submit(Server* s, Model* m, bool include_lattice, int mode) {
+    std::unique_ptr<Model> owned(m);
+    if (mode != CPM_MODE_EXACT_REPLAY && mode != CPM_MODE_SUBLATTICE)
+        throw Error(CPM_ERR_ARG, "unknown mode");
+    if (mode == CPM_MODE_SUBLATTICE && !sublattice_supported(s->e, *m))
+        throw Error(CPM_ERR_ARG,
+                    "sublattice mode needs an LDS-resident lattice with even periodic sides");
+    auto r = std::make_unique<Request>();
+    r->sch = schedule(m->cfg);
+    r->info.footprint = device_footprint(s->e, *m);
+    for (const Step& st : r->sch) r->info.mcs_total += st.mcs;
+    if (r->info.footprint > s->opt.hbm_budget)
+        throw Error(CPM_ERR_OOM, "request needs " + std::to_string(r->info.footprint) +
+                                     " bytes of device memory, above the server's budget of " +
+                                     std::to_string(s->opt.hbm_budget));
+    r->m = std::move(owned);
+    r->mode = mode;
+    r->include_lattice = include_lattice;
+    uint64_t id;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stop) throw Error(CPM_ERR_ARG, "server is shut down");
+        id = r->id = s->next_id++;
+        r->info.state = CPM_REQ_QUEUED;
+        r->info.submit_s = s->now();
+        s->queue.push_back(r.get());
+        s->reqs.emplace(id, std::move(r));
+        s->st.submitted += 1;
+        s->st.queued = s->queue.size();
+    }
+    s->cv_work.notify_one();
+    return id;
+}
+
+bool serve_poll(Server* s, uint64_t id, double timeout_s, RequestInfo& info, Result* out) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) throw Error(CPM_ERR_ARG, "unknown request id " + std::to_string(id));
+    Request* r = it->second.get();
+    auto finished = [&] {
+        return s->stop || r->info.state == CPM_REQ_DONE || r->info.state == CPM_REQ_FAILED;
+    };
+    if (timeout_s > 0 && !finished())
+        s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), finished);
+    info = r->info;
+    if (r->info.state == CPM_REQ_FAILED) {
+        const std::string msg = r->info.error;
+        s->reqs.erase(it);
+        throw Error(CPM_ERR_DEVICE, msg);
+    }
+    if (r->info.state != CPM_REQ_DONE) return false;
+    if (out) *out = std::move(r->result);
+    s->reqs.erase(it);
+    return true;
+}
+
+ServerStats serve_stats(Server* s) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    ServerStats st = s->st;
+    st.uptime_s = s->now();
+    return st;
+}
+
+}  // namespace cpm

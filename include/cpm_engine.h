@@ -250,6 +250,55 @@ uint64_t cpm_derive_seed_u64(uint64_t master_seed, uint64_t index);
    bytes; 3D: 2^20 little-endian 64-bit words), for verification. */
 cpm_status cpm_engine_sp_tables(cpm_engine* engine, uint8_t* table2d, uint64_t* table3d_words);
 
+/*
+ * Serving.  No counterpart in the reference, which runs one configuration
+ * per call; each request here is one output::run_with
+ * (crates/cpm-core/src/output.rs:434-529) and returns the same RunResult.
+ * A server owns a worker thread that admits queued requests in arrival
+ * order while their device footprint fits the HBM budget, and advances all
+ * admitted requests together in rounds (continuous batching): each round
+ * gives every request up to quantum_attempts attempts (whole MCS, never
+ * past its next sample), so requests that arrive while others run join at
+ * the next round and leave when done.  A request's result is the same
+ * alone and in any mix: exact replay follows the reference's stream
+ * whatever the chunking, and sublattice randomness is indexed by absolute
+ * MCS.
+ */
+typedef struct cpm_server cpm_server;
+typedef struct {
+    uint64_t hbm_budget_bytes; /* 0: 80 % of the device's free memory      */
+    uint32_t max_active;       /* requests on the device at once; 0: 4096  */
+    uint64_t quantum_attempts; /* per request per round; 0: 2^22           */
+} cpm_server_options;
+enum { CPM_REQ_QUEUED = 0, CPM_REQ_ACTIVE = 1, CPM_REQ_DONE = 2, CPM_REQ_FAILED = 3 };
+typedef struct {
+    int32_t state;                    /* CPM_REQ_*                          */
+    double submit_s, admit_s, done_s; /* seconds since the server started   */
+    uint64_t footprint_bytes, mcs_done, mcs_total;
+} cpm_request_info;
+typedef struct {
+    uint64_t submitted, completed, failed, queued, active;
+    uint64_t hbm_budget_bytes, hbm_in_use_bytes;
+    uint64_t rounds, attempts;
+    double busy_s, uptime_s;
+} cpm_server_stats;
+cpm_status cpm_serve_create(cpm_engine* engine, const cpm_server_options* options,
+                            cpm_server** out);
+/* Stops the worker after its current round; unfinished requests are
+   dropped. */
+void cpm_serve_destroy(cpm_server* server);
+/* Validates and initialises the configuration in the caller's thread
+   (errors as cpm_model_create / cpm_model_initialize), then queues it. */
+cpm_status cpm_serve_submit(cpm_server* server, const cpm_config* config,
+                            int32_t include_lattice, int32_t mode, uint64_t* request_id,
+                            int32_t* used_default);
+/* Waits up to timeout_s (0: returns at once).  *result is set (and the
+   request forgotten) once it is done, else NULL; a failed request returns
+   its error (and is forgotten). */
+cpm_status cpm_serve_poll(cpm_server* server, uint64_t request_id, double timeout_s,
+                          cpm_request_info* info, cpm_result** result);
+cpm_status cpm_serve_stats(cpm_server* server, cpm_server_stats* out);
+
 #ifdef __cplusplus
 }
 #endif
