@@ -111,6 +111,79 @@ def copy_calibration(mib: int = 256, reps: int = 4):
     print(f"[bench] copy calibration: {reps} device copies of {mib} MiB", flush=True)
 
 
+def serving_trace(cpm, n_requests: int = 1500, rate: float = 200.0, seed: int = 7,
+                  quantum: int = 1 << 22, max_active: int = 4096):
+    """The serving workload: a Poisson stream (mean `rate` requests/s) of
+    mixed run requests — 70 % 2D periodic lattices of side U{64..256}, 30 %
+    3D periodic lattices of side U{32..40} (the LDS-resident range) — with
+    2-4 cell types, V* = 40, I* = 30 (2D) / 120 (3D), lambda_V = 5,
+    lambda_I = 1, J: medium row 16, type pairs U[2,14], density 0.8 (2D) /
+    0.5 (3D), and U[40,160] MCS each (readout, one sample per 20 MCS).
+    Requests are initialised on 16 host threads as they arrive and queued
+    on one server; reported: completion latency (server queue + device +
+    readouts) p50/p99, sims/hour and copy attempts/s over the trace."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    rng = np.random.default_rng(seed)
+    specs, t = [], 0.0
+    for i in range(n_requests):
+        t += rng.exponential(1.0 / rate)
+        if rng.random() < 0.7:
+            side = int(rng.integers(64, 257))
+            grid, it, phi = (side, side), 30, 0.8
+        else:
+            side = int(rng.integers(32, 41))
+            grid, it, phi = (side, side, side), 120, 0.5
+        T = int(rng.integers(2, 5))
+        J = np.zeros((T + 1, T + 1))
+        J[0, 1:] = J[1:, 0] = 16.0
+        for a in range(1, T + 1):
+            for b in range(a, T + 1):
+                J[a, b] = J[b, a] = float(rng.uniform(2, 14))
+        mcs = int(rng.integers(40, 161)) // 20 * 20
+        specs.append(dict(at=t, grid=grid, T=T, J=J.tolist(), it=it, phi=phi, mcs=mcs,
+                          seed=int(cpm.derive_seed_u64(seed, i))))
+
+    def build(sp):
+        sim = cpm.CPM(grid=sp["grid"], boundary="periodic", seed=sp["seed"])
+        n = cpm.cells_for_density(sp["grid"], 40, sp["phi"])
+        for k in range(sp["T"]):
+            sim.register_cell_type(f"t{k}", 40, sp["it"], 5.0, 1.0)
+            sim.add_cells(f"t{k}", n // sp["T"] + (1 if k < n % sp["T"] else 0))
+        sim.set_adhesion(sp["J"])
+        return sim
+
+    import warnings as _w
+    with _w.catch_warnings():
+        _w.simplefilter("ignore")
+        sims = [build(sp) for sp in specs]
+    with cpm.Server(quantum_attempts=quantum, max_active=max_active) as srv, \
+            ThreadPoolExecutor(16) as pool:
+        t0 = time.perf_counter()
+        futs = []
+        for sp, sim in zip(specs, sims):
+            delay = sp["at"] - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            futs.append(pool.submit(lambda s=sim, m=sp["mcs"]: srv.submit(
+                s, burn_in_mcs=0, readout_mcs=m, sampling_interval_mcs=20)))
+        reqs = [f.result() for f in futs]
+        for r in reqs:
+            r.result()
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    lat = np.array([r.latency_s for r in reqs])
+    return {"requests": n_requests, "offered_rate_per_s": rate, "wall_s": wall,
+            "latency_p50_s": float(np.percentile(lat, 50)),
+            "latency_p99_s": float(np.percentile(lat, 99)),
+            "sims_per_hour": n_requests * 3600.0 / wall,
+            "attempts_per_s": st["attempts"] / wall, "rounds": st["rounds"],
+            "device_s": st["device_s"], "host_s": st["busy_s"] - st["device_s"],
+            "mode": "exact_replay",
+            "mix": "70% 2D side U{64..256}, 30% 3D side U{32..40}, 2-4 types, "
+                   "U[40,160] MCS per request"}
+
+
 def cpu_baseline(params, seconds: float = 12.0):
     """The oracle (C restatement of the reference loop) on the host cores:
     replicas of the same workload advanced by one sampling interval each,
@@ -156,6 +229,7 @@ def main():
     ap.add_argument("--interval", type=int, default=INTERVAL)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-mcs", type=int, default=20)
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device copy (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -216,6 +290,12 @@ def main():
     step_s = wall / args.steps
     sims_per_hour = len(reps) * 3600.0 / (t_init + 20 * step_s)
 
+    serving = None
+    if not args.no_serving:
+        serving = serving_trace(cpm)
+        print(f"[bench] serving: {serving['requests']} requests in {serving['wall_s']:.1f} s, "
+              f"p50 {serving['latency_p50_s']:.3f} s, p99 {serving['latency_p99_s']:.3f} s",
+              flush=True)
     if args.copy_calibration:
         copy_calibration()
     cpu = None if args.no_cpu else cpu_baseline(params)
@@ -246,6 +326,7 @@ def main():
                      "avg_launch_ms": avg_launch_s * 1e3,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
 

@@ -79,7 +79,7 @@ class CpmServerStats(C.Structure):
                 ("queued", C.c_uint64), ("active", C.c_uint64),
                 ("hbm_budget_bytes", C.c_uint64), ("hbm_in_use_bytes", C.c_uint64),
                 ("rounds", C.c_uint64), ("attempts", C.c_uint64), ("busy_s", C.c_double),
-                ("uptime_s", C.c_double)]
+                ("device_s", C.c_double), ("uptime_s", C.c_double)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

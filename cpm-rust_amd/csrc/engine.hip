@@ -5,13 +5,14 @@
 // execution paths share the same arithmetic:
 //
 //  * LDS path (lattices whose haloed u16 copy fits in LDS): one wave per
-//    replica.  The replica's Xoshiro256++ stream is produced ahead of time
-//    by k_rng_fill — each replica's stream split into 16 segments started
-//    with GF(2) jump polynomials, so the whole chip generates it — into a
-//    2 MiB HBM buffer.  k_lds_engine keeps sigma (haloed, u16), cell types,
-//    V and I resident in LDS, holds a 128-draw window of the stream in
-//    registers, parses attempts with ballots + find-first-set on the scalar
-//    unit, and evaluates the 8/26-site neighbourhood lane-parallel.
+//    replica, one launch per call.  The wave writes its Xoshiro256++
+//    stream ahead into a 2 MiB HBM buffer — 64 segments of 4096 draws, lane
+//    g jumping to segment g with a GF(2) jump polynomial — and refills it
+//    whenever its cursor reaches the end.  k_lds_engine keeps sigma (haloed,
+//    packed u16), V and I resident in LDS, holds a 128-draw window of the
+//    stream in registers, evaluates the attempt starting at every window
+//    position lane-parallel, and follows the real chain of attempts by
+//    pointer doubling.
 //  * Global path (everything else, e.g. 64^3 lattices): one lane per
 //    replica running the serial loop against HBM-resident state.
 //
@@ -43,9 +44,9 @@ namespace cpm {
                         std::string(#expr) + ": " + hipGetErrorString(_e));            \
     } while (0)
 
-constexpr uint32_t kSegLen = 16384;            // draws per stream segment
-constexpr uint32_t kSegs = 16;                 // segments per fill
-constexpr uint32_t kFill = kSegLen * kSegs;    // draws buffered per replica per round
+constexpr uint32_t kSegLen = 4096;             // draws per stream segment (one lane's share)
+constexpr uint32_t kSegs = 64;                 // segments per fill (one per lane)
+constexpr uint32_t kFill = kSegLen * kSegs;    // draws buffered per replica
 
 // ---------------------------------------------------------------------------
 // Replica descriptor: geometry, stencils, parameters, state pointers and the
@@ -65,18 +66,15 @@ struct Rep {
     uint32_t* contact;
     const double* params;     // lambda_v[T], lambda_i[T], J[(T+1)^2]
     const uint32_t* targets;  // V*[T], I*[T]
-    uint64_t* draws;          // kFill buffered draws (LDS path)
+    uint64_t* draws;          // kFill buffered draws + kSegs segment states (LDS path)
     uint64_t rng[4];          // consumer state at the start / end of a call
     double energy;
     uint64_t mcs;
     uint64_t todo;            // MCS left to run in this call
     uint32_t attempt_in_mcs;  // attempts done within the current MCS
-    uint32_t cursor;          // next unconsumed draw in `draws`
     uint32_t done;
-    uint32_t filled;          // the stream buffer was filled in this call
     uint32_t error;           // an attempt needed more draws than a window holds
-    uint64_t fill_state[4];   // state at draws[0]
-    uint64_t seg_state[kSegs][4];
+    uint64_t fill_state[4];   // sublattice mode: [0] keys its randomness
     unsigned long long cnt[6];
 };
 
@@ -287,49 +285,30 @@ __global__ void __launch_bounds__(64) k_exact_replay(Rep* __restrict__ reps,
 // LDS path
 // ===========================================================================
 
-struct JumpTable {
-    uint64_t q[kSegs][4];  // x^(g * kSegLen) mod p(x)
-};
+// x^(g * kSegLen) mod p(x): the jump from a fill's start to segment g
+__constant__ uint64_t c_jump[kSegs][4];
 
-// Fills every unfinished replica's draw buffer with kFill consecutive
-// draws of its stream starting at fill_state.  One lane per (replica,
-// segment): the lane jumps to its segment start and steps kSegLen times.
-__global__ void __launch_bounds__(256) k_rng_fill(Rep* __restrict__ reps, uint32_t n,
-                                                  JumpTable jt) {
-    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t r = tid / kSegs, g = tid % kSegs;
-    if (r >= n) return;
-    Rep& R = reps[r];
-    if (R.done || !R.draws) return;
-    if (g == 0) R.filled = 1;
-    Xoshiro s;
-    for (int k = 0; k < 4; ++k) s.s[k] = R.fill_state[k];
-    if (g) {  // state kSegLen * g draws ahead: sum_i q_i T^i s
+// The wave writes kFill consecutive draws of the stream whose state at the
+// buffer start is s0: lane g jumps to draw g * kSegLen (sum of T^i s0 over
+// the polynomial's terms) and steps through its segment.  Segment start
+// states go to segs[g] for later rebasing.
+__device__ void wave_fill(uint64_t* __restrict__ draws, uint64_t* __restrict__ segs,
+                          const uint64_t* s0, uint32_t lane) {
+    Xoshiro s{{s0[0], s0[1], s0[2], s0[3]}};
+    if (lane) {
         Xoshiro acc{{0, 0, 0, 0}};
         for (int i = 0; i < 256; ++i) {
-            if ((jt.q[g][i >> 6] >> (i & 63)) & 1)
+            if ((c_jump[lane][i >> 6] >> (i & 63)) & 1)
                 for (int k = 0; k < 4; ++k) acc.s[k] ^= s.s[k];
             xo_next(s);
         }
         s = acc;
     }
-    for (int k = 0; k < 4; ++k) R.seg_state[g][k] = s.s[k];
-    uint64_t* __restrict__ out = R.draws + (size_t)g * kSegLen;
+    for (int k = 0; k < 4; ++k) segs[lane * 4 + k] = s.s[k];
+    uint64_t* __restrict__ out = draws + (size_t)lane * kSegLen;
     for (uint32_t i = 0; i < kSegLen; ++i) out[i] = xo_next(s);
-}
-
-// Consumer RNG state after a call: the segment state containing the cursor,
-// stepped to the cursor.
-__global__ void __launch_bounds__(64) k_rng_finalize(Rep* __restrict__ reps, uint32_t n) {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    Rep& R = reps[r];
-    if (!R.draws || !R.filled) return;
-    const uint32_t g = min(R.cursor / kSegLen, kSegs - 1);
-    Xoshiro s;
-    for (int k = 0; k < 4; ++k) s.s[k] = R.seg_state[g][k];
-    for (uint32_t i = g * kSegLen; i < R.cursor; ++i) xo_next(s);
-    for (int k = 0; k < 4; ++k) R.rng[k] = s.s[k];
+    __threadfence();  // the wave reads other lanes' segments next
+    __syncthreads();
 }
 
 __device__ __forceinline__ uint32_t rl32(uint32_t v, uint32_t lane) {
@@ -485,7 +464,7 @@ struct EvalMasks {
 // attempt costs about 1/20 of a lane-parallel evaluation.  NE is the
 // energy stencil size; NT = 3^D - 1 the full shell.
 template <int D, int NE>
-__global__ void __launch_bounds__(64) k_lds_engine(Rep* __restrict__ reps,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 ? 4 : 1))) k_lds_engine(Rep* __restrict__ reps,
                                                    const uint32_t* __restrict__ idx,
                                                    const uint64_t* __restrict__ sp3d) {
     constexpr int NT = D == 2 ? 8 : 26;
@@ -560,8 +539,16 @@ __global__ void __launch_bounds__(64) k_lds_engine(Rep* __restrict__ reps,
     uint32_t c_att = 0, c_nn = 0, c_top = 0, c_en = 0, c_acc = 0, c_tie = 0;
     bool dirty = false, error = false;
 
+    // ---- the stream buffer, written by this wave from the call's RNG state
+    uint64_t* const gdraws = R.draws;
+    uint64_t* const segs = R.draws + kFill;  // start state of each segment
+    {
+        const uint64_t s0[4] = {R.rng[0], R.rng[1], R.rng[2], R.rng[3]};
+        wave_fill(gdraws, segs, s0, lane);
+    }
+
     // ---- the 128-draw register window: halves [base, base+64), [base+64, base+128)
-    uint32_t base = R.cursor, p = 0;
+    uint32_t base = 0, p = 0;
     Half h0, h1;
     uint64_t m0[D + 1], m1[D + 1];
     bool h1_valid = false;
@@ -686,7 +673,26 @@ __global__ void __launch_bounds__(64) k_lds_engine(Rep* __restrict__ reps,
         M.tie = uni64(__ballot(ok && tie));                                                  \
     } while (0)
 
-    if (base + 64 > kFill) goto exit_loop;  // no full window left in this fill
+    // The buffer is restarted at the segment holding absolute position `at`
+    // (its draws regenerated from the segment's start state) whenever the
+    // window would run past its end.
+#define CPM_REFILL(at_)                                                                      \
+    do {                                                                                     \
+        const uint32_t at = (at_), g = min(at / kSegLen, kSegs - 1);                          \
+        /* a vector load: the segment states were written by vector stores */                \
+        const uint64_t mine = segs[g * 4 + (lane & 3)];                                       \
+        const uint64_t s0[4] = {rl64(mine, 0), rl64(mine, 1), rl64(mine, 2), rl64(mine, 3)}; \
+        __syncthreads();                                                                     \
+        wave_fill(gdraws, segs, s0, lane);                                                   \
+        base = at - g * kSegLen;                                                             \
+        p = 0;                                                                               \
+        h1_valid = true;                                                                     \
+        prep_half<D>(h0, draws[base + lane], range, zone, m0);                               \
+        prep_half<D>(h1, draws[base + 64 + lane], range, zone, m1);                          \
+        w_next = draws[base + 128 + lane];                                                   \
+        need_decode = true;                                                                  \
+    } while (0)
+
     h1_valid = base + 128 <= kFill;
     prep_half<D>(h0, draws[base + lane], range, zone, m0);
     prep_half<D>(h1, h1_valid ? draws[base + 64 + lane] : 0ull, range, zone, m1);
@@ -842,7 +848,10 @@ __global__ void __launch_bounds__(64) k_lds_engine(Rep* __restrict__ reps,
         p = np;
         if (todo == 0) break;
         if (p >= 64) {  // slide the window by one half
-            if (!h1_valid) break;
+            if (!h1_valid) {
+                CPM_REFILL(base + p);
+                continue;
+            }
             base += 64;
             p -= 64;
             h0 = h1;
@@ -855,14 +864,17 @@ __global__ void __launch_bounds__(64) k_lds_engine(Rep* __restrict__ reps,
             continue;
         }
         // The attempt at p does not fit in the window: restart the window
-        // at p, or stop the round (the next round's window begins here).
+        // at p (refilling the buffer if the window would run past its end).
         // An attempt needing more than 128 draws even from a window of its
         // own is reported, never looped on.
         if (p == 0 && h1_valid) {
             error = true;
             break;
         }
-        if (base + p + 128 > kFill) break;
+        if (base + p + 192 > kFill) {
+            CPM_REFILL(base + p);
+            continue;
+        }
         base += p;
         p = 0;
         h1_valid = true;
@@ -871,12 +883,17 @@ __global__ void __launch_bounds__(64) k_lds_engine(Rep* __restrict__ reps,
         w_next = base + 192 <= kFill ? draws[base + 128 + lane] : 0ull;
         need_decode = true;
     }
-    base += p;  // absolute cursor of the next unconsumed draw
-
-exit_loop:
+    base += p;  // buffer position of the next unconsumed draw
 #undef CPM_DECODE
 #undef CPM_EVALUATE
+#undef CPM_REFILL
     __syncthreads();
+    // the consumer's RNG state: the segment's start state (a vector load:
+    // it was written by vector stores) stepped to the cursor
+    const uint32_t g_end = min(base / kSegLen, kSegs - 1);
+    const uint64_t seg_end = segs[g_end * 4 + (lane & 3)];
+    const uint64_t st_end[4] = {rl64(seg_end, 0), rl64(seg_end, 1), rl64(seg_end, 2),
+                                rl64(seg_end, 3)};
     if (dirty) {
         for (uint32_t i = lane; i < n_padded; i += 64) R.sigma[i] = sig[i] & kIdMask;
         for (uint32_t i = lane; i < n_cells; i += 64) {
@@ -895,18 +912,13 @@ exit_loop:
         R.cnt[3] += c_en;
         R.cnt[4] += c_acc;
         R.cnt[5] += c_tie;
+        R.done = 1;
         if (error) {
             R.error = 1;
-            R.done = 1;
-        } else if (todo == 0) {
-            R.cursor = base;
-            R.done = 1;
         } else {
-            // rebase the stream onto the segment holding the cursor; the
-            // next fill regenerates from that segment's start
-            const uint32_t g = min(base / kSegLen, kSegs - 1);
-            for (int k = 0; k < 4; ++k) R.fill_state[k] = R.seg_state[g][k];
-            R.cursor = base - g * kSegLen;
+            Xoshiro st{{st_end[0], st_end[1], st_end[2], st_end[3]}};
+            for (uint32_t i = g_end * kSegLen; i < base; ++i) xo_next(st);
+            for (int k = 0; k < 4; ++k) R.rng[k] = st.s[k];
         }
     }
 }
@@ -1308,8 +1320,7 @@ struct Engine {
     // these streams, concurrently, after each round's stream fill
     static constexpr int kClassStreams = 4;
     hipStream_t cls_stream[kClassStreams] = {};
-    hipEvent_t cls_done[kClassStreams] = {}, fill_done = nullptr;
-    JumpTable jt{};
+    hipEvent_t cls_done[kClassStreams] = {};
     uint32_t max_lds = 0;
     std::vector<hipEvent_t> launch_ev;  // start/stop pairs around replay launches
     std::mutex mu;
@@ -1329,7 +1340,6 @@ Engine* engine_create(int device) {
         HIP_OK(hipStreamCreateWithFlags(&e->cls_stream[i], hipStreamNonBlocking));
         HIP_OK(hipEventCreateWithFlags(&e->cls_done[i], hipEventDisableTiming));
     }
-    HIP_OK(hipEventCreateWithFlags(&e->fill_done, hipEventDisableTiming));
     uint8_t t2[256];
     sp_table2d(t2);
     uint32_t w[8] = {0};
@@ -1337,9 +1347,13 @@ Engine* engine_create(int device) {
         if (t2[p]) w[p >> 5] |= 1u << (p & 31);
     HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_sp2d), w, sizeof w));
     xo_jump_selfcheck();
-    for (uint32_t g = 0; g < kSegs; ++g) {
-        JumpPoly q = xo_jump_poly((uint64_t)g * kSegLen);
-        std::memcpy(e->jt.q[g], q.w, 32);
+    {
+        uint64_t jump[kSegs][4];
+        for (uint32_t g = 0; g < kSegs; ++g) {
+            JumpPoly q = xo_jump_poly((uint64_t)g * kSegLen);
+            std::memcpy(jump[g], q.w, 32);
+        }
+        HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_jump), jump, sizeof jump));
     }
     int lds = 0;
     HIP_OK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
@@ -1393,7 +1407,6 @@ void engine_destroy(Engine* e) {
         if (e->cls_stream[i]) (void)hipStreamDestroy(e->cls_stream[i]);
         if (e->cls_done[i]) (void)hipEventDestroy(e->cls_done[i]);
     }
-    if (e->fill_done) (void)hipEventDestroy(e->fill_done);
     for (hipEvent_t ev : e->launch_ev) (void)hipEventDestroy(ev);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
@@ -1437,7 +1450,7 @@ static Layout layout_of(const Model& m, bool with_draws) {
     L.contact = take((n + 1) * (n + 1) * 4);
     L.params = take((2 * T + (T + 1) * (T + 1)) * 8);
     L.targets = take(2 * T * 4);
-    L.draws = with_draws ? take((size_t)kFill * 8) : 0;
+    L.draws = with_draws ? take((size_t)kFill * 8 + (size_t)kSegs * 32) : 0;
     L.total = off;
     return L;
 }
@@ -1648,37 +1661,98 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
     ensure_buf(e->reps_dev, e->reps_cap, n);
     ensure_buf(e->idx_dev, e->idx_cap, n);
     std::vector<Rep> reps(n);
-    // partition: global-path replicas, and LDS classes keyed by (D, LDS bytes)
+    // partition: global-path replicas, and LDS replicas by kernel
+    // instantiation (100 * D + energy stencil size)
     std::vector<uint32_t> order, global_idx;
-    std::map<std::pair<int, uint32_t>, std::vector<uint32_t>> classes;
+    std::map<int, std::vector<uint32_t>> by_kernel;
     for (size_t i = 0; i < n; ++i) {
         Model* m = models[i];
         Rep R = m->dev->rep;
-        for (int k = 0; k < 4; ++k) R.rng[k] = R.fill_state[k] = m->rng.s[k];
+        for (int k = 0; k < 4; ++k) R.rng[k] = m->rng.s[k];
         R.energy = m->energy;
         R.mcs = m->mcs;
         R.todo = todo[i];
         R.attempt_in_mcs = 0;
-        R.cursor = 0;
         R.done = todo[i] == 0;
-        R.filled = 0;
         R.error = 0;
         for (int k = 0; k < 6; ++k) R.cnt[k] = 0;
         reps[i] = R;
         if (!todo[i]) continue;
         if (m->dev->lds)
-            classes[{m->D * 100 + m->n_energy, lds_class(e, m->dev->lds_bytes)}].push_back((uint32_t)i);
+            by_kernel[m->D * 100 + m->n_energy].push_back((uint32_t)i);
         else
             global_idx.push_back((uint32_t)i);
     }
+    // One launch per group, the groups of all kernels running concurrently
+    // on the class streams.  A launch's LDS allocation is its largest
+    // member's, so each kernel's replicas are sorted by footprint class and
+    // cut into contiguous groups (as many as the streams allow, shared
+    // between kernels in proportion to their replica counts) that minimise
+    // the LDS allocated in total.
     struct Launch {
         int D;
         uint32_t lds, first, count;
     };
     std::vector<Launch> launches;
-    for (auto& [key, v] : classes) {
-        launches.push_back({key.first, key.second, (uint32_t)order.size(), (uint32_t)v.size()});
-        order.insert(order.end(), v.begin(), v.end());
+    {
+        size_t n_lds = 0;
+        for (auto& kv : by_kernel) n_lds += kv.second.size();
+        const size_t n_kernels = by_kernel.size();
+        const size_t spare =
+            Engine::kClassStreams > n_kernels ? Engine::kClassStreams - n_kernels : 0;
+        for (auto& [key, v] : by_kernel) {
+            std::vector<std::pair<uint32_t, uint32_t>> cls(v.size());  // (class bytes, replica)
+            for (size_t j = 0; j < v.size(); ++j)
+                cls[j] = {lds_class(e, models[v[j]]->dev->lds_bytes), v[j]};
+            std::stable_sort(cls.begin(), cls.end(),
+                             [](auto& a, auto& b) { return a.first < b.first; });
+            // distinct classes with counts
+            std::vector<uint32_t> size_k;
+            std::vector<size_t> count_k, start_k;
+            for (size_t j = 0; j < cls.size(); ++j) {
+                if (size_k.empty() || cls[j].first != size_k.back()) {
+                    size_k.push_back(cls[j].first);
+                    count_k.push_back(0);
+                    start_k.push_back(j);
+                }
+                ++count_k.back();
+            }
+            const size_t K = size_k.size();
+            const size_t G = std::min(K, 1 + (n_lds ? spare * v.size() / n_lds : 0));
+            // best[g][k]: least allocation covering classes [0, k) in g groups
+            const double inf = 1e300;
+            std::vector<std::vector<double>> best(G + 1, std::vector<double>(K + 1, inf));
+            std::vector<std::vector<size_t>> cut(G + 1, std::vector<size_t>(K + 1, 0));
+            best[0][0] = 0.0;
+            for (size_t g = 1; g <= G; ++g)
+                for (size_t k = 1; k <= K; ++k) {
+                    double cnt = 0.0;
+                    for (size_t a = k; a-- > 0;) {  // last group = classes [a, k)
+                        cnt += (double)count_k[a];
+                        const double c = best[g - 1][a] + cnt * (double)size_k[k - 1];
+                        if (c < best[g][k]) {
+                            best[g][k] = c;
+                            cut[g][k] = a;
+                        }
+                    }
+                }
+            size_t g_best = 1;
+            for (size_t g = 1; g <= G; ++g)
+                if (best[g][K] < best[g_best][K]) g_best = g;
+            std::vector<std::pair<size_t, size_t>> groups;  // class ranges
+            for (size_t g = g_best, k = K; g > 0; --g) {
+                groups.push_back({cut[g][k], k});
+                k = cut[g][k];
+            }
+            // largest footprints first: the biggest lattices carry the
+            // longest serial chains, so their workgroups are dispatched first
+            for (auto [a, b] : groups) {
+                const size_t first = start_k[a], last = start_k[b - 1] + count_k[b - 1];
+                launches.push_back({key, size_k[b - 1], (uint32_t)order.size(),
+                                    (uint32_t)(last - first)});
+                for (size_t j = first; j < last; ++j) order.push_back(cls[j].second);
+            }
+        }
     }
     const uint32_t global_first = (uint32_t)order.size();
     order.insert(order.end(), global_idx.begin(), global_idx.end());
@@ -1688,6 +1762,7 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
         HIP_OK(hipMemcpyAsync(e->idx_dev, order.data(), order.size() * 4, hipMemcpyHostToDevice,
                               e->stream));
     HIP_OK(hipEventRecord(e->ev0, e->stream));
+    HIP_OK(hipEventRecord(e->cls_done[0], e->stream));  // descriptors uploaded
     if (!global_idx.empty()) {
         const uint32_t ng = (uint32_t)global_idx.size();
         hipLaunchKernelGGL(k_exact_replay, dim3((ng + 63) / 64), dim3(64), 0, e->stream,
@@ -1697,74 +1772,27 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
     double replay_ms = 0.0;
     uint64_t replay_launches = 0;
     if (!launches.empty()) {
-        // rounds of (fill every unfinished stream, advance every unfinished
-        // replica); a replica is finished once its MCS target is reached
-        std::vector<uint32_t> done(n);
-        uint64_t max_draws = 0;
-        for (auto& l : launches)
-            for (uint32_t j = 0; j < l.count; ++j) {
-                const Model* m = models[order[l.first + j]];
-                max_draws = std::max<uint64_t>(max_draws, todo[order[l.first + j]] * m->n_sites * 8);
-            }
-        uint64_t rounds_guess = max_draws / kFill + 1;
-        // every round consumes at least kFill - kSegLen - 256 draws of each
-        // unfinished replica; bound the loop well above what any run needs
-        const uint64_t max_rounds = 64 + 8 * (max_draws / (kFill - kSegLen - 256) + 1);
-        uint64_t rounds = 0;
-        for (;;) {
-            if (rounds > max_rounds)
-                throw Error(CPM_ERR_DEVICE, "exact-replay rounds did not converge");
-            const uint64_t burst = std::max<uint64_t>(1, std::min<uint64_t>(rounds_guess, 8));
-            rounds += burst;
-            const size_t n_ev = 2 * burst * launches.size();
-            while (e->launch_ev.size() < n_ev) {
-                hipEvent_t ev;
-                HIP_OK(hipEventCreate(&ev));
-                e->launch_ev.push_back(ev);
-            }
-            size_t ev_i = 0;
-            for (uint64_t b = 0; b < burst; ++b) {
-                const uint32_t threads = (uint32_t)n * kSegs;
-                hipLaunchKernelGGL(k_rng_fill, dim3((threads + 255) / 256), dim3(256), 0,
-                                   e->stream, e->reps_dev, (uint32_t)n, e->jt);
-                HIP_OK(hipGetLastError());
-                HIP_OK(hipEventRecord(e->fill_done, e->stream));
-                const int ns = (int)std::min<size_t>(launches.size(), Engine::kClassStreams);
-                for (int k = 0; k < ns; ++k)
-                    HIP_OK(hipStreamWaitEvent(e->cls_stream[k], e->fill_done, 0));
-                for (size_t li = 0; li < launches.size(); ++li) {
-                    const Launch& l = launches[li];
-                    hipStream_t st = e->cls_stream[li % Engine::kClassStreams];
-                    HIP_OK(hipEventRecord(e->launch_ev[ev_i], st));
-                    launch_lds_engine(l.D, l.lds, l.count, st, e->reps_dev, e->idx_dev + l.first,
-                                      e->sp3d);
-                    HIP_OK(hipGetLastError());
-                    HIP_OK(hipEventRecord(e->launch_ev[ev_i + 1], st));
-                    ev_i += 2;
-                }
-                for (int k = 0; k < ns; ++k) {
-                    HIP_OK(hipEventRecord(e->cls_done[k], e->cls_stream[k]));
-                    HIP_OK(hipStreamWaitEvent(e->stream, e->cls_done[k], 0));
-                }
-            }
-            rounds_guess = rounds_guess > burst ? rounds_guess - burst : 1;
-            // gather done flags
-            bool all = true;
-            HIP_OK(hipMemcpyAsync(reps.data(), e->reps_dev, n * sizeof(Rep), hipMemcpyDeviceToHost,
-                                  e->stream));
-            HIP_OK(hipStreamSynchronize(e->stream));
-            for (size_t i = 0; i < ev_i; i += 2) {
-                float lms = 0.f;
-                HIP_OK(hipEventElapsedTime(&lms, e->launch_ev[i], e->launch_ev[i + 1]));
-                replay_ms += lms;
-                ++replay_launches;
-            }
-            for (size_t i = 0; i < n; ++i) all &= reps[i].done != 0;
-            if (all) break;
+        // one launch per group: each wave runs its replica to its MCS
+        // target, refilling its own stream buffer as it goes
+        while (e->launch_ev.size() < 2 * launches.size()) {
+            hipEvent_t ev;
+            HIP_OK(hipEventCreate(&ev));
+            e->launch_ev.push_back(ev);
         }
-        hipLaunchKernelGGL(k_rng_finalize, dim3(((uint32_t)n + 63) / 64), dim3(64), 0, e->stream,
-                           e->reps_dev, (uint32_t)n);
-        HIP_OK(hipGetLastError());
+        const int ns = (int)std::min<size_t>(launches.size(), Engine::kClassStreams);
+        for (int k = 0; k < ns; ++k) HIP_OK(hipStreamWaitEvent(e->cls_stream[k], e->cls_done[0], 0));
+        for (size_t li = 0; li < launches.size(); ++li) {
+            const Launch& l = launches[li];
+            hipStream_t st = e->cls_stream[li % Engine::kClassStreams];
+            HIP_OK(hipEventRecord(e->launch_ev[2 * li], st));
+            launch_lds_engine(l.D, l.lds, l.count, st, e->reps_dev, e->idx_dev + l.first, e->sp3d);
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipEventRecord(e->launch_ev[2 * li + 1], st));
+        }
+        for (int k = 0; k < ns; ++k) {
+            HIP_OK(hipEventRecord(e->cls_done[k], e->cls_stream[k]));
+            HIP_OK(hipStreamWaitEvent(e->stream, e->cls_done[k], 0));
+        }
     }
     HIP_OK(hipEventRecord(e->ev1, e->stream));
     HIP_OK(hipMemcpyAsync(reps.data(), e->reps_dev, n * sizeof(Rep), hipMemcpyDeviceToHost,
@@ -1772,6 +1800,12 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
     HIP_OK(hipStreamSynchronize(e->stream));
     float ms = 0.f;
     HIP_OK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+    for (size_t li = 0; li < launches.size(); ++li) {
+        float lms = 0.f;
+        HIP_OK(hipEventElapsedTime(&lms, e->launch_ev[2 * li], e->launch_ev[2 * li + 1]));
+        replay_ms += lms;
+        ++replay_launches;
+    }
     if (counters) {
         counters->kernel_ms += ms;
         counters->replay_launch_ms += replay_ms;

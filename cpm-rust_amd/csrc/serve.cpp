@@ -116,6 +116,7 @@ void Server::round() {
     } catch (const std::exception& ex) {
         failure = ex.what();
     }
+    const double t_device = now();
     std::vector<Request*> finished, sampled;
     if (failure.empty()) {
         for (size_t i = 0; i < n; ++i) {
@@ -125,16 +126,14 @@ void Server::round() {
             if (r->in_step < r->sch[r->step].mcs) continue;
             const Step& s = r->sch[r->step];
             if (s.trace) r->trace.push_back(r->m->energy);
-            if (s.sample) {
-                sync_to_host(r->m.get());
-                sampled.push_back(r);
-            }
+            if (s.sample) sampled.push_back(r);
             r->in_step = 0;
             if (++r->step == r->sch.size()) finished.push_back(r);
         }
         try {
             parallel_for(sampled.size(), [&](size_t j) {
                 Request* r = sampled[j];
+                sync_to_host(r->m.get());
                 r->result.samples.push_back(take_sample(*r->m, r->include_lattice));
             });
             parallel_for(finished.size(), [&](size_t j) {
@@ -152,6 +151,7 @@ void Server::round() {
     st.rounds += 1;
     st.attempts += cnt.attempts;
     st.busy_s += t_end - t_start;
+    st.device_s += t_device - t_start;
     if (!failure.empty()) finished = active;  // every active request fails
     for (Request* r : finished) {
         if (failure.empty()) {

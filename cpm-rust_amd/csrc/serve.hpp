@@ -30,7 +30,7 @@ struct RequestInfo {
 struct ServerStats {
     uint64_t submitted = 0, completed = 0, failed = 0, queued = 0, active = 0;
     uint64_t hbm_budget = 0, hbm_in_use = 0, rounds = 0, attempts = 0;
-    double busy_s = 0, uptime_s = 0;
+    double busy_s = 0, device_s = 0, uptime_s = 0;
 };
 
 Server* serve_create(Engine* e, const ServeOptions& o);

@@ -280,7 +280,9 @@ typedef struct {
     uint64_t submitted, completed, failed, queued, active;
     uint64_t hbm_budget_bytes, hbm_in_use_bytes;
     uint64_t rounds, attempts;
-    double busy_s, uptime_s;
+    double busy_s;   /* time in rounds                                       */
+    double device_s; /* of which in the batched engine calls                 */
+    double uptime_s;
 } cpm_server_stats;
 cpm_status cpm_serve_create(cpm_engine* engine, const cpm_server_options* options,
                             cpm_server** out);
