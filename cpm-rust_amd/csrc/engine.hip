@@ -66,7 +66,8 @@ struct Rep {
     uint32_t* contact;
     const double* params;     // lambda_v[T], lambda_i[T], J[(T+1)^2]
     const uint32_t* targets;  // V*[T], I*[T]
-    uint64_t* draws;          // kFill buffered draws + kSegs segment states (LDS path)
+    uint64_t* draws;          // kFill buffered draws + kSegs segment states (wave path)
+    uint16_t* sig16;          // packed sigma in HBM (wave path with sigma outside LDS)
     uint64_t rng[4];          // consumer state at the start / end of a call
     double energy;
     uint64_t mcs;
@@ -341,7 +342,8 @@ struct LdsLayout {
     uint32_t sigma, type, vol, itf, J, lv, li, tv, ti, total;
 };
 
-__host__ __device__ inline LdsLayout lds_layout(uint32_t n_padded, uint32_t n_cells, uint32_t T) {
+__host__ __device__ inline LdsLayout lds_layout(uint32_t n_padded, uint32_t n_cells, uint32_t T,
+                                                bool sigma_in_lds = true) {
     LdsLayout L;
     uint32_t off = 0;
     auto take = [&](uint32_t bytes, uint32_t align) {
@@ -357,7 +359,7 @@ __host__ __device__ inline LdsLayout lds_layout(uint32_t n_padded, uint32_t n_ce
     L.itf = take(n_cells * 4 + 4, 4);
     L.tv = take(T * 4, 4);
     L.ti = take(T * 4, 4);
-    L.sigma = take(n_padded * 2, 4);
+    L.sigma = take(sigma_in_lds ? n_padded * 2 : 0, 4);
     L.type = take(n_cells + 1, 4);
     L.total = (off + 15) / 16 * 16;
     return L;
@@ -454,7 +456,23 @@ struct EvalMasks {
     uint64_t ok, null_, acc, top, en, tie;
 };
 
-// Exact replay on the LDS path: one wave per replica.  Each lane decodes
+// Packed sigma words, in LDS or (G) in HBM.  The HBM copy is read through
+// global-address-space loads (flat loads would also count against the LDS
+// wait counter).
+template <bool G>
+struct SigmaRef {
+    uint16_t* p;
+    __device__ uint32_t ld(int64_t i) const { return p[i]; }
+    __device__ void st(int64_t i, uint32_t v) const { p[i] = (uint16_t)v; }
+};
+template <>
+struct SigmaRef<true> {
+    __attribute__((address_space(1))) uint16_t* p;
+    __device__ uint32_t ld(int64_t i) const { return p[i]; }
+    __device__ void st(int64_t i, uint32_t v) const { p[i] = (uint16_t)v; }
+};
+
+// Exact replay on the wave path: one wave per replica.  Each lane decodes
 // the attempt that would start at its own position of the 64-draw window
 // (the first accepted draw of each consumer, found in the 128-draw window
 // by mask search) and evaluates it completely against the current lattice.
@@ -463,7 +481,7 @@ struct EvalMasks {
 // commits it, re-evaluates, and goes on.  Accepted copies are rare, so an
 // attempt costs about 1/20 of a lane-parallel evaluation.  NE is the
 // energy stencil size; NT = 3^D - 1 the full shell.
-template <int D, int NE>
+template <int D, int NE, bool SG>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 ? 4 : 1))) k_lds_engine(Rep* __restrict__ reps,
                                                    const uint32_t* __restrict__ idx,
                                                    const uint64_t* __restrict__ sp3d) {
@@ -474,8 +492,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 
     const uint32_t lane = threadIdx.x;
     const uint32_t T = R.n_types, T1 = T + 1, n_cells = R.n_cells;
     const uint32_t n_padded = R.n_padded, n_sites = R.n_sites;
-    const LdsLayout Ly = lds_layout(n_padded, n_cells, T);
-    uint16_t* sig = (uint16_t*)(smem + Ly.sigma);
+    const LdsLayout Ly = lds_layout(n_padded, n_cells, T, !SG);
+    SigmaRef<SG> sig;
+    if constexpr (SG)
+        sig.p = (__attribute__((address_space(1))) uint16_t*)R.sig16;
+    else
+        sig.p = (uint16_t*)(smem + Ly.sigma);
     uint32_t* vol = (uint32_t*)(smem + Ly.vol);  // vol[id - 1]
     uint32_t* itf = (uint32_t*)(smem + Ly.itf);
     double* J = (double*)(smem + Ly.J);
@@ -488,8 +510,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 
     const uint32_t* __restrict__ gtyp = R.type_index;
     for (uint32_t i = lane; i < n_padded; i += 64) {
         const uint32_t id = R.sigma[i];
-        sig[i] = (uint16_t)(id ? id | ((gtyp[id - 1] + 1) << 12) : 0u);
+        sig.st(i, id ? id | ((gtyp[id - 1] + 1) << 12) : 0u);
     }
+    if constexpr (SG) __threadfence();
     for (uint32_t i = lane; i < n_cells; i += 64) {
         vol[i] = R.volume[i];
         itf[i] = R.interface[i];
@@ -588,8 +611,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 
 #define CPM_EVALUATE()                                                                       \
     do {                                                                                     \
         const uint32_t t = dec_t, e = dec_end;                                               \
-        ev.Lw = sig[t];                                                                      \
-        ev.Gw = sig[dec_s];                                                                  \
+        ev.Lw = sig.ld(t);                                                                   \
+        ev.Gw = sig.ld(dec_s);                                                               \
         const uint32_t L = ev.Lw & kIdMask, G = ev.Gw & kIdMask;                             \
         const uint32_t lidx = ev.Lw >> 12, gidx = ev.Gw >> 12;                               \
         const bool nn = dec_ok && ev.Lw != ev.Gw;                                            \
@@ -599,7 +622,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 
             const int o0 = code % 3 - 1, o1 = (code / 3) % 3 - 1, o2 = code / 9 - 1;         \
             const int32_t a = D == 2 ? o0 * (int32_t)stride[0] + o1                          \
                                      : o0 * (int32_t)stride[0] + o1 * (int32_t)stride[1 % D] + o2; \
-            nb[m] = sig[(int32_t)t + a];                                                     \
+            nb[m] = sig.ld((int32_t)t + a);                                                  \
         }                                                                                    \
         ev.vL = vol[L ? L - 1 : 0];                                                          \
         ev.vG = vol[G ? G - 1 : 0];                                                          \
@@ -766,14 +789,15 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 
             for (int d = 0; d < D; ++d) cval[d] = rl32(dec_c[d], p);
             // dynamics.rs:73-98 — contact counts, pre-move lattice
             if (lane < NE) {
-                const uint32_t nbe = sig[(int32_t)t + eoff] & kIdMask;
+                const uint32_t nbe = sig.ld((int32_t)t + eoff) & kIdMask;
                 if (nbe != L)
                     atomicSub(&R.contact[(uint64_t)min(L, nbe) * ncell1 + max(L, nbe)], 1u);
                 if (nbe != G)
                     atomicAdd(&R.contact[(uint64_t)min(G, nbe) * ncell1 + max(G, nbe)], 1u);
             }
+            __syncthreads();  // every lane has read t's neighbourhood
             if (lane == 0) {
-                sig[t] = (uint16_t)Gw;
+                sig.st(t, Gw);
                 if (L) {
                     vol[L - 1] = vL - 1;
                     itf[L - 1] = (uint32_t)((int32_t)iL + dl_i);
@@ -834,10 +858,11 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 
                             f += opt[d][r % nopt[d]] * stride[d];
                             r /= nopt[d];
                         }
-                        sig[f] = (uint16_t)Gw;
+                        sig.st(f, Gw);
                     }
                 }
             }
+            if constexpr (SG) __threadfence();  // the next evaluation reads sigma from HBM
             __syncthreads();
             energy += dh;
             p = np;
@@ -895,7 +920,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 
     const uint64_t st_end[4] = {rl64(seg_end, 0), rl64(seg_end, 1), rl64(seg_end, 2),
                                 rl64(seg_end, 3)};
     if (dirty) {
-        for (uint32_t i = lane; i < n_padded; i += 64) R.sigma[i] = sig[i] & kIdMask;
+        for (uint32_t i = lane; i < n_padded; i += 64) R.sigma[i] = sig.ld(i) & kIdMask;
         for (uint32_t i = lane; i < n_cells; i += 64) {
             R.volume[i] = vol[i];
             R.interface[i] = itf[i];
@@ -922,6 +947,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 
         }
     }
 }
+
+// key = 1000 * (sigma in HBM) + 100 * D + energy stencil size
+#define CPM_WAVE_KERNELS(X)                                                            \
+    X(204, 2, 4, false) X(208, 2, 8, false) X(306, 3, 6, false) X(318, 3, 18, false)   \
+    X(326, 3, 26, false) X(1204, 2, 4, true) X(1208, 2, 8, true) X(1306, 3, 6, true)   \
+    X(1318, 3, 18, true) X(1326, 3, 26, true)
 
 // ===========================================================================
 // Sublattice-parallel mode: halo-disjoint checkerboard colours updated in
@@ -1299,13 +1330,18 @@ __global__ void __launch_bounds__(256) k_sp3d(SpMasks M, uint64_t* __restrict__ 
 struct Device {
     char* base = nullptr;
     Rep rep{};  // device pointers + static fields
-    bool lds = false;
-    uint32_t lds_bytes = 0;
+    bool lds = false;     // wave path
+    bool gsig = false;    // sigma does not fit in LDS: always in HBM
+    uint32_t lds_bytes = 0;       // LDS with sigma in LDS (when it fits)
+    uint32_t lds_bytes_hbm = 0;   // LDS with sigma in HBM
 };
 
 // LDS kept free of the dynamic allocation for the kernels' static LDS
 // (the sublattice kernel's counters and the compiler's own use)
 constexpr uint32_t kLdsStaticReserve = 1024;
+// Share of the chip's LDS the sigma placement plans to fill: workgroups of
+// mixed sizes do not pack a CU's LDS perfectly.
+constexpr double kLdsPlacementFill = 0.75;
 
 struct Engine {
     int device = 0;
@@ -1322,6 +1358,7 @@ struct Engine {
     hipStream_t cls_stream[kClassStreams] = {};
     hipEvent_t cls_done[kClassStreams] = {};
     uint32_t max_lds = 0;
+    uint32_t n_cu = 0;
     std::vector<hipEvent_t> launch_ev;  // start/stop pairs around replay launches
     std::mutex mu;
 };
@@ -1358,10 +1395,13 @@ Engine* engine_create(int device) {
     int lds = 0;
     HIP_OK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
     e->max_lds = (uint32_t)lds;
-    for (auto fn : {(const void*)k_lds_engine<2, 4>, (const void*)k_lds_engine<2, 8>,
-                    (const void*)k_lds_engine<3, 6>, (const void*)k_lds_engine<3, 18>,
-                    (const void*)k_lds_engine<3, 26>, (const void*)k_sublattice<2>,
+    int n_cu = 0;
+    HIP_OK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device));
+    e->n_cu = (uint32_t)n_cu;
+#define CPM_FN(k, d, ne, gs) (const void*)k_lds_engine<d, ne, gs>,
+    for (auto fn : {CPM_WAVE_KERNELS(CPM_FN)(const void*)k_sublattice<2>,
                     (const void*)k_sublattice<3>})
+#undef CPM_FN
         HIP_OK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    lds - kLdsStaticReserve));
     return e;
@@ -1429,7 +1469,7 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct Layout {
     size_t sigma, type_index, volume, interface, sum_pos, sum_outer, contact, params, targets,
-        draws, total;
+        draws, sig16, total;
 };
 
 static Layout layout_of(const Model& m, bool with_draws) {
@@ -1451,30 +1491,38 @@ static Layout layout_of(const Model& m, bool with_draws) {
     L.params = take((2 * T + (T + 1) * (T + 1)) * 8);
     L.targets = take(2 * T * 4);
     L.draws = with_draws ? take((size_t)kFill * 8 + (size_t)kSegs * 32) : 0;
+    L.sig16 = with_draws ? take(m.n_padded * 2) : 0;
     L.total = off;
     return L;
 }
 
-static bool lds_eligible(const Engine* e, const Model& m, uint32_t* bytes) {
-    // packed LDS words: 12-bit ids, 4-bit adhesion indices
+// Whether the model runs on the wave path (one wave per replica), and with
+// sigma in LDS or in HBM (*gsig): packed 16-bit words need at most 4095
+// cells and 15 types; sigma stays in LDS when it fits beside the tables.
+static bool lds_eligible(const Engine* e, const Model& m, uint32_t* bytes, bool* gsig = nullptr) {
     if (m.n_cells > kMaxPackedCells || m.n_padded > (1u << 30)) return false;
     if (m.cfg.types.size() > 15) return false;
-    const LdsLayout Ly =
-        lds_layout((uint32_t)m.n_padded, (uint32_t)m.n_cells, (uint32_t)m.cfg.types.size());
-    *bytes = Ly.total;
-    return Ly.total + kLdsStaticReserve <= e->max_lds;
+    const uint32_t T = (uint32_t)m.cfg.types.size();
+    const LdsLayout in = lds_layout((uint32_t)m.n_padded, (uint32_t)m.n_cells, T, true);
+    const LdsLayout out = lds_layout((uint32_t)m.n_padded, (uint32_t)m.n_cells, T, false);
+    const bool fits = in.total + kLdsStaticReserve <= e->max_lds;
+    if (gsig) *gsig = !fits;
+    *bytes = fits ? in.total : out.total;
+    if (!gsig) return fits;  // callers that need sigma in LDS
+    return out.total + kLdsStaticReserve <= e->max_lds;
 }
 
 // LDS bytes a launch reserves for replicas needing `bytes` (grouping key).
 static uint32_t lds_class(const Engine* e, uint32_t bytes) {
-    return std::min((bytes + 4095) / 4096 * 4096, e->max_lds - kLdsStaticReserve);
+    return std::min((bytes + 1023) / 1024 * 1024, e->max_lds - kLdsStaticReserve);
 }
 
 static void upload(Engine* e, Model* m) {
     if (m->dev) return;
     if (!m->initialized) throw Error(CPM_ERR_ARG, "model is not initialized");
     uint32_t lds_bytes = 0;
-    const bool lds = lds_eligible(e, *m, &lds_bytes);
+    bool gsig = false;
+    const bool lds = lds_eligible(e, *m, &lds_bytes, &gsig);
     const Layout L = layout_of(*m, lds);
     auto* d = new Device();
     hipError_t err = hipMalloc(&d->base, L.total);
@@ -1484,7 +1532,11 @@ static void upload(Engine* e, Model* m) {
     }
     m->dev = d;
     d->lds = lds;
+    d->gsig = lds && gsig;
     d->lds_bytes = lds_bytes;
+    d->lds_bytes_hbm = lds_layout((uint32_t)m->n_padded, (uint32_t)m->n_cells,
+                                  (uint32_t)m->cfg.types.size(), false)
+                           .total;
     if (lds) {
         // the LDS kernel addresses the full shell from the strides and maps
         // the energy stencil into it at compile time
@@ -1529,6 +1581,7 @@ static void upload(Engine* e, Model* m) {
     R.params = (const double*)(d->base + L.params);
     R.targets = (const uint32_t*)(d->base + L.targets);
     R.draws = lds ? (uint64_t*)(d->base + L.draws) : nullptr;
+    R.sig16 = lds ? (uint16_t*)(d->base + L.sig16) : nullptr;
     const size_t T = R.n_types;
     std::vector<double> params(2 * T + (T + 1) * (T + 1));
     std::vector<uint32_t> targets(2 * T);
@@ -1559,12 +1612,13 @@ static void upload(Engine* e, Model* m) {
 
 size_t device_footprint(Engine* e, const Model& m) {
     uint32_t bytes = 0;
-    return layout_of(m, lds_eligible(e, m, &bytes)).total;
+    bool gsig = false;
+    return layout_of(m, lds_eligible(e, m, &bytes, &gsig)).total;
 }
 
 bool sublattice_supported(Engine* e, const Model& m) {
     uint32_t bytes = 0;
-    if (!lds_eligible(e, m, &bytes)) return false;
+    if (!lds_eligible(e, m, &bytes)) return false;  // sigma must fit in LDS
     const LdsLayout Ly = lds_layout((uint32_t)m.n_padded, (uint32_t)m.n_cells,
                                     (uint32_t)m.cfg.types.size());
     if (sl_lds_bytes(Ly, (uint32_t)m.n_cells) + kLdsStaticReserve > e->max_lds) return false;
@@ -1629,16 +1683,15 @@ static void add_counters(cpm_counters* c, const Rep& R) {
 static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint64_t* todo,
                            cpm_counters* counters);
 
-// key = 100 * D + energy stencil size
+
 static void launch_lds_engine(int key, uint32_t lds, uint32_t count, hipStream_t st, Rep* reps,
                               const uint32_t* idx, const uint64_t* sp3d) {
     const dim3 g(count), b(64);
     switch (key) {
-        case 204: hipLaunchKernelGGL((k_lds_engine<2, 4>), g, b, lds, st, reps, idx, sp3d); break;
-        case 208: hipLaunchKernelGGL((k_lds_engine<2, 8>), g, b, lds, st, reps, idx, sp3d); break;
-        case 306: hipLaunchKernelGGL((k_lds_engine<3, 6>), g, b, lds, st, reps, idx, sp3d); break;
-        case 318: hipLaunchKernelGGL((k_lds_engine<3, 18>), g, b, lds, st, reps, idx, sp3d); break;
-        case 326: hipLaunchKernelGGL((k_lds_engine<3, 26>), g, b, lds, st, reps, idx, sp3d); break;
+#define CPM_CASE(k, d, ne, gs)                                                                \
+    case k: hipLaunchKernelGGL((k_lds_engine<d, ne, gs>), g, b, lds, st, reps, idx, sp3d); break;
+        CPM_WAVE_KERNELS(CPM_CASE)
+#undef CPM_CASE
         default: throw Error(CPM_ERR_DEVICE, "no exact-replay kernel for this stencil");
     }
 }
@@ -1661,8 +1714,38 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
     ensure_buf(e->reps_dev, e->reps_cap, n);
     ensure_buf(e->idx_dev, e->idx_cap, n);
     std::vector<Rep> reps(n);
-    // partition: global-path replicas, and LDS replicas by kernel
-    // instantiation (100 * D + energy stencil size)
+    // Sigma placement for this batch: a lattice too big for LDS keeps sigma
+    // in HBM; the others keep it in LDS smallest first, while the whole
+    // batch's LDS still fits the chip at once (so every replica is resident
+    // from the start and the largest lattices' serial chains set the time);
+    // the rest keep sigma in HBM (read through L2) and use LDS for their
+    // per-cell tables only.
+    std::vector<char> in_hbm(n, 0);
+    {
+        std::vector<uint32_t> cand;
+        double demand = 0.0;
+        for (size_t i = 0; i < n; ++i) {
+            const Model* m = models[i];
+            if (!todo[i] || !m->dev->lds) continue;
+            in_hbm[i] = 1;
+            demand += (double)lds_class(e, m->dev->lds_bytes_hbm);
+            if (!m->dev->gsig) cand.push_back((uint32_t)i);
+        }
+        std::stable_sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) {
+            return models[a]->dev->lds_bytes < models[b]->dev->lds_bytes;
+        });
+        const double capacity = kLdsPlacementFill * (double)e->n_cu * (double)e->max_lds;
+        for (uint32_t i : cand) {
+            const Device* d = models[i]->dev;
+            const double extra =
+                (double)lds_class(e, d->lds_bytes) - (double)lds_class(e, d->lds_bytes_hbm);
+            if (demand + extra > capacity) break;
+            demand += extra;
+            in_hbm[i] = 0;
+        }
+    }
+    // partition: global-path replicas, and wave-path replicas by kernel
+    // instantiation (1000 * sigma in HBM + 100 * D + energy stencil size)
     std::vector<uint32_t> order, global_idx;
     std::map<int, std::vector<uint32_t>> by_kernel;
     for (size_t i = 0; i < n; ++i) {
@@ -1679,7 +1762,7 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
         reps[i] = R;
         if (!todo[i]) continue;
         if (m->dev->lds)
-            by_kernel[m->D * 100 + m->n_energy].push_back((uint32_t)i);
+            by_kernel[(in_hbm[i] ? 1000 : 0) + m->D * 100 + m->n_energy].push_back((uint32_t)i);
         else
             global_idx.push_back((uint32_t)i);
     }
@@ -1702,8 +1785,10 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
             Engine::kClassStreams > n_kernels ? Engine::kClassStreams - n_kernels : 0;
         for (auto& [key, v] : by_kernel) {
             std::vector<std::pair<uint32_t, uint32_t>> cls(v.size());  // (class bytes, replica)
-            for (size_t j = 0; j < v.size(); ++j)
-                cls[j] = {lds_class(e, models[v[j]]->dev->lds_bytes), v[j]};
+            for (size_t j = 0; j < v.size(); ++j) {
+                const Device* d = models[v[j]]->dev;
+                cls[j] = {lds_class(e, key >= 1000 ? d->lds_bytes_hbm : d->lds_bytes), v[j]};
+            }
             std::stable_sort(cls.begin(), cls.end(),
                              [](auto& a, auto& b) { return a.first < b.first; });
             // distinct classes with counts

@@ -75,11 +75,14 @@ CASES = [
                              ti=100, phi=0.5, boundary="fixed"), 8),
     ("sorting3d_26", dict(kind="sort", grid=(12, 12, 12), seed=23, n_types=2, tv=60, ti=120,
                           phi=0.4, energy="twenty_six"), 8),
-    # lattices whose haloed u16 copy exceeds LDS run on the global-memory path
-    ("global_path_2d", dict(kind="sort", grid=(300, 296), seed=31, n_types=3, tv=60, ti=40,
-                            phi=0.6), 2),
-    ("global_path_3d", dict(kind="sort", grid=(48, 48, 48), seed=32, n_types=2, tv=300,
-                            ti=300, phi=0.2, boundary="fixed"), 1),
+    # lattices whose haloed u16 copy exceeds LDS keep sigma in HBM
+    ("hbm_sigma_2d", dict(kind="sort", grid=(300, 296), seed=31, n_types=3, tv=60, ti=40,
+                          phi=0.6), 2),
+    ("hbm_sigma_3d", dict(kind="sort", grid=(48, 48, 48), seed=32, n_types=2, tv=300,
+                          ti=300, phi=0.2, boundary="fixed"), 1),
+    # more cells than 12-bit packed ids hold: one lane per replica, state in HBM
+    ("lane_path_4352_cells", dict(kind="sort", grid=(160, 160), seed=33, n_types=2, tv=5,
+                                  ti=10, phi=0.85), 2),
     ("long_run_many_rounds", dict(kind="quick", grid=(30, 30), seed=9), 2000),
 ]
 
