@@ -28,7 +28,7 @@ sys.path.insert(0, os.path.join(ROOT, "cpm-rust_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-DOMINANT_KERNEL = "k_lds_engine<2, 8>"  # exact replay, 2D, Moore energy stencil
+DOMINANT_KERNEL = "k_lds_engine<2, 8, *>"  # exact replay, 2D, Moore stencil, either sigma placement
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_lds_engine_2d_run_batch4096_pmc.txt")
 SIDES = (48, 64, 96, 128)
 INTERVAL = 500
@@ -112,7 +112,7 @@ def copy_calibration(mib: int = 256, reps: int = 4):
 
 
 def serving_trace(cpm, n_requests: int = 1500, rate: float = 200.0, seed: int = 7,
-                  quantum: int = 1 << 22, max_active: int = 4096):
+                  quantum: int = 1 << 17, max_active: int = 4096):
     """The serving workload: a Poisson stream (mean `rate` requests/s) of
     mixed run requests — 70 % 2D periodic lattices of side U{64..256}, 30 %
     3D periodic lattices of side U{32..40} (the LDS-resident range) — with

@@ -59,7 +59,7 @@ class Server:
     """
 
     def __init__(self, hbm_budget: int | None = None, max_active: int = 4096,
-                 quantum_attempts: int = 1 << 22):
+                 quantum_attempts: int = 1 << 18):
         opt = N.CpmServerOptions(int(hbm_budget or 0), int(max_active), int(quantum_attempts))
         self.h = C.c_void_p()
         N.check(N.lib().cpm_serve_create(N.engine(), C.byref(opt), C.byref(self.h)))

@@ -182,7 +182,7 @@ Server* serve_create(Engine* e, const ServeOptions& o) {
     s->opt = o;
     if (!s->opt.hbm_budget) s->opt.hbm_budget = (uint64_t)(0.8 * (double)device_free_bytes(e));
     if (!s->opt.max_active) s->opt.max_active = 4096;
-    if (!s->opt.quantum) s->opt.quantum = 1ull << 22;
+    if (!s->opt.quantum) s->opt.quantum = 1ull << 18;
     s->st.hbm_budget = s->opt.hbm_budget;
     s->t0 = Clock::now();
     s->worker = std::thread([s] { s->loop(); });

@@ -268,7 +268,7 @@ typedef struct cpm_server cpm_server;
 typedef struct {
     uint64_t hbm_budget_bytes; /* 0: 80 % of the device's free memory      */
     uint32_t max_active;       /* requests on the device at once; 0: 4096  */
-    uint64_t quantum_attempts; /* per request per round; 0: 2^22           */
+    uint64_t quantum_attempts; /* per request per round; 0: 2^18           */
 } cpm_server_options;
 enum { CPM_REQ_QUEUED = 0, CPM_REQ_ACTIVE = 1, CPM_REQ_DONE = 2, CPM_REQ_FAILED = 3 };
 typedef struct {
