@@ -20,6 +20,7 @@
 // reference's operation order (crates/cpm-core/src/energy.rs:92-103,
 // 167-202, 242-264, 521-523); sums of +-1.0 are formed from popcounts,
 // which is exact.
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -68,6 +69,7 @@ struct Rep {
     const uint32_t* targets;  // V*[T], I*[T]
     uint64_t* draws;          // kFill buffered draws + kSegs segment states (wave path)
     uint16_t* sig16;          // packed sigma in HBM (wave path with sigma outside LDS)
+    unsigned long long* scratch;  // grid sublattice: per-cell priorities + 2 counters
     uint64_t rng[4];          // consumer state at the start / end of a call
     double energy;
     uint64_t mcs;
@@ -1278,6 +1280,303 @@ __global__ void __launch_bounds__(kSlThreads) k_sublattice(Rep* __restrict__ rep
 }
 
 // ===========================================================================
+// Sublattice mode for lattices whose sigma does not fit in LDS: a
+// cooperative launch over several replicas, each owning a contiguous range
+// of `blocks_per_rep` workgroups; all of them work on colour phases in step,
+// with packed sigma words (HBM copy), V, I and the per-cell priority words
+// in HBM.  Same schedule, priority rounds and counter-based randomness as
+// k_sublattice, with grid-wide barriers where that kernel uses workgroup
+// barriers; a replica's batches are as many sites as its workgroups have
+// threads.  Every loop that contains a barrier has the same trip count on
+// every workgroup (max_todo MCS, max_batches batches per phase, rounds
+// until no replica has a pending attempt).
+// ===========================================================================
+__device__ __forceinline__ void grid_barrier(cooperative_groups::grid_group& grid) {
+    __threadfence();
+    grid.sync();
+    __threadfence();
+}
+
+__device__ __forceinline__ unsigned long long load_agent(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int D>
+__global__ void __launch_bounds__(kSlThreads) k_sublattice_grid(
+    Rep* __restrict__ reps, const uint32_t* __restrict__ idx, uint32_t blocks_per_rep,
+    uint64_t max_todo, uint32_t max_batches, unsigned long long* __restrict__ pending_count,
+    const uint64_t* __restrict__ sp3d) {
+    namespace cg = cooperative_groups;
+    cg::grid_group grid = cg::this_grid();
+    __shared__ unsigned long long s_cnt[6];
+    Rep& R = reps[idx[blockIdx.x / blocks_per_rep]];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t gtid = (blockIdx.x % blocks_per_rep) * kSlThreads + tid;
+    const uint32_t gthreads = blocks_per_rep * kSlThreads;
+    const bool leader = blockIdx.x == 0 && tid == 0;
+    const uint32_t T = R.n_types, T1 = T + 1, n_cells = R.n_cells;
+    const uint32_t n_padded = R.n_padded;
+    __attribute__((address_space(1))) uint16_t* sig =
+        (__attribute__((address_space(1))) uint16_t*)R.sig16;
+    __attribute__((address_space(1))) uint32_t* vol =
+        (__attribute__((address_space(1))) uint32_t*)R.volume;
+    __attribute__((address_space(1))) uint32_t* itf =
+        (__attribute__((address_space(1))) uint32_t*)R.interface;
+    unsigned long long* prio = (unsigned long long*)R.scratch;  // [n_cells]
+    const double* __restrict__ J = R.params + 2 * T;
+    const double* __restrict__ lv = R.params;
+    const double* __restrict__ li = R.params + T;
+    const uint32_t* __restrict__ tv = R.targets;
+    const uint32_t* __restrict__ ti = R.targets + T;
+    const uint32_t* __restrict__ gtyp = R.type_index;
+    for (uint32_t i = gtid; i < n_padded; i += gthreads) {
+        const uint32_t id = R.sigma[i];
+        sig[i] = (uint16_t)(id ? id | ((gtyp[id - 1] + 1) << 12) : 0u);
+    }
+    for (uint32_t i = gtid; i < n_cells; i += gthreads) prio[i] = 0;
+    if (leader) pending_count[0] = pending_count[1] = 0;
+    if (tid < 6) s_cnt[tid] = 0;
+    grid_barrier(grid);
+
+    uint32_t dims[D], stride[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        dims[d] = R.dims[d];
+        stride[d] = R.strides[d];
+    }
+    const uint32_t n_copy = R.n_copy, n_energy = R.n_energy, n_topo = R.n_topo;
+    const bool periodic = R.boundary == CPM_BOUNDARY_PERIODIC;
+    const uint64_t ncell1 = (uint64_t)n_cells + 1;
+    const uint64_t key = R.fill_state[0];
+    unsigned long long c_att = 0, c_nn = 0, c_top = 0, c_en = 0, c_acc = 0;
+    constexpr uint32_t NC = 1u << D;
+    uint32_t round_no = 0;
+
+    const uint64_t todo = R.todo;
+    for (uint64_t m = 0; m < max_todo; ++m) {
+        const uint64_t mcs = R.mcs + m;
+        for (uint32_t sub = 0; sub < kSlSub; ++sub) {
+            const uint64_t phase0 = (mcs * kSlSub + sub) * NC;
+            uint32_t order[NC];
+            for (uint32_t c = 0; c < NC; ++c) order[c] = c;
+            uint64_t hperm = splitmix_at(key, ~phase0);
+            for (uint32_t c = NC - 1; c > 0; --c) {
+                const uint32_t j = (uint32_t)((hperm & 0xffff) % (c + 1));
+                hperm >>= 16;
+                const uint32_t tmp = order[c];
+                order[c] = order[j];
+                order[j] = tmp;
+            }
+            for (uint32_t ci = 0; ci < NC; ++ci) {
+                const uint32_t col = order[ci];
+                const uint64_t phase = phase0 + col;
+                uint32_t half[D], nsite = 1;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    const uint32_t cd = (col >> d) & 1u;
+                    half[d] = dims[d] > cd ? (dims[d] - cd + 1) / 2 : 0;
+                    nsite *= half[d];
+                }
+                const uint32_t rot =
+                    nsite ? (uint32_t)(splitmix_at(key ^ 0x5bd1e995ull, phase) % nsite) : 0u;
+                for (uint32_t bi = 0; bi < max_batches; ++bi) {
+                    const uint32_t jb = bi * gthreads + gtid;
+                    uint32_t j = 0, c[D], t = 0, Lw = 0, Gw = 0;
+                    unsigned long long pr = 0;
+                    uint64_t ctr = 0;
+                    bool pending = false;
+                    if (m < todo && jb < nsite) {
+                        j = jb + rot;
+                        if (j >= nsite) j -= nsite;
+                        uint32_t rem = j;
+#pragma unroll
+                        for (int d = D - 1; d >= 0; --d) {
+                            c[d] = 2 * (rem % half[d]) + ((col >> d) & 1u);
+                            rem /= half[d];
+                            t += (c[d] + 1) * stride[d];
+                        }
+                        ctr = (phase * (uint64_t)R.n_sites + j) * 4;
+                        const uint64_t h0 = splitmix_at(key, ctr);
+                        if ((h0 & (kSlSub - 1)) == 0) {
+                            const uint32_t k = (uint32_t)(((h0 >> 32) * (uint64_t)n_copy) >> 32);
+                            const uint32_t s = (uint32_t)((int32_t)t + R.copy_off[k]);
+                            c_att++;
+                            Lw = sig[t];
+                            Gw = sig[s];
+                            if (Lw != Gw) {
+                                c_nn++;
+                                const uint32_t L = Lw & kIdMask;
+                                bool simple = true;
+                                if (L != 0) {
+                                    uint32_t pat = 0;
+                                    for (uint32_t i = 0; i < n_topo; ++i)
+                                        pat |= (uint32_t)((sig[(int32_t)t + R.topo_off[i]] & kIdMask) == L) << i;
+                                    c_top++;
+                                    simple = D == 2 ? ((c_sp2d[pat >> 5] >> (pat & 31)) & 1u)
+                                                    : ((sp3d[pat >> 6] >> (pat & 63)) & 1ull);
+                                }
+                                if (simple) {
+                                    pending = true;
+                                    pr = 0x8000000000000000ull |
+                                         (splitmix_at(key, ctr + 1) & 0x7fffffffff000000ull) | j;
+                                }
+                            }
+                        }
+                    }
+                    for (;;) {
+                        const uint32_t L = Lw & kIdMask, G = Gw & kIdMask;
+                        const uint32_t slot = round_no & 1;
+                        if (pending) {
+                            if (L) atomicMax(&prio[L - 1], pr);
+                            if (G) atomicMax(&prio[G - 1], pr);
+                            atomicAdd(&pending_count[slot], 1ull);
+                        }
+                        if (leader) pending_count[slot ^ 1] = 0;
+                        grid_barrier(grid);
+                        const unsigned long long left = load_agent(&pending_count[slot]);
+                        ++round_no;
+                        if (left == 0) break;
+                        const bool win = pending && (!L || load_agent(&prio[L - 1]) == pr) &&
+                                         (!G || load_agent(&prio[G - 1]) == pr);
+                        grid_barrier(grid);
+                        if (pending) {
+                            if (L) prio[L - 1] = 0;
+                            if (G) prio[G - 1] = 0;
+                        }
+                        if (win) {
+                            pending = false;
+                            const uint32_t vL = L ? vol[L - 1] : 0u;
+                            if (L == 0 || vL > 1) {
+                                c_en++;
+                                const uint32_t lidx = Lw >> 12, gidx = Gw >> 12;
+                                const uint32_t tl = lidx ? lidx - 1 : 0u, tg = gidx ? gidx - 1 : 0u;
+                                int nl = 0, ng = 0;
+                                double dadh = 0.0;  // energy.rs:242-264, stencil order
+                                for (uint32_t q = 0; q < n_energy; ++q) {
+                                    const uint32_t w = sig[(int32_t)t + R.energy_off[q]];
+                                    const uint32_t nb = w & kIdMask, ni = w >> 12;
+                                    nl += nb == L;
+                                    ng += nb == G;
+                                    const double before = nb != L ? J[lidx * T1 + ni] : 0.0;
+                                    const double after = nb != G ? J[gidx * T1 + ni] : 0.0;
+                                    dadh += 1.0 * (after - before);
+                                }
+                                const int dl_i = 2 * nl - (int)n_energy, dg_i = (int)n_energy - 2 * ng;
+                                double dvol = 0.0, ditf = 0.0;
+                                if (L) {
+                                    const double v = (double)vL;
+                                    const double x1 = (v - 1.0) - (double)tv[tl], x0 = v - (double)tv[tl];
+                                    dvol += lv[tl] * (x1 * x1) - lv[tl] * (x0 * x0);
+                                    const double i0 = (double)itf[L - 1];
+                                    const double y1 = (i0 + (double)dl_i) - (double)ti[tl], y0 = i0 - (double)ti[tl];
+                                    ditf += li[tl] * (y1 * y1) - li[tl] * (y0 * y0);
+                                }
+                                if (G) {
+                                    const double v = (double)vol[G - 1];
+                                    const double x1 = (v + 1.0) - (double)tv[tg], x0 = v - (double)tv[tg];
+                                    dvol += lv[tg] * (x1 * x1) - lv[tg] * (x0 * x0);
+                                    const double i0 = (double)itf[G - 1];
+                                    const double y1 = (i0 + (double)dg_i) - (double)ti[tg], y0 = i0 - (double)ti[tg];
+                                    ditf += li[tg] * (y1 * y1) - li[tg] * (y0 * y0);
+                                }
+                                const double dh = dvol + ditf + dadh;
+                                bool accept = dh <= 0.0;
+                                if (!accept) {
+                                    const uint64_t hu = splitmix_at(key, ctr + 2);
+                                    const double u = (double)(hu >> 11) * (1.0 / 9007199254740992.0);
+                                    accept = u < exp(-dh);
+                                }
+                                if (accept) {
+                                    c_acc++;
+                                    for (uint32_t q = 0; q < n_energy; ++q) {
+                                        const uint32_t nb = sig[(int32_t)t + R.energy_off[q]] & kIdMask;
+                                        if (nb != L)
+                                            atomicSub(&R.contact[(uint64_t)min(L, nb) * ncell1 + max(L, nb)], 1u);
+                                        if (nb != G)
+                                            atomicAdd(&R.contact[(uint64_t)min(G, nb) * ncell1 + max(G, nb)], 1u);
+                                    }
+                                    sig[t] = (uint16_t)Gw;
+                                    if (L) {
+                                        vol[L - 1] = vL - 1;
+                                        itf[L - 1] = (uint32_t)((int32_t)itf[L - 1] + dl_i);
+                                        int64_t* sp = R.sum_pos + (uint64_t)(L - 1) * D;
+                                        int64_t* so = R.sum_outer + (uint64_t)(L - 1) * D * D;
+                                        for (int d = 0; d < D; ++d) sp[d] -= (int64_t)c[d];
+                                        for (int d = 0; d < D; ++d)
+                                            for (int e2 = 0; e2 < D; ++e2)
+                                                so[d * D + e2] -= (int64_t)c[d] * (int64_t)c[e2];
+                                    }
+                                    if (G) {
+                                        const uint32_t vg_new = vol[G - 1] + 1;
+                                        vol[G - 1] = vg_new;
+                                        itf[G - 1] = (uint32_t)((int32_t)itf[G - 1] + dg_i);
+                                        int64_t* sp = R.sum_pos + (uint64_t)(G - 1) * D;
+                                        int64_t* so = R.sum_outer + (uint64_t)(G - 1) * D * D;
+                                        const uint32_t vbg = max(vg_new - 1, 1u);
+                                        int64_t img[D];
+                                        for (int d = 0; d < D; ++d) {
+                                            const double cen = (double)sp[d] / (double)vbg;
+                                            int64_t best = (int64_t)c[d];
+                                            if (periodic) {
+                                                double bd = fabs((double)best - cen);
+                                                const int64_t lo = (int64_t)c[d] - (int64_t)dims[d];
+                                                const int64_t hi = (int64_t)c[d] + (int64_t)dims[d];
+                                                const double dlo = fabs((double)lo - cen);
+                                                if (dlo < bd) {
+                                                    best = lo;
+                                                    bd = dlo;
+                                                }
+                                                if (fabs((double)hi - cen) < bd) best = hi;
+                                            }
+                                            img[d] = best;
+                                        }
+                                        for (int d = 0; d < D; ++d) sp[d] += img[d];
+                                        for (int d = 0; d < D; ++d)
+                                            for (int e2 = 0; e2 < D; ++e2) so[d * D + e2] += img[d] * img[e2];
+                                    }
+                                    if (periodic) {
+                                        uint32_t opt[D][3], nopt[D], total = 1;
+                                        for (int d = 0; d < D; ++d) {
+                                            nopt[d] = 1;
+                                            opt[d][0] = c[d] + 1;
+                                            if (c[d] == 0) opt[d][nopt[d]++] = dims[d] + 1;
+                                            if (c[d] == dims[d] - 1) opt[d][nopt[d]++] = 0;
+                                            total *= nopt[d];
+                                        }
+                                        for (uint32_t q = 1; q < total; ++q) {
+                                            uint32_t r = q, f = 0;
+                                            for (int d = D - 1; d >= 0; --d) {
+                                                f += opt[d][r % nopt[d]] * stride[d];
+                                                r /= nopt[d];
+                                            }
+                                            sig[f] = (uint16_t)Gw;
+                                        }
+                                    }
+                                }
+                            }
+                        }
+                        grid_barrier(grid);
+                    }
+                }
+            }
+        }
+    }
+    atomicAdd(&s_cnt[0], c_att);
+    atomicAdd(&s_cnt[1], c_nn);
+    atomicAdd(&s_cnt[2], c_top);
+    atomicAdd(&s_cnt[3], c_en);
+    atomicAdd(&s_cnt[4], c_acc);
+    __syncthreads();
+    if (tid < 5) atomicAdd((unsigned long long*)&R.cnt[tid], s_cnt[tid]);
+    for (uint32_t i = gtid; i < n_padded; i += gthreads) R.sigma[i] = sig[i] & kIdMask;
+    if (gtid == 0) {
+        R.mcs += todo;
+        R.todo = 0;
+        R.done = 1;
+    }
+}
+
+// ===========================================================================
 // 3D simple-point table, generated from the definition on the device: one
 // thread per 64 patterns (simple_point.rs:242-276, 340-354).
 // ===========================================================================
@@ -1360,6 +1659,10 @@ struct Engine {
     uint32_t max_lds = 0;
     uint32_t n_cu = 0;
     std::vector<hipEvent_t> launch_ev;  // start/stop pairs around replay launches
+    uint32_t* grid_idx = nullptr;       // grid sublattice: replica per workgroup range
+    size_t grid_idx_cap = 0;
+    unsigned long long* grid_counters = nullptr;  // grid sublattice: round counters
+    size_t grid_counters_cap = 0;
     std::mutex mu;
 };
 
@@ -1441,6 +1744,8 @@ void engine_destroy(Engine* e) {
     if (e->sp3d) (void)hipFree(e->sp3d);
     if (e->reps_dev) (void)hipFree(e->reps_dev);
     if (e->idx_dev) (void)hipFree(e->idx_dev);
+    if (e->grid_idx) (void)hipFree(e->grid_idx);
+    if (e->grid_counters) (void)hipFree(e->grid_counters);
     if (e->ev0) (void)hipEventDestroy(e->ev0);
     if (e->ev1) (void)hipEventDestroy(e->ev1);
     for (int i = 0; i < Engine::kClassStreams; ++i) {
@@ -1469,7 +1774,7 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct Layout {
     size_t sigma, type_index, volume, interface, sum_pos, sum_outer, contact, params, targets,
-        draws, sig16, total;
+        draws, sig16, scratch, total;
 };
 
 static Layout layout_of(const Model& m, bool with_draws) {
@@ -1492,6 +1797,7 @@ static Layout layout_of(const Model& m, bool with_draws) {
     L.targets = take(2 * T * 4);
     L.draws = with_draws ? take((size_t)kFill * 8 + (size_t)kSegs * 32) : 0;
     L.sig16 = with_draws ? take(m.n_padded * 2) : 0;
+    L.scratch = take(m.n_cells * 8 + 16);
     L.total = off;
     return L;
 }
@@ -1582,6 +1888,7 @@ static void upload(Engine* e, Model* m) {
     R.targets = (const uint32_t*)(d->base + L.targets);
     R.draws = lds ? (uint64_t*)(d->base + L.draws) : nullptr;
     R.sig16 = lds ? (uint16_t*)(d->base + L.sig16) : nullptr;
+    R.scratch = (unsigned long long*)(d->base + L.scratch);
     const size_t T = R.n_types;
     std::vector<double> params(2 * T + (T + 1) * (T + 1));
     std::vector<uint32_t> targets(2 * T);
@@ -1616,12 +1923,20 @@ size_t device_footprint(Engine* e, const Model& m) {
     return layout_of(m, lds_eligible(e, m, &bytes, &gsig)).total;
 }
 
-bool sublattice_supported(Engine* e, const Model& m) {
+// Sublattice mode runs in LDS (k_sublattice) when sigma, the tables and the
+// priority words fit, else grid-wide (k_sublattice_grid).
+static bool sublattice_in_lds(Engine* e, const Model& m) {
     uint32_t bytes = 0;
-    if (!lds_eligible(e, m, &bytes)) return false;  // sigma must fit in LDS
+    if (!lds_eligible(e, m, &bytes)) return false;
     const LdsLayout Ly = lds_layout((uint32_t)m.n_padded, (uint32_t)m.n_cells,
                                     (uint32_t)m.cfg.types.size());
-    if (sl_lds_bytes(Ly, (uint32_t)m.n_cells) + kLdsStaticReserve > e->max_lds) return false;
+    return sl_lds_bytes(Ly, (uint32_t)m.n_cells) + kLdsStaticReserve <= e->max_lds;
+}
+
+bool sublattice_supported(Engine* e, const Model& m) {
+    uint32_t bytes = 0;
+    bool gsig = false;
+    if (!lds_eligible(e, m, &bytes, &gsig)) return false;  // packed sigma words
     if (m.cfg.boundary == CPM_BOUNDARY_PERIODIC)
         for (int d = 0; d < m.D; ++d)
             if (m.dims[d] % 2) return false;
@@ -1920,12 +2235,10 @@ static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint
         for (size_t i = 0; i < n; ++i) {
             Model* m = models[i];
             upload(e, m);
-            const LdsLayout Ly = lds_layout((uint32_t)m->n_padded, (uint32_t)m->n_cells,
-                                            (uint32_t)m->cfg.types.size());
-            if (!m->dev->lds || sl_lds_bytes(Ly, (uint32_t)m->n_cells) + kLdsStaticReserve > e->max_lds)
+            if (!m->dev->lds)
                 throw Error(CPM_ERR_ARG,
-                            "sublattice mode needs the lattice resident in LDS; this lattice "
-                            "is too large");
+                            "sublattice mode needs packed lattice words (at most 4095 cells "
+                            "and 15 cell types)");
             if (m->cfg.boundary == CPM_BOUNDARY_PERIODIC)
                 for (int d = 0; d < m->D; ++d)
                     if (m->dims[d] % 2)
@@ -1938,6 +2251,7 @@ static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint
         ensure_buf(e->idx_dev, e->idx_cap, n);
         std::vector<Rep> reps(n);
         std::map<std::pair<int, uint32_t>, std::vector<uint32_t>> classes;
+        std::vector<uint32_t> grid_path;  // lattices too large for LDS
         for (size_t i = 0; i < n; ++i) {
             Model* m = models[i];
             Rep R = m->dev->rep;
@@ -1953,10 +2267,14 @@ static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint
                     m->has_sublattice_key = true;
                 }
                 R.fill_state[0] = m->sublattice_key;
-                const LdsLayout Ly = lds_layout((uint32_t)m->n_padded, (uint32_t)m->n_cells,
-                                                (uint32_t)m->cfg.types.size());
-                classes[{m->D, lds_class(e, sl_lds_bytes(Ly, (uint32_t)m->n_cells))}].push_back(
-                    (uint32_t)i);
+                if (sublattice_in_lds(e, *m)) {
+                    const LdsLayout Ly = lds_layout((uint32_t)m->n_padded, (uint32_t)m->n_cells,
+                                                    (uint32_t)m->cfg.types.size());
+                    classes[{m->D, lds_class(e, sl_lds_bytes(Ly, (uint32_t)m->n_cells))}]
+                        .push_back((uint32_t)i);
+                } else {
+                    grid_path.push_back((uint32_t)i);
+                }
             }
             reps[i] = R;
         }
@@ -1980,6 +2298,56 @@ static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint
             HIP_OK(hipGetLastError());
             first += cnt;
         }
+        // large replicas: cooperative launches per dimension, each replica
+        // owning an equal range of co-resident workgroups
+        if (!grid_path.empty()) {
+            ensure_buf(e->grid_counters, e->grid_counters_cap, 2);
+            for (int D : {2, 3}) {
+                std::vector<uint32_t> sel;
+                for (uint32_t i : grid_path)
+                    if (models[i]->D == D) sel.push_back(i);
+                if (sel.empty()) continue;
+                const void* fn = D == 2 ? (const void*)k_sublattice_grid<2>
+                                        : (const void*)k_sublattice_grid<3>;
+                int per_cu = 0;
+                HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kSlThreads, 0));
+                const uint64_t resident = (uint64_t)std::max(per_cu, 1) * e->n_cu;
+                for (size_t at = 0; at < sel.size();) {
+                    const size_t nr = std::min<size_t>(sel.size() - at, resident);
+                    uint64_t max_sites = 1, max_todo = 0;
+                    for (size_t k = at; k < at + nr; ++k) {
+                        const Model* m = models[sel[k]];
+                        uint64_t sites = 1;
+                        for (int d = 0; d < D; ++d) sites *= (m->dims[d] + 1) / 2;
+                        max_sites = std::max(max_sites, sites);
+                        max_todo = std::max<uint64_t>(max_todo, todo[sel[k]]);
+                    }
+                    uint32_t bpr = (uint32_t)std::max<uint64_t>(
+                        1, std::min<uint64_t>((max_sites + kSlThreads - 1) / kSlThreads,
+                                              resident / nr));
+                    uint32_t max_batches =
+                        (uint32_t)((max_sites + (uint64_t)bpr * kSlThreads - 1) /
+                                   ((uint64_t)bpr * kSlThreads));
+                    // one index entry per replica; workgroups map to it by division
+                    std::vector<uint32_t> ids(sel.begin() + at, sel.begin() + at + nr);
+                    ensure_buf(e->grid_idx, e->grid_idx_cap, ids.size());
+                    HIP_OK(hipMemcpyAsync(e->grid_idx, ids.data(), ids.size() * 4,
+                                          hipMemcpyHostToDevice, e->stream));
+                    Rep* reps_arg = e->reps_dev;
+                    const uint32_t* idx_arg = e->grid_idx;
+                    unsigned long long* cnt_arg = e->grid_counters;
+                    const uint64_t* sp3d = e->sp3d;
+                    void* args[] = {(void*)&reps_arg, (void*)&idx_arg, (void*)&bpr,
+                                    (void*)&max_todo, (void*)&max_batches, (void*)&cnt_arg,
+                                    (void*)&sp3d};
+                    HIP_OK(hipLaunchCooperativeKernel(fn, dim3((uint32_t)(nr * bpr)),
+                                                      dim3(kSlThreads), args, 0, e->stream));
+                    // the next launch reuses the index buffer
+                    HIP_OK(hipStreamSynchronize(e->stream));
+                    at += nr;
+                }
+            }
+        }
         HIP_OK(hipEventRecord(e->ev1, e->stream));
         HIP_OK(hipMemcpyAsync(reps.data(), e->reps_dev, n * sizeof(Rep), hipMemcpyDeviceToHost,
                               e->stream));
@@ -1994,8 +2362,8 @@ static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint
             add_counters(counters, reps[i]);
         }
     }
-    // the running energy is re-established from scratch: concurrent moves
-    // price dH against per-cell values other moves may be changing
+    // the running energy is recomputed from scratch: a running FP64 sum of
+    // concurrently accepted moves would depend on their order
     for (size_t i = 0; i < n; ++i) {
         if (!todo[i]) continue;
         sync_to_host(models[i]);

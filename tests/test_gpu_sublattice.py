@@ -64,20 +64,23 @@ def stats(res, side, boundary):
     return out
 
 
-def run_ensemble(mode, mcs, side, boundary):
+def run_ensemble(mode, mcs, side, boundary, seeds=SEEDS):
     sims = [sorting_sim(cpm.derive_seed_u64(2026, i), side=side, boundary=boundary)
-            for i in range(SEEDS)]
+            for i in range(seeds)]
     res = cpm.run_batch_configs(sims, burn_in_mcs=0, readout_mcs=mcs, sampling_interval_mcs=mcs,
                                 mode=mode)
     return [stats(r, side, boundary) for r in res]
 
 
-@pytest.mark.parametrize("boundary,side,mcs", [("periodic", 48, 400), ("fixed", 40, 300)])
-def test_sublattice_summary_statistics_match_exact_replay_ks(boundary, side, mcs):
+@pytest.mark.parametrize("boundary,side,mcs,seeds", [
+    ("periodic", 48, 400, SEEDS), ("fixed", 40, 300, SEEDS),
+    # too large for LDS: the grid-wide kernel (64 seeds keep the test short)
+    ("periodic", 280, 60, 64)])
+def test_sublattice_summary_statistics_match_exact_replay_ks(boundary, side, mcs, seeds):
     from scipy.stats import ks_2samp
 
-    exact = run_ensemble("exact_replay", mcs, side, boundary)
-    sub = run_ensemble("sublattice", mcs, side, boundary)
+    exact = run_ensemble("exact_replay", mcs, side, boundary, seeds)
+    sub = run_ensemble("sublattice", mcs, side, boundary, seeds)
     report = {}
     for key in exact[0]:
         a = np.array([x[key] for x in exact])
@@ -88,26 +91,48 @@ def test_sublattice_summary_statistics_match_exact_replay_ks(boundary, side, mcs
         assert p > 0.01, f"{key}: KS p={p:.4g} exact mean {ma:.4f} sublattice mean {mb:.4f}"
 
 
-@pytest.mark.parametrize("dim", [2, 3])
-def test_sublattice_bookkeeping_is_consistent(dim):
+def large_3d(seed=5):
+    grid = (48, 48, 48)
+    sim = cpm.CPM(grid=grid, boundary="fixed", seed=seed)
+    sim.register_cell_type("a", 300, 300, 3.0, 0.5)
+    sim.add_cells("a", 60)
+    sim.set_adhesion([[0.0, 8.0], [8.0, 3.0]])
+    sim.initialize(warn_on_default_init=False)
+    return sim, grid
+
+
+@pytest.mark.parametrize("case", ["2d", "3d", "2d_grid", "3d_grid"])
+def test_sublattice_bookkeeping_is_consistent(case):
     """checker.rs:36-136 on device snapshots: after parallel updates, V, I,
     contact and the position sums (mod the periodic box) equal a
     from-scratch recomputation; the energy is recomputed exactly; cells stay
-    non-empty."""
-    if dim == 2:
+    non-empty.  The _grid cases are too large for LDS and run the grid-wide
+    kernel."""
+    dim = 2 if case.startswith("2d") else 3
+    boundary = "periodic"
+    mcs = 50
+    if case == "2d":
         sim = sorting_sim(77, side=40)
         grid = (40, 40)
-    else:
+    elif case == "2d_grid":
+        sim = sorting_sim(78, side=284)
+        grid = (284, 284)
+        mcs = 10
+    elif case == "3d":
         grid = (16, 16, 16)
         sim = cpm.CPM(grid=grid, boundary="periodic", seed=5)
         sim.register_cell_type("a", 60, 100, 3.0, 0.5)
         sim.add_cells("a", 8)
         sim.set_adhesion([[0.0, 8.0], [8.0, 3.0]])
         sim.initialize(warn_on_default_init=False)
+    else:
+        sim, grid = large_3d()
+        boundary = "fixed"
+        mcs = 5
     rep = cpm.Replica(sim)
-    cnt = rep.run_mcs(50, mode="sublattice")
+    cnt = rep.run_mcs(mcs, mode="sublattice")
     # each site is attempted Binomial(4, 1/4) times per MCS: n_sites on average
-    n = 50 * int(np.prod(grid))
+    n = mcs * int(np.prod(grid))
     assert abs(cnt["attempts"] - n) < 6 * np.sqrt(n * 0.75)
     assert cnt["accepted"] > 0
     snap = rep.snapshot()
@@ -117,9 +142,11 @@ def test_sublattice_bookkeeping_is_consistent(dim):
     assert np.array_equal(vol, snap["volume"])
     assert snap["volume"].min() >= 1
     # recompute interface / contact with the oracle's definitions
-    cfg = O.Config(grid=grid, boundary="periodic", seed=0,
+    tspec = O.CellTypeSpec("a", 300, 300, 3.0, 0.5) if case == "3d_grid" else \
+        O.CellTypeSpec("a", 60, 100, 3.0, 0.5)
+    cfg = O.Config(grid=grid, boundary=boundary, seed=0,
                    cell_types=[O.CellTypeSpec(f"t{i}", 30, 24, 5.0, 1.0) for i in range(2)]
-                   if dim == 2 else [O.CellTypeSpec("a", 60, 100, 3.0, 0.5)],
+                   if dim == 2 else [tspec],
                    cell_counts=[1, 1] if dim == 2 else [1],
                    adhesion=[[0.0, 16.0, 16.0], [16.0, 3.0, 11.0], [16.0, 11.0, 5.0]]
                    if dim == 2 else [[0.0, 8.0], [8.0, 3.0]])
@@ -134,12 +161,13 @@ def test_sublattice_bookkeeping_is_consistent(dim):
     assert np.isclose(rep.global_energy(), snap["energy"], rtol=0, atol=0)
 
 
-def test_sublattice_is_reproducible_and_seed_sensitive():
-    a = cpm.Replica(sorting_sim(5, side=32))
-    b = cpm.Replica(sorting_sim(5, side=32))
-    c = cpm.Replica(sorting_sim(6, side=32))
+@pytest.mark.parametrize("side,mcs", [(32, 30), (284, 4)])
+def test_sublattice_is_reproducible_and_seed_sensitive(side, mcs):
+    a = cpm.Replica(sorting_sim(5, side=side))
+    b = cpm.Replica(sorting_sim(5, side=side))
+    c = cpm.Replica(sorting_sim(6, side=side))
     for r in (a, b, c):
-        r.run_mcs(30, mode="sublattice")
+        r.run_mcs(mcs, mode="sublattice")
     sa, sb, sc = a.snapshot(), b.snapshot(), c.snapshot()
     assert np.array_equal(sa["lattice"], sb["lattice"]) and sa["energy"] == sb["energy"]
     assert not np.array_equal(sa["lattice"], sc["lattice"])
