@@ -1289,7 +1289,10 @@ __global__ void __launch_bounds__(kSlThreads) k_sublattice(Rep* __restrict__ rep
 // barriers; a replica's batches are as many sites as its workgroups have
 // threads.  Every loop that contains a barrier has the same trip count on
 // every workgroup (max_todo MCS, max_batches batches per phase, rounds
-// until no replica has a pending attempt).
+// until no replica has a pending attempt).  A round costs one barrier: the
+// priority words and the pending counters rotate through three slots, so a
+// slot is cleared two rounds after its bids, when no workgroup can still be
+// reading it.
 // ===========================================================================
 __device__ __forceinline__ void grid_barrier(cooperative_groups::grid_group& grid) {
     __threadfence();
@@ -1322,7 +1325,7 @@ __global__ void __launch_bounds__(kSlThreads) k_sublattice_grid(
         (__attribute__((address_space(1))) uint32_t*)R.volume;
     __attribute__((address_space(1))) uint32_t* itf =
         (__attribute__((address_space(1))) uint32_t*)R.interface;
-    unsigned long long* prio = (unsigned long long*)R.scratch;  // [n_cells]
+    unsigned long long* prio = (unsigned long long*)R.scratch;  // [3][n_cells]
     const double* __restrict__ J = R.params + 2 * T;
     const double* __restrict__ lv = R.params;
     const double* __restrict__ li = R.params + T;
@@ -1333,8 +1336,8 @@ __global__ void __launch_bounds__(kSlThreads) k_sublattice_grid(
         const uint32_t id = R.sigma[i];
         sig[i] = (uint16_t)(id ? id | ((gtyp[id - 1] + 1) << 12) : 0u);
     }
-    for (uint32_t i = gtid; i < n_cells; i += gthreads) prio[i] = 0;
-    if (leader) pending_count[0] = pending_count[1] = 0;
+    for (uint32_t i = gtid; i < 3 * n_cells; i += gthreads) prio[i] = 0;
+    if (leader) pending_count[0] = pending_count[1] = pending_count[2] = 0;
     if (tid < 6) s_cnt[tid] = 0;
     grid_barrier(grid);
 
@@ -1351,6 +1354,9 @@ __global__ void __launch_bounds__(kSlThreads) k_sublattice_grid(
     unsigned long long c_att = 0, c_nn = 0, c_top = 0, c_en = 0, c_acc = 0;
     constexpr uint32_t NC = 1u << D;
     uint32_t round_no = 0;
+    // cells this thread bid for in each of the last three rounds (0: none),
+    // indexed by round % 3, cleared from their slot two rounds later
+    uint32_t owe_l[3] = {0, 0, 0}, owe_g[3] = {0, 0, 0};
 
     const uint64_t todo = R.todo;
     for (uint64_t m = 0; m < max_todo; ++m) {
@@ -1425,24 +1431,37 @@ __global__ void __launch_bounds__(kSlThreads) k_sublattice_grid(
                     }
                     for (;;) {
                         const uint32_t L = Lw & kIdMask, G = Gw & kIdMask;
-                        const uint32_t slot = round_no & 1;
-                        if (pending) {
-                            if (L) atomicMax(&prio[L - 1], pr);
-                            if (G) atomicMax(&prio[G - 1], pr);
-                            atomicAdd(&pending_count[slot], 1ull);
+                        const uint32_t slot = round_no % 3, next = (round_no + 1) % 3;
+                        unsigned long long* P = prio + (size_t)slot * n_cells;
+                        {   // clear this thread's bids of two rounds ago (slot `next`)
+                            unsigned long long* Q = prio + (size_t)next * n_cells;
+                            if (owe_l[next]) Q[owe_l[next] - 1] = 0;
+                            if (owe_g[next]) Q[owe_g[next] - 1] = 0;
+                            owe_l[next] = owe_g[next] = 0;
                         }
-                        if (leader) pending_count[slot ^ 1] = 0;
+                        if (pending) {
+                            if (L) atomicMax(&P[L - 1], pr);
+                            if (G) atomicMax(&P[G - 1], pr);
+                            atomicAdd(&pending_count[slot], 1ull);
+                            owe_l[slot] = L;
+                            owe_g[slot] = G;
+                        }
+                        if (leader) pending_count[next] = 0;
                         grid_barrier(grid);
                         const unsigned long long left = load_agent(&pending_count[slot]);
                         ++round_no;
-                        if (left == 0) break;
-                        const bool win = pending && (!L || load_agent(&prio[L - 1]) == pr) &&
-                                         (!G || load_agent(&prio[G - 1]) == pr);
-                        grid_barrier(grid);
-                        if (pending) {
-                            if (L) prio[L - 1] = 0;
-                            if (G) prio[G - 1] = 0;
+                        if (left == 0) {
+                            // clear the previous round's bids before their
+                            // slot comes round again (a barrier follows)
+                            const uint32_t prev = (slot + 2) % 3;
+                            unsigned long long* Q = prio + (size_t)prev * n_cells;
+                            if (owe_l[prev]) Q[owe_l[prev] - 1] = 0;
+                            if (owe_g[prev]) Q[owe_g[prev] - 1] = 0;
+                            owe_l[prev] = owe_g[prev] = 0;
+                            break;
                         }
+                        const bool win = pending && (!L || load_agent(&P[L - 1]) == pr) &&
+                                         (!G || load_agent(&P[G - 1]) == pr);
                         if (win) {
                             pending = false;
                             const uint32_t vL = L ? vol[L - 1] : 0u;
@@ -1555,7 +1574,6 @@ __global__ void __launch_bounds__(kSlThreads) k_sublattice_grid(
                                 }
                             }
                         }
-                        grid_barrier(grid);
                     }
                 }
             }
@@ -1797,7 +1815,7 @@ static Layout layout_of(const Model& m, bool with_draws) {
     L.targets = take(2 * T * 4);
     L.draws = with_draws ? take((size_t)kFill * 8 + (size_t)kSegs * 32) : 0;
     L.sig16 = with_draws ? take(m.n_padded * 2) : 0;
-    L.scratch = take(m.n_cells * 8 + 16);
+    L.scratch = take(m.n_cells * 24 + 16);  // grid sublattice: 3 priority slots
     L.total = off;
     return L;
 }
@@ -2301,7 +2319,7 @@ static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint
         // large replicas: cooperative launches per dimension, each replica
         // owning an equal range of co-resident workgroups
         if (!grid_path.empty()) {
-            ensure_buf(e->grid_counters, e->grid_counters_cap, 2);
+            ensure_buf(e->grid_counters, e->grid_counters_cap, 3);
             for (int D : {2, 3}) {
                 std::vector<uint32_t> sel;
                 for (uint32_t i : grid_path)
