@@ -988,6 +988,11 @@ __device__ __forceinline__ uint64_t splitmix_at(uint64_t key, uint64_t ctr) {
 }
 
 constexpr uint32_t kSlThreads = 256;
+// Sublattice lattices with at least this many sites run grid-wide
+// (k_sublattice_grid), smaller ones one workgroup per replica.  The choice
+// depends on the replica alone, so its trajectory does not depend on what
+// else is in the batch.
+constexpr uint64_t kSlGridSites = 16384;
 constexpr uint32_t kSlSub = 4;  // sub-sweeps per MCS (a power of two)
 
 // Dynamic LDS of the sublattice kernel: the exact-replay layout plus one
@@ -997,18 +1002,24 @@ __host__ __device__ inline uint32_t sl_lds_bytes(const LdsLayout& Ly, uint32_t n
     return Ly.total + (n_cells * 8 + 15) / 16 * 16;
 }
 
-template <int D>
-__global__ void __launch_bounds__(kSlThreads) k_sublattice(Rep* __restrict__ reps,
-                                                          const uint32_t* __restrict__ idx,
-                                                          const uint64_t* __restrict__ sp3d) {
+// NT threads per workgroup; SG: sigma in HBM (the packed copy) with the
+// per-cell tables and priorities still in LDS.
+template <int D, bool SG, uint32_t NT>
+__global__ void __launch_bounds__(NT) k_sublattice(Rep* __restrict__ reps,
+                                                   const uint32_t* __restrict__ idx,
+                                                   const uint64_t* __restrict__ sp3d) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ unsigned long long s_cnt[6];
     Rep& R = reps[idx[blockIdx.x]];
     const uint32_t tid = threadIdx.x;
     const uint32_t T = R.n_types, T1 = T + 1, n_cells = R.n_cells;
     const uint32_t n_padded = R.n_padded;
-    const LdsLayout Ly = lds_layout(n_padded, n_cells, T);
-    uint16_t* sig = (uint16_t*)(smem + Ly.sigma);  // packed: id | (type + 1) << 12
+    const LdsLayout Ly = lds_layout(n_padded, n_cells, T, !SG);
+    SigmaRef<SG> sig;  // packed: id | (type + 1) << 12
+    if constexpr (SG)
+        sig.p = (__attribute__((address_space(1))) uint16_t*)R.sig16;
+    else
+        sig.p = (uint16_t*)(smem + Ly.sigma);
     uint32_t* vol = (uint32_t*)(smem + Ly.vol);
     uint32_t* itf = (uint32_t*)(smem + Ly.itf);
     double* J = (double*)(smem + Ly.J);
@@ -1018,23 +1029,24 @@ __global__ void __launch_bounds__(kSlThreads) k_sublattice(Rep* __restrict__ rep
     uint32_t* ti = (uint32_t*)(smem + Ly.ti);
     unsigned long long* prio = (unsigned long long*)(smem + sl_prio_offset(Ly));
     const uint32_t* __restrict__ gtyp = R.type_index;
-    for (uint32_t i = tid; i < n_padded; i += kSlThreads) {
+    for (uint32_t i = tid; i < n_padded; i += NT) {
         const uint32_t id = R.sigma[i];
-        sig[i] = (uint16_t)(id ? id | ((gtyp[id - 1] + 1) << 12) : 0u);
+        sig.st(i, id ? id | ((gtyp[id - 1] + 1) << 12) : 0u);
     }
-    for (uint32_t i = tid; i < n_cells; i += kSlThreads) {
+    for (uint32_t i = tid; i < n_cells; i += NT) {
         vol[i] = R.volume[i];
         itf[i] = R.interface[i];
         prio[i] = 0;
     }
-    for (uint32_t i = tid; i < T1 * T1; i += kSlThreads) J[i] = R.params[2 * T + i];
-    for (uint32_t i = tid; i < T; i += kSlThreads) {
+    for (uint32_t i = tid; i < T1 * T1; i += NT) J[i] = R.params[2 * T + i];
+    for (uint32_t i = tid; i < T; i += NT) {
         lv[i] = R.params[i];
         li[i] = R.params[T + i];
         tv[i] = R.targets[i];
         ti[i] = R.targets[T + i];
     }
     if (tid < 6) s_cnt[tid] = 0;
+    if constexpr (SG) __threadfence();  // the staged sigma words, HBM
     __syncthreads();
 
     uint32_t dims[D], stride[D];
@@ -1078,7 +1090,7 @@ __global__ void __launch_bounds__(kSlThreads) k_sublattice(Rep* __restrict__ rep
                 if (nsite == 0) continue;
                 // batches start at a random rotation of the colour's sites
                 const uint32_t rot = (uint32_t)(splitmix_at(key ^ 0x5bd1e995ull, phase) % nsite);
-                for (uint32_t b0 = 0; b0 < nsite; b0 += kSlThreads) {
+                for (uint32_t b0 = 0; b0 < nsite; b0 += NT) {
                     const uint32_t jb = b0 + tid;
                     uint32_t j = 0, c[D], t = 0, s = 0, Lw = 0, Gw = 0;
                     unsigned long long pr = 0;
@@ -1100,8 +1112,8 @@ __global__ void __launch_bounds__(kSlThreads) k_sublattice(Rep* __restrict__ rep
                             const uint32_t k = (uint32_t)(((h0 >> 32) * (uint64_t)n_copy) >> 32);
                             s = (uint32_t)((int32_t)t + R.copy_off[k]);
                             c_att++;
-                            Lw = sig[t];
-                            Gw = sig[s];
+                            Lw = sig.ld(t);
+                            Gw = sig.ld(s);
                             if (Lw != Gw) {
                                 c_nn++;
                                 const uint32_t L = Lw & kIdMask;
@@ -1109,7 +1121,7 @@ __global__ void __launch_bounds__(kSlThreads) k_sublattice(Rep* __restrict__ rep
                                 if (L != 0) {
                                     uint32_t pat = 0;
                                     for (uint32_t i = 0; i < n_topo; ++i)
-                                        pat |= (uint32_t)((sig[(int32_t)t + R.topo_off[i]] & kIdMask) == L) << i;
+                                        pat |= (uint32_t)((sig.ld((int32_t)t + R.topo_off[i]) & kIdMask) == L) << i;
                                     c_top++;
                                     simple = D == 2 ? ((c_sp2d[pat >> 5] >> (pat & 31)) & 1u)
                                                     : ((sp3d[pat >> 6] >> (pat & 63)) & 1ull);
@@ -1149,7 +1161,7 @@ __global__ void __launch_bounds__(kSlThreads) k_sublattice(Rep* __restrict__ rep
                                 int nl = 0, ng = 0;
                                 double dadh = 0.0;  // energy.rs:242-264, stencil order
                                 for (uint32_t q = 0; q < n_energy; ++q) {
-                                    const uint32_t w = sig[(int32_t)t + R.energy_off[q]];
+                                    const uint32_t w = sig.ld((int32_t)t + R.energy_off[q]);
                                     const uint32_t nb = w & kIdMask, ni = w >> 12;
                                     nl += nb == L;
                                     ng += nb == G;
@@ -1186,13 +1198,13 @@ __global__ void __launch_bounds__(kSlThreads) k_sublattice(Rep* __restrict__ rep
                                     c_acc++;
                                     // dynamics.rs:73-98: contacts on the pre-move lattice
                                     for (uint32_t q = 0; q < n_energy; ++q) {
-                                        const uint32_t nb = sig[(int32_t)t + R.energy_off[q]] & kIdMask;
+                                        const uint32_t nb = sig.ld((int32_t)t + R.energy_off[q]) & kIdMask;
                                         if (nb != L)
                                             atomicSub(&R.contact[(uint64_t)min(L, nb) * ncell1 + max(L, nb)], 1u);
                                         if (nb != G)
                                             atomicAdd(&R.contact[(uint64_t)min(G, nb) * ncell1 + max(G, nb)], 1u);
                                     }
-                                    sig[t] = (uint16_t)Gw;
+                                    sig.st(t, Gw);
                                     // this round owns L's and G's V, I and position sums
                                     if (L) {
                                         vol[L - 1] = vL - 1;
@@ -1247,13 +1259,13 @@ __global__ void __launch_bounds__(kSlThreads) k_sublattice(Rep* __restrict__ rep
                                                 f += opt[d][r % nopt[d]] * stride[d];
                                                 r /= nopt[d];
                                             }
-                                            sig[f] = (uint16_t)Gw;
+                                            sig.st(f, Gw);
                                         }
                                     }
                                 }
                             }
                         }
-                        __threadfence_block();
+                        if constexpr (SG) __threadfence();  // sigma writes, HBM
                     }
                 }
                 __syncthreads();
@@ -1266,8 +1278,8 @@ __global__ void __launch_bounds__(kSlThreads) k_sublattice(Rep* __restrict__ rep
     atomicAdd(&s_cnt[3], c_en);
     atomicAdd(&s_cnt[4], c_acc);
     __syncthreads();
-    for (uint32_t i = tid; i < n_padded; i += kSlThreads) R.sigma[i] = sig[i] & kIdMask;
-    for (uint32_t i = tid; i < n_cells; i += kSlThreads) {
+    for (uint32_t i = tid; i < n_padded; i += NT) R.sigma[i] = sig.ld(i) & kIdMask;
+    for (uint32_t i = tid; i < n_cells; i += NT) {
         R.volume[i] = vol[i];
         R.interface[i] = itf[i];
     }
@@ -1720,8 +1732,8 @@ Engine* engine_create(int device) {
     HIP_OK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device));
     e->n_cu = (uint32_t)n_cu;
 #define CPM_FN(k, d, ne, gs) (const void*)k_lds_engine<d, ne, gs>,
-    for (auto fn : {CPM_WAVE_KERNELS(CPM_FN)(const void*)k_sublattice<2>,
-                    (const void*)k_sublattice<3>})
+    for (auto fn : {CPM_WAVE_KERNELS(CPM_FN)(const void*)k_sublattice<2, false, kSlThreads>,
+                    (const void*)k_sublattice<3, false, kSlThreads>})
 #undef CPM_FN
         HIP_OK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    lds - kLdsStaticReserve));
@@ -1941,14 +1953,16 @@ size_t device_footprint(Engine* e, const Model& m) {
     return layout_of(m, lds_eligible(e, m, &bytes, &gsig)).total;
 }
 
-// Sublattice mode runs in LDS (k_sublattice) when sigma, the tables and the
-// priority words fit, else grid-wide (k_sublattice_grid).
-static bool sublattice_in_lds(Engine* e, const Model& m) {
-    uint32_t bytes = 0;
-    if (!lds_eligible(e, m, &bytes)) return false;
-    const LdsLayout Ly = lds_layout((uint32_t)m.n_padded, (uint32_t)m.n_cells,
-                                    (uint32_t)m.cfg.types.size());
-    return sl_lds_bytes(Ly, (uint32_t)m.n_cells) + kLdsStaticReserve <= e->max_lds;
+// LDS bytes of k_sublattice for a model: sigma in LDS when it fits with the
+// tables and priority words (*sg = false), else the tables alone.
+static uint32_t sublattice_lds(Engine* e, const Model& m, bool* sg) {
+    const uint32_t T = (uint32_t)m.cfg.types.size();
+    const LdsLayout in = lds_layout((uint32_t)m.n_padded, (uint32_t)m.n_cells, T, true);
+    const uint32_t with_sigma = sl_lds_bytes(in, (uint32_t)m.n_cells);
+    *sg = with_sigma + kLdsStaticReserve > e->max_lds;
+    if (!*sg) return with_sigma;
+    const LdsLayout out = lds_layout((uint32_t)m.n_padded, (uint32_t)m.n_cells, T, false);
+    return sl_lds_bytes(out, (uint32_t)m.n_cells);
 }
 
 bool sublattice_supported(Engine* e, const Model& m) {
@@ -2269,7 +2283,7 @@ static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint
         ensure_buf(e->idx_dev, e->idx_cap, n);
         std::vector<Rep> reps(n);
         std::map<std::pair<int, uint32_t>, std::vector<uint32_t>> classes;
-        std::vector<uint32_t> grid_path;  // lattices too large for LDS
+        std::vector<uint32_t> grid_path;  // lattices of kSlGridSites sites and more
         for (size_t i = 0; i < n; ++i) {
             Model* m = models[i];
             Rep R = m->dev->rep;
@@ -2285,14 +2299,12 @@ static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint
                     m->has_sublattice_key = true;
                 }
                 R.fill_state[0] = m->sublattice_key;
-                if (sublattice_in_lds(e, *m)) {
-                    const LdsLayout Ly = lds_layout((uint32_t)m->n_padded, (uint32_t)m->n_cells,
-                                                    (uint32_t)m->cfg.types.size());
-                    classes[{m->D, lds_class(e, sl_lds_bytes(Ly, (uint32_t)m->n_cells))}]
-                        .push_back((uint32_t)i);
-                } else {
+                bool sg = false;
+                const uint32_t bytes = sublattice_lds(e, *m, &sg);
+                if (m->n_sites < kSlGridSites && !sg && bytes + kLdsStaticReserve <= e->max_lds)
+                    classes[{m->D * 100, lds_class(e, bytes)}].push_back((uint32_t)i);
+                else
                     grid_path.push_back((uint32_t)i);
-                }
             }
             reps[i] = R;
         }
@@ -2306,15 +2318,20 @@ static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint
         HIP_OK(hipEventRecord(e->ev0, e->stream));
         uint32_t first = 0;
         for (auto& [key, v] : classes) {
-            const uint32_t cnt = (uint32_t)v.size();
-            if (key.first == 2)
-                hipLaunchKernelGGL(k_sublattice<2>, dim3(cnt), dim3(kSlThreads), key.second,
-                                   e->stream, e->reps_dev, e->idx_dev + first, e->sp3d);
-            else
-                hipLaunchKernelGGL(k_sublattice<3>, dim3(cnt), dim3(kSlThreads), key.second,
-                                   e->stream, e->reps_dev, e->idx_dev + first, e->sp3d);
+            const dim3 g((uint32_t)v.size());
+            const uint32_t lds = key.second;
+            Rep* rp = e->reps_dev;
+            const uint32_t* ix = e->idx_dev + first;
+            switch (key.first) {
+#define CPM_SL(k, d, sg, nt)                                                                 \
+    case k: hipLaunchKernelGGL((k_sublattice<d, sg, nt>), g, dim3(nt), lds, e->stream, rp, ix, \
+                               e->sp3d); break;
+                CPM_SL(200, 2, false, kSlThreads) CPM_SL(300, 3, false, kSlThreads)
+#undef CPM_SL
+                default: throw Error(CPM_ERR_DEVICE, "no sublattice kernel for this key");
+            }
             HIP_OK(hipGetLastError());
-            first += cnt;
+            first += (uint32_t)v.size();
         }
         // large replicas: cooperative launches per dimension, each replica
         // owning an equal range of co-resident workgroups
@@ -2330,19 +2347,23 @@ static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint
                 int per_cu = 0;
                 HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kSlThreads, 0));
                 const uint64_t resident = (uint64_t)std::max(per_cu, 1) * e->n_cu;
+                // every replica covers a whole colour phase in one batch
+                // (its trajectory then does not depend on the launch it is
+                // in), so a launch takes as many replicas as fit resident
+                uint64_t max_sites = 1;
+                for (uint32_t i : sel) {
+                    uint64_t sites = 1;
+                    for (int d = 0; d < D; ++d) sites *= (models[i]->dims[d] + 1) / 2;
+                    max_sites = std::max(max_sites, sites);
+                }
+                const uint32_t bpr = (uint32_t)std::max<uint64_t>(
+                    1, std::min<uint64_t>((max_sites + kSlThreads - 1) / kSlThreads, resident));
+                const size_t per_launch = std::max<size_t>(1, resident / bpr);
                 for (size_t at = 0; at < sel.size();) {
-                    const size_t nr = std::min<size_t>(sel.size() - at, resident);
-                    uint64_t max_sites = 1, max_todo = 0;
-                    for (size_t k = at; k < at + nr; ++k) {
-                        const Model* m = models[sel[k]];
-                        uint64_t sites = 1;
-                        for (int d = 0; d < D; ++d) sites *= (m->dims[d] + 1) / 2;
-                        max_sites = std::max(max_sites, sites);
+                    const size_t nr = std::min<size_t>(sel.size() - at, per_launch);
+                    uint64_t max_todo = 0;
+                    for (size_t k = at; k < at + nr; ++k)
                         max_todo = std::max<uint64_t>(max_todo, todo[sel[k]]);
-                    }
-                    uint32_t bpr = (uint32_t)std::max<uint64_t>(
-                        1, std::min<uint64_t>((max_sites + kSlThreads - 1) / kSlThreads,
-                                              resident / nr));
                     uint32_t max_batches =
                         (uint32_t)((max_sites + (uint64_t)bpr * kSlThreads - 1) /
                                    ((uint64_t)bpr * kSlThreads));

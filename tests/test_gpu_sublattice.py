@@ -171,3 +171,22 @@ def test_sublattice_is_reproducible_and_seed_sensitive(side, mcs):
     sa, sb, sc = a.snapshot(), b.snapshot(), c.snapshot()
     assert np.array_equal(sa["lattice"], sb["lattice"]) and sa["energy"] == sb["energy"]
     assert not np.array_equal(sa["lattice"], sc["lattice"])
+
+
+def test_sublattice_result_does_not_depend_on_the_batch():
+    """A replica's sublattice trajectory is a function of the replica alone:
+    the same alone as in a batch of other sizes, on both kernels (the
+    workgroup-per-replica kernel below 16384 sites, grid-wide above)."""
+    def snap(sim, others, mcs):
+        reps = [cpm.Replica(sim)] + [cpm.Replica(o) for o in others]
+        cpm.Replica.run_mcs_many(reps, mcs, mode="sublattice")
+        return reps[0].snapshot()
+
+    for side, mcs in ((40, 12), (160, 3)):
+        alone = snap(sorting_sim(404, side=side), [], mcs)
+        mixed = snap(sorting_sim(404, side=side),
+                     [sorting_sim(500 + i, side=s) for i, s in enumerate((32, 48, 130, 190, 64))],
+                     mcs)
+        assert np.array_equal(alone["lattice"], mixed["lattice"]), side
+        assert np.array_equal(alone["volume"], mixed["volume"]), side
+        assert alone["energy"] == mixed["energy"], side
