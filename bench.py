@@ -184,6 +184,89 @@ def serving_trace(cpm, n_requests: int = 1500, rate: float = 200.0, seed: int = 
                    "U[40,160] MCS per request"}
 
 
+def sorting_256(cpm, seed: int = 1):
+    """The single-replica sorting workload: 256x256 periodic, 4 cell types
+    scattered at random, phi = 0.8 via cells_for_density with V* = 40,
+    I* = 30, lambda_V = 5, lambda_I = 1, J medium row 16 and type pairs
+    U[2,14] drawn from the seed."""
+    rng = np.random.default_rng(seed)
+    T = 4
+    J = np.zeros((T + 1, T + 1))
+    J[0, 1:] = J[1:, 0] = 16.0
+    for a in range(1, T + 1):
+        for b in range(a, T + 1):
+            J[a, b] = J[b, a] = float(rng.uniform(2, 14))
+    sim = cpm.CPM(grid=(256, 256), boundary="periodic", seed=seed)
+    n = cpm.cells_for_density((256, 256), 40, 0.8)
+    for t in range(T):
+        sim.register_cell_type(f"t{t}", 40, 30, 5.0, 1.0)
+        sim.add_cells(f"t{t}", n // T + (1 if t < n % T else 0))
+    sim.set_adhesion(J.tolist())
+    sim.initialize(warn_on_default_init=False)
+    return sim
+
+
+def organoid_64(cpm, seed: int = 1):
+    """The 3D organoid workload: 64^3 fixed boundary, 3 cell types in nested
+    spherical shells (radii 10 / 18 / 24 sites) cut into cells of about
+    V* = 1048 sites (Voronoi within each shell, each cell reduced to its
+    largest face-connected piece), I* = 2760, lambda_V = 10, lambda_I = 2,
+    J medium row 16 and type pairs U[2,16] drawn from the seed."""
+    from scipy import ndimage
+
+    rng = np.random.default_rng(seed)
+    L = 64
+    c = (L - 1) / 2.0
+    z, y, x = np.meshgrid(np.arange(L), np.arange(L), np.arange(L), indexing="ij")
+    r = np.sqrt((x - c) ** 2 + (y - c) ** 2 + (z - c) ** 2)
+    lat = np.zeros((L, L, L), dtype=np.int64)
+    type_of, nid = [], 0
+    for t, (r0, r1) in enumerate(((0, 10), (10, 18), (18, 24))):
+        pts = np.argwhere((r >= r0) & (r < r1))
+        k = max(1, int(round(len(pts) / 1048)))
+        seeds = pts[rng.choice(len(pts), k, replace=False)]
+        own = ((pts[:, None, :] - seeds[None, :, :]) ** 2).sum(-1).argmin(1)
+        lat[tuple(pts.T)] = own + 1 + nid
+        type_of += [t] * k
+        nid += k
+    for cid in range(1, nid + 1):
+        m = lat == cid
+        lab, nl = ndimage.label(m)
+        if nl > 1:
+            sizes = ndimage.sum(m, lab, range(1, nl + 1))
+            lat[m & (lab != 1 + int(np.argmax(sizes)))] = 0
+    T = 3
+    J = np.zeros((T + 1, T + 1))
+    J[0, 1:] = J[1:, 0] = 16.0
+    for a in range(1, T + 1):
+        for b in range(a, T + 1):
+            J[a, b] = J[b, a] = float(rng.uniform(2, 16))
+    sim = cpm.CPM(grid=(L, L, L), boundary="fixed", seed=seed)
+    for t in range(T):
+        sim.register_cell_type(f"t{t}", 1048, 2760, 10.0, 2.0)
+    sim.set_adhesion(J.tolist())
+    sim.initialize(lattice=lat.ravel().tolist(), cell_type_of=type_of)
+    return sim
+
+
+def single_replica_rates(cpm):
+    """Copy attempts/s of one replica of each single-lattice workload, in
+    exact replay and in sublattice mode (short runs after one warm MCS)."""
+    out = {}
+    for name, make, mcs in (("sorting_256x256_4types", sorting_256, 20),
+                            ("organoid_64cubed_3types", organoid_64, 4)):
+        row = {}
+        for mode in ("exact_replay", "sublattice"):
+            rep = cpm.Replica(make(cpm))
+            rep.run_mcs(1, mode=mode)
+            t0 = time.perf_counter()
+            c = rep.run_mcs(mcs, mode=mode)
+            row[mode + "_attempts_per_s"] = c["attempts"] / (time.perf_counter() - t0)
+        row["mcs_timed"] = mcs
+        out[name] = row
+    return out
+
+
 def cpu_baseline(params, seconds: float = 12.0):
     """The oracle (C restatement of the reference loop) on the host cores:
     replicas of the same workload advanced by one sampling interval each,
@@ -290,6 +373,10 @@ def main():
     step_s = wall / args.steps
     sims_per_hour = len(reps) * 3600.0 / (t_init + 20 * step_s)
 
+    single = single_replica_rates(cpm)
+    print("[bench] single replicas: " + ", ".join(
+        f"{k} exact {v['exact_replay_attempts_per_s'] / 1e6:.1f} M/s, sublattice "
+        f"{v['sublattice_attempts_per_s'] / 1e6:.1f} M/s" for k, v in single.items()), flush=True)
     serving = None
     if not args.no_serving:
         serving = serving_trace(cpm)
@@ -327,6 +414,7 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
         "serving": serving,
+        "single_replica": single,
     }
     print(json.dumps(line), flush=True)
 

@@ -1668,9 +1668,13 @@ struct Device {
 // LDS kept free of the dynamic allocation for the kernels' static LDS
 // (the sublattice kernel's counters and the compiler's own use)
 constexpr uint32_t kLdsStaticReserve = 1024;
-// Share of the chip's LDS the sigma placement plans to fill: workgroups of
-// mixed sizes do not pack a CU's LDS perfectly.
+// Share of the chip's LDS the sigma placement plans to fill (workgroups of
+// mixed sizes do not pack a CU's LDS perfectly): the default, and the
+// candidates the autotuner tries (0: sigma in HBM for every replica; 8:
+// sigma in LDS wherever it fits).
 constexpr double kLdsPlacementFill = 0.75;
+constexpr double kFillCandidates[] = {0.0, 0.5, 0.75, 1.0, 8.0};
+constexpr int kNumFillCandidates = sizeof(kFillCandidates) / sizeof(kFillCandidates[0]);
 
 struct Engine {
     int device = 0;
@@ -1689,6 +1693,8 @@ struct Engine {
     uint32_t max_lds = 0;
     uint32_t n_cu = 0;
     std::vector<hipEvent_t> launch_ev;  // start/stop pairs around replay launches
+    // autotuned sigma placement: batch signature -> best fill candidate
+    std::map<uint64_t, int> tuned_fill;
     uint32_t* grid_idx = nullptr;       // grid sublattice: replica per workgroup range
     size_t grid_idx_cap = 0;
     unsigned long long* grid_counters = nullptr;  // grid sublattice: round counters
@@ -2043,15 +2049,10 @@ static void launch_lds_engine(int key, uint32_t lds, uint32_t count, hipStream_t
     }
 }
 
-void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* todo, int mode,
-                  cpm_counters* counters) {
-    if (mode == CPM_MODE_SUBLATTICE) {
-        run_sublattice(e, models, n, todo, counters);
-        return;
-    }
-    if (mode != CPM_MODE_EXACT_REPLAY) throw Error(CPM_ERR_ARG, "unknown mode");
-    std::lock_guard<std::mutex> lock(e->mu);
-    HIP_OK(hipSetDevice(e->device));
+// One exact-replay call with a given sigma placement fill (caller holds the
+// engine lock).
+static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t* todo,
+                      double fill, cpm_counters* counters) {
     bool any3d = false;
     for (size_t i = 0; i < n; ++i) {
         upload(e, models[i]);
@@ -2081,7 +2082,7 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
         std::stable_sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) {
             return models[a]->dev->lds_bytes < models[b]->dev->lds_bytes;
         });
-        const double capacity = kLdsPlacementFill * (double)e->n_cu * (double)e->max_lds;
+        const double capacity = fill * (double)e->n_cu * (double)e->max_lds;
         for (uint32_t i : cand) {
             const Device* d = models[i]->dev;
             const double extra =
@@ -2408,6 +2409,83 @@ static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint
         sync_to_host(models[i]);
         models[i]->energy = models[i]->global_energy();
     }
+}
+
+// Coarse signature of an exact-replay batch for the placement autotuner:
+// dimensions present, log2 of the replica count and the batch's LDS demand
+// (sigma in LDS everywhere it fits) in quarters of the chip's LDS.
+static uint64_t batch_signature(Engine* e, Model* const* models, size_t n, const uint64_t* todo) {
+    double demand = 0.0;
+    uint64_t dims = 0, count = 0;
+    for (size_t i = 0; i < n; ++i) {
+        if (!todo[i] || !models[i]->dev || !models[i]->dev->lds) continue;
+        demand += (double)models[i]->dev->lds_bytes;
+        dims |= 1ull << models[i]->D;
+        ++count;
+    }
+    const uint64_t q = (uint64_t)std::min(64.0, 4.0 * demand / ((double)e->n_cu * e->max_lds));
+    uint64_t lg = 0;
+    while ((1ull << lg) < count) ++lg;
+    return dims | (lg << 8) | (q << 16);
+}
+
+// Exact replay with an autotuned sigma placement.  Results do not depend on
+// the placement or on how the MCS are split into launches, so the first
+// large enough call of a new batch signature is split: a slice of its MCS
+// under each fill candidate, timed, then the rest under the fastest, which
+// later calls with that signature reuse.
+void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* todo, int mode,
+                  cpm_counters* counters) {
+    if (mode == CPM_MODE_SUBLATTICE) {
+        run_sublattice(e, models, n, todo, counters);
+        return;
+    }
+    if (mode != CPM_MODE_EXACT_REPLAY) throw Error(CPM_ERR_ARG, "unknown mode");
+    std::lock_guard<std::mutex> lock(e->mu);
+    HIP_OK(hipSetDevice(e->device));
+    for (size_t i = 0; i < n; ++i) upload(e, models[i]);
+    const uint64_t sig = batch_signature(e, models, n, todo);
+    auto it = e->tuned_fill.find(sig);
+    uint64_t max_todo = 0;
+    for (size_t i = 0; i < n; ++i) max_todo = std::max(max_todo, todo[i]);
+    const uint64_t slice = max_todo / 32;  // per candidate: ~3 % of the call
+    if (it != e->tuned_fill.end() || slice == 0 || n < 2) {
+        run_exact(e, models, n, todo,
+                  it != e->tuned_fill.end() ? kFillCandidates[it->second] : kLdsPlacementFill,
+                  counters);
+        return;
+    }
+    std::vector<uint64_t> left(todo, todo + n), part(n);
+    int best = -1;
+    double best_rate = 0.0;
+    for (int c = 0; c < kNumFillCandidates; ++c) {
+        for (size_t i = 0; i < n; ++i) {
+            part[i] = std::min(left[i], slice);
+            left[i] -= part[i];
+        }
+        cpm_counters cc{};
+        run_exact(e, models, n, part.data(), kFillCandidates[c], &cc);
+        const double rate = cc.kernel_ms > 0 ? (double)cc.attempts / cc.kernel_ms : 0.0;
+        if (best < 0 || rate > best_rate) {
+            best = c;
+            best_rate = rate;
+        }
+        if (counters) {
+            counters->attempts += cc.attempts;
+            counters->non_null += cc.non_null;
+            counters->topology += cc.topology;
+            counters->energy += cc.energy;
+            counters->accepted += cc.accepted;
+            counters->exp_near_ties += cc.exp_near_ties;
+            counters->kernel_ms += cc.kernel_ms;
+            counters->replay_launches += cc.replay_launches;
+            counters->replay_launch_ms += cc.replay_launch_ms;
+        }
+    }
+    e->tuned_fill[sig] = best;
+    bool more = false;
+    for (size_t i = 0; i < n; ++i) more |= left[i] != 0;
+    if (more) run_exact(e, models, n, left.data(), kFillCandidates[best], counters);
 }
 
 void run_mcs(Engine* e, Model* const* models, size_t n, uint64_t n_mcs, int mode,
