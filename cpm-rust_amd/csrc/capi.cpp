@@ -435,6 +435,7 @@ cpm_status cpm_serve_stats(cpm_server* server, cpm_server_stats* out) {
         out->rounds = st.rounds;
         out->attempts = st.attempts;
         out->busy_s = st.busy_s;
+        out->device_s = st.device_s;
         out->uptime_s = st.uptime_s;
     });
 }
