@@ -441,22 +441,19 @@ __host__ __device__ constexpr int shell_index(int D, int NE, int q) {
     return -1;
 }
 
-// Per-lane speculative evaluation of the attempt decoded at this lane's
-// window position, on the current lattice (dynamics.rs:22-69 and
-// energy.rs:521-523): early-out, simple-point and volume checks, the local
-// dH in FP64 in the reference's order, and the Metropolis test with the
-// u draw taken from the window when dH > 0.
-struct LaneEval {
-    uint32_t next;  // window position after the attempt
-    uint32_t Lw, Gw;
-    int32_t dl, dg;
-    uint32_t vL, vG, iL, iG;
-    double dh;
-};
-
-struct EvalMasks {
-    uint64_t ok, null_, acc, top, en, tie;
-};
+// Meta word of a decoded attempt, per slot and lane of the wave path: the
+// window position after its draws, and what its evaluation found.
+constexpr uint32_t kMe = 0xFF;      // position after the attempt's draws (window-relative)
+constexpr uint32_t kMv = 1u << 8;   // every draw inside the 128-draw window
+constexpr uint32_t kMn = 1u << 9;   // non-null: sigma(t) != sigma(s)
+constexpr uint32_t kMu = 1u << 10;  // dH > 0 after the checks: u is drawn
+constexpr uint32_t kMa = 1u << 11;  // accepted
+constexpr uint32_t kMp = 1u << 12;  // topology checked (t not medium)
+constexpr uint32_t kMg = 1u << 13;  // passed the checks: energy evaluated
+constexpr uint32_t kMt = 1u << 14;  // u within 4 ulp of exp(-dH)
+// LDS the wave path adds after the replica's tables: one evaluation pass's
+// 64 staged attempts (target, source, u draw).
+constexpr uint32_t kStageBytes = 64 * 16;
 
 // Packed sigma words, in LDS or (G) in HBM.  The HBM copy is read through
 // global-address-space loads (flat loads would also count against the LDS
@@ -474,15 +471,17 @@ struct SigmaRef<true> {
     __device__ void st(int64_t i, uint32_t v) const { p[i] = (uint16_t)v; }
 };
 
-// Exact replay on the wave path: one wave per replica.  Each lane decodes
-// the attempt that would start at its own position of the 64-draw window
-// (the first accepted draw of each consumer, found in the 128-draw window
-// by mask search) and evaluates it completely against the current lattice.
-// The wave then walks the actual chain of attempts through those
-// evaluations (a lane read per attempt) up to the first accepted copy,
-// commits it, re-evaluates, and goes on.  Accepted copies are rare, so an
-// attempt costs about 1/20 of a lane-parallel evaluation.  NE is the
-// energy stencil size; NT = 3^D - 1 the full shell.
+// Exact replay on the wave path: one wave per replica.  The stream is cut
+// into 64-position windows; lane i of a window decodes the attempt that
+// would start at window position i (the first accepted draw of each
+// consumer, found in the window's 128 draws by mask search).  A block of KB
+// windows is decoded at once; its attempts that are not null on the current
+// lattice (about one in six) are packed onto consecutive lanes through LDS
+// and evaluated completely, 64 per pass.  The wave then walks the actual
+// chain of attempts through each window (pointer doubling) up to the first
+// accepted copy, evaluates that attempt again on every lane, commits it,
+// re-evaluates the rest of the block and goes on.  NE is the energy stencil
+// size; NT = 3^D - 1 the full shell.
 template <int D, int NE, bool SG>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 ? 4 : 1))) k_lds_engine(Rep* __restrict__ reps,
                                                    const uint32_t* __restrict__ idx,
@@ -572,64 +571,96 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 
         wave_fill(gdraws, segs, s0, lane);
     }
 
-    // ---- the 128-draw register window: halves [base, base+64), [base+64, base+128)
+    // ---- blocks of KB consecutive 64-position windows.  `base` is the
+    // buffer position where slot 0's window starts, p the chain position in
+    // it; the decode of the window in slot k reads the 128 draws from its
+    // start, halves hr[k] and hr[k + 1] of the block.
+    constexpr int KB = D == 2 ? 3 : 2;
+    uint4* const stage = (uint4*)(smem + Ly.total);  // one evaluation pass's attempts
     uint32_t base = 0, p = 0;
-    Half h0, h1;
-    uint64_t m0[D + 1], m1[D + 1];
-    bool h1_valid = false;
-    uint64_t w_next = 0;
-    // per-lane decode of the attempt starting at this lane's window position
-    uint32_t dec_t = 0, dec_s = 0, dec_end = 0, dec_c[D];
-    bool dec_ok = false;
-    LaneEval ev;
-    EvalMasks M;
-    bool need_decode = true, need_eval = true;
+    uint64_t hr[KB + 1];
+    uint32_t st_t[KB], st_s[KB], st_m[KB];  // per slot and lane: target, source, meta
+    uint64_t st_u[KB];                      // the draw after the attempt (its u when dH > 0)
+#pragma unroll
+    for (int j = 0; j <= KB; ++j) hr[j] = 0;
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+        st_t[k] = st_s[k] = t_safe;
+        st_m[k] = 0;
+        st_u[k] = 0;
+    }
+    uint32_t nvalid = 0, commit_p = 0;
+    bool need_block = true, need_eval = false, hr_ready = false, pending = false;
 
-#define CPM_DECODE()                                                                         \
+    // The buffer is restarted at the segment holding position `at` (its
+    // draws regenerated from the segment's start state).
+#define CPM_REFILL(at_)                                                                      \
     do {                                                                                     \
-        const uint64_t hv = h1_valid ? ~0ull : 0ull;                                         \
+        const uint32_t at = (at_), g = min(at / kSegLen, kSegs - 1);                          \
+        /* a vector load: the segment states were written by vector stores */                \
+        const uint64_t mine = segs[g * 4 + (lane & 3)];                                       \
+        const uint64_t s0[4] = {rl64(mine, 0), rl64(mine, 1), rl64(mine, 2), rl64(mine, 3)}; \
+        __syncthreads();                                                                     \
+        wave_fill(gdraws, segs, s0, lane);                                                   \
+        base = at - g * kSegLen;                                                             \
+        p = 0;                                                                               \
+        hr_ready = false;                                                                    \
+        need_block = true;                                                                   \
+    } while (0)
+
+    // Decode of slot k: lane i takes the attempt starting at window
+    // position i — the first accepted draw of each consumer in turn, found
+    // by mask search — and the draw right after it.
+#define CPM_DECODE(k)                                                                        \
+    do {                                                                                     \
         uint32_t pos = lane, t = 0, kv = 0;                                                  \
         bool valid = true;                                                                   \
         _Pragma("unroll") for (int c = 0; c <= D; ++c) {                                     \
-            const uint32_t q = lane_next_bit(m0[c], m1[c] & hv, pos);                        \
+            const uint32_t q = lane_next_bit(m0[c], m1[c], pos);                             \
             valid = valid && q < 128;                                                        \
             const uint32_t v0 = bperm(h0.hi[c], q & 63), v1 = bperm(h1.hi[c], q & 63);       \
             const uint32_t v = q < 64 ? v0 : v1;                                             \
-            if (c < D) {                                                                     \
-                dec_c[c] = v;                                                                \
+            if (c < D)                                                                       \
                 t += (v + 1) * stride[c];                                                    \
-            } else {                                                                         \
+            else                                                                             \
                 kv = v;                                                                      \
-            }                                                                                \
             pos = q + 1;                                                                     \
         }                                                                                    \
         const uint32_t off = bperm(coff, kv);                                                \
-        dec_t = valid ? t : t_safe;                                                          \
-        dec_s = valid ? (uint32_t)((int32_t)t + (int32_t)off) : t_safe;                      \
-        dec_end = pos;                                                                       \
-        dec_ok = valid;                                                                      \
+        st_t[k] = valid ? t : t_safe;                                                        \
+        st_s[k] = valid ? (uint32_t)((int32_t)t + (int32_t)off) : t_safe;                    \
+        st_m[k] = min(pos, (uint32_t)kMe) | (valid ? kMv : 0u);                               \
+        const uint32_t ue = min(pos, 127u), ul = ue & 63;                                    \
+        const uint32_t lo0 = bperm((uint32_t)h0.w, ul), hi0 = bperm((uint32_t)(h0.w >> 32), ul); \
+        const uint32_t lo1 = bperm((uint32_t)h1.w, ul), hi1 = bperm((uint32_t)(h1.w >> 32), ul); \
+        st_u[k] = ue < 64 ? ((uint64_t)hi0 << 32 | lo0) : ((uint64_t)hi1 << 32 | lo1);      \
     } while (0)
 
-#define CPM_EVALUATE()                                                                       \
+    // Full evaluation of one attempt per lane on the current lattice
+    // (dynamics.rs:22-69, energy.rs:521-523): early-out, simple-point and
+    // volume checks, the local dH in FP64 in the reference's order, and the
+    // Metropolis test with u = gen::<f64>() of the draw after the attempt.
+#define CPM_EVAL(IT_, IS_, IU_, IV_)                                                         \
     do {                                                                                     \
-        const uint32_t t = dec_t, e = dec_end;                                               \
-        ev.Lw = sig.ld(t);                                                                   \
-        ev.Gw = sig.ld(dec_s);                                                               \
-        const uint32_t L = ev.Lw & kIdMask, G = ev.Gw & kIdMask;                             \
-        const uint32_t lidx = ev.Lw >> 12, gidx = ev.Gw >> 12;                               \
-        const bool nn = dec_ok && ev.Lw != ev.Gw;                                            \
+        const uint32_t et_ = (IT_), es_ = (IS_);                                                \
+        const uint64_t eu_ = (IU_);                                                           \
+        ev_Lw = sig.ld(et_);                                                                   \
+        ev_Gw = sig.ld(es_);                                                                  \
+        const uint32_t L = ev_Lw & kIdMask, G = ev_Gw & kIdMask;                             \
+        const uint32_t lidx = ev_Lw >> 12, gidx = ev_Gw >> 12;                               \
+        const bool nn = (IV_) && ev_Lw != ev_Gw;                                             \
         uint32_t nb[NT];                                                                     \
         _Pragma("unroll") for (int m = 0; m < NT; ++m) {                                     \
             const int code = m < NT / 2 ? m : m + 1;                                         \
             const int o0 = code % 3 - 1, o1 = (code / 3) % 3 - 1, o2 = code / 9 - 1;         \
             const int32_t a = D == 2 ? o0 * (int32_t)stride[0] + o1                          \
                                      : o0 * (int32_t)stride[0] + o1 * (int32_t)stride[1 % D] + o2; \
-            nb[m] = sig.ld((int32_t)t + a);                                                  \
+            nb[m] = sig.ld((int32_t)et_ + a);                                                  \
         }                                                                                    \
-        ev.vL = vol[L ? L - 1 : 0];                                                          \
-        ev.vG = vol[G ? G - 1 : 0];                                                          \
-        ev.iL = itf[L ? L - 1 : 0];                                                          \
-        ev.iG = itf[G ? G - 1 : 0];                                                          \
+        ev_vL = vol[L ? L - 1 : 0];                                                          \
+        ev_vG = vol[G ? G - 1 : 0];                                                          \
+        ev_iL = itf[L ? L - 1 : 0];                                                          \
+        ev_iG = itf[G ? G - 1 : 0];                                                          \
         uint32_t pat = 0;                                                                    \
         _Pragma("unroll") for (int m = 0; m < NT; ++m)                                       \
             pat |= (uint32_t)((nb[m] & kIdMask) == L) << m;                                  \
@@ -642,7 +673,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 
         } else {                                                                             \
             simple = top ? ((sp3d[pat >> 6] >> (pat & 63)) & 1ull) : false;                  \
         }                                                                                    \
-        const bool go = nn && (L == 0 || (simple && ev.vL > 1));                             \
+        const bool go = nn && (L == 0 || (simple && ev_vL > 1));                             \
         double dadh = 0.0; /* summed in stencil order */                                     \
         int cl = 0, cg = 0;                                                                  \
         _Pragma("unroll") for (int q = 0; q < NE; ++q) {                                     \
@@ -653,142 +684,67 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 
             cl += nbe == L;                                                                  \
             cg += nbe == G;                                                                  \
         }                                                                                    \
-        ev.dl = 2 * cl - NE;                                                                 \
-        ev.dg = NE - 2 * cg;                                                                 \
+        ev_dl = 2 * cl - NE;                                                                 \
+        ev_dg = NE - 2 * cg;                                                                 \
         const uint32_t tl = lidx ? lidx - 1 : 0u, tg = gidx ? gidx - 1 : 0u;                 \
         double dvol = 0.0, ditf = 0.0;                                                       \
         if (L) {                                                                             \
-            const double v = (double)ev.vL, T0 = (double)tv[tl];                             \
+            const double v = (double)ev_vL, T0 = (double)tv[tl];                             \
             const double x1 = (v - 1.0) - T0, x0 = v - T0;                                   \
             dvol += lv[tl] * (x1 * x1) - lv[tl] * (x0 * x0);                                 \
-            const double i0 = (double)ev.iL, I0 = (double)ti[tl];                            \
-            const double y1 = (i0 + (double)ev.dl) - I0, y0 = i0 - I0;                       \
+            const double i0 = (double)ev_iL, I0 = (double)ti[tl];                            \
+            const double y1 = (i0 + (double)ev_dl) - I0, y0 = i0 - I0;                       \
             ditf += li[tl] * (y1 * y1) - li[tl] * (y0 * y0);                                 \
         }                                                                                    \
         if (G) {                                                                             \
-            const double v = (double)ev.vG, T0 = (double)tv[tg];                             \
+            const double v = (double)ev_vG, T0 = (double)tv[tg];                             \
             const double x1 = (v + 1.0) - T0, x0 = v - T0;                                   \
             dvol += lv[tg] * (x1 * x1) - lv[tg] * (x0 * x0);                                 \
-            const double i0 = (double)ev.iG, I0 = (double)ti[tg];                            \
-            const double y1 = (i0 + (double)ev.dg) - I0, y0 = i0 - I0;                       \
+            const double i0 = (double)ev_iG, I0 = (double)ti[tg];                            \
+            const double y1 = (i0 + (double)ev_dg) - I0, y0 = i0 - I0;                       \
             ditf += li[tg] * (y1 * y1) - li[tg] * (y0 * y0);                                 \
         }                                                                                    \
-        ev.dh = dvol + ditf + dadh;                                                          \
-        const bool needu = go && !(ev.dh <= 0.0);                                            \
-        const bool euok = !needu || e < (h1_valid ? 128u : 64u);                             \
-        /* the u draw, rng.gen::<f64>(), at window position e */                             \
-        const uint32_t ue = min(e, 127u), ul = ue & 63;                                      \
-        const uint32_t lo0 = bperm((uint32_t)h0.w, ul), hi0 = bperm((uint32_t)(h0.w >> 32), ul); \
-        const uint32_t lo1 = bperm((uint32_t)h1.w, ul), hi1 = bperm((uint32_t)(h1.w >> 32), ul); \
-        const uint64_t uw = ue < 64 ? ((uint64_t)hi0 << 32 | lo0) : ((uint64_t)hi1 << 32 | lo1); \
+        ev_dh = dvol + ditf + dadh;                                                          \
+        const bool needu = go && !(ev_dh <= 0.0);                                            \
         bool acc = go, tie = false;                                                          \
         if (needu) {                                                                         \
-            const double u = (double)(uw >> 11) * (1.0 / 9007199254740992.0);                \
-            const double pr = exp(-ev.dh);                                                   \
+            const double u = (double)(eu_ >> 11) * (1.0 / 9007199254740992.0);                \
+            const double pr = exp(-ev_dh);                                                   \
             acc = u < pr;                                                                    \
             tie = fabs(u - pr) <= pr * 0x1p-50;                                              \
         }                                                                                    \
-        ev.next = e + (needu ? 1u : 0u);                                                     \
-        const bool ok = dec_ok && euok;                                                      \
-        M.ok = uni64(__ballot(ok));                                                          \
-        M.null_ = uni64(__ballot(dec_ok && !nn));                                            \
-        M.acc = uni64(__ballot(ok && acc));                                                  \
-        M.top = uni64(__ballot(top));                                                        \
-        M.en = uni64(__ballot(go));                                                          \
-        M.tie = uni64(__ballot(ok && tie));                                                  \
+        ev_f = (nn ? kMn : 0u) | (needu ? kMu : 0u) | (acc ? kMa : 0u) | (top ? kMp : 0u) |  \
+               (go ? kMg : 0u) | (tie ? kMt : 0u);                                           \
     } while (0)
 
-    // The buffer is restarted at the segment holding absolute position `at`
-    // (its draws regenerated from the segment's start state) whenever the
-    // window would run past its end.
-#define CPM_REFILL(at_)                                                                      \
-    do {                                                                                     \
-        const uint32_t at = (at_), g = min(at / kSegLen, kSegs - 1);                          \
-        /* a vector load: the segment states were written by vector stores */                \
-        const uint64_t mine = segs[g * 4 + (lane & 3)];                                       \
-        const uint64_t s0[4] = {rl64(mine, 0), rl64(mine, 1), rl64(mine, 2), rl64(mine, 3)}; \
-        __syncthreads();                                                                     \
-        wave_fill(gdraws, segs, s0, lane);                                                   \
-        base = at - g * kSegLen;                                                             \
-        p = 0;                                                                               \
-        h1_valid = true;                                                                     \
-        prep_half<D>(h0, draws[base + lane], range, zone, m0);                               \
-        prep_half<D>(h1, draws[base + 64 + lane], range, zone, m1);                          \
-        w_next = draws[base + 128 + lane];                                                   \
-        need_decode = true;                                                                  \
-    } while (0)
-
-    h1_valid = base + 128 <= kFill;
-    prep_half<D>(h0, draws[base + lane], range, zone, m0);
-    prep_half<D>(h1, h1_valid ? draws[base + 64 + lane] : 0ull, range, zone, m1);
-    w_next = base + 192 <= kFill ? draws[base + 128 + lane] : 0ull;
-    while (todo > 0) {
-        if (need_decode) {
-            CPM_DECODE();
-            need_decode = false;
-            need_eval = true;
-        }
-        if (need_eval) {
-            CPM_EVALUATE();
-            need_eval = false;
-        }
-        // Walk the chain through the evaluated attempts up to the first
-        // accepted copy, the window's end or the MCS target, in one step:
-        // lane k finds the k-th attempt position x = next^k(p) by pointer
-        // doubling.  An attempt takes at least D + 1 >= 3 draws, so at most
-        // 22 attempts start in a 64-draw window and 5 doublings suffice.
-        uint32_t jb = ((M.ok >> lane) & 1) ? ev.next : 127u;  // next^(2^b), >= 64 = leaves
-        uint32_t x = p;
-#pragma unroll
-        for (int b = 0; b < 5; ++b) {
-            const uint32_t y = bperm(jb, x & 63);
-            if (((lane >> b) & 1) && x < 64) x = y;
-            if (b < 4) {
-                const uint32_t z = bperm(jb, jb & 63);
-                if (jb < 64) jb = z;
-            }
-        }
-        const uint32_t xs = x & 63;
-        const bool vk = lane < 32 && x < 64 && ((M.ok >> xs) & 1);
-        const uint32_t k_inv = (uint32_t)__builtin_ctzll(uni64(__ballot(!vk)));
-        const uint64_t acc_k = uni64(__ballot(vk && ((M.acc >> xs) & 1)));
-        const uint32_t k_acc = acc_k ? (uint32_t)__builtin_ctzll(acc_k) : 64u;
-        // attempts left before the MCS target: (todo - 1) * n_sites + (n_sites - ain)
-        const uint64_t left = (todo - 1) * (uint64_t)n_sites + (n_sites - ain);
-        uint32_t n_att = min(k_inv, left < 64 ? (uint32_t)left : 64u);
-        const bool hit = k_acc < n_att;
-        if (hit) n_att = k_acc + 1;
-        {
-            const bool w = lane < n_att;
-            c_att += n_att;
-            c_nn += __popcll(uni64(__ballot(w && !((M.null_ >> xs) & 1))));
-            c_top += __popcll(uni64(__ballot(w && ((M.top >> xs) & 1))));
-            c_en += __popcll(uni64(__ballot(w && ((M.en >> xs) & 1))));
-            c_acc += hit ? 1u : 0u;
-            c_tie += __popcll(uni64(__ballot(w && ((M.tie >> xs) & 1))));
-        }
-        // monte_carlo.rs:330-356 — MCS accounting
-        ain += n_att;
-        while (ain >= n_sites) {
-            ain -= n_sites;
-            ++mcs;
-            --todo;
-        }
-        const uint32_t np = rl32(x, n_att);  // the position after the walked attempts
-        if (hit) p = rl32(x, k_acc);
-        if (hit) {
+    for (;;) {
+        // ---- an accepted attempt (slot 0, position commit_p): evaluated
+        // again on every lane, then committed (dynamics.rs:73-224)
+        if (pending) {
+            pending = false;
             dirty = true;
-            const uint32_t t = rl32(dec_t, p);
-            const uint32_t Lw = rl32(ev.Lw, p), Gw = rl32(ev.Gw, p);
+            const uint32_t t = rl32(st_t[0], commit_p);
+            uint32_t ev_Lw, ev_Gw, ev_vL, ev_vG, ev_iL, ev_iG, ev_f;
+            int32_t ev_dl, ev_dg;
+            double ev_dh;
+            CPM_EVAL(t, rl32(st_s[0], commit_p), rl64(st_u[0], commit_p), true);
+            (void)ev_f;
+            const uint32_t Lw = uni(ev_Lw), Gw = uni(ev_Gw);
             const uint32_t L = Lw & kIdMask, G = Gw & kIdMask;
-            const int32_t dl_i = (int32_t)rl32((uint32_t)ev.dl, p);
-            const int32_t dg_i = (int32_t)rl32((uint32_t)ev.dg, p);
-            const uint32_t vL = rl32(ev.vL, p), vG = rl32(ev.vG, p);
-            const uint32_t iL = rl32(ev.iL, p), iG = rl32(ev.iG, p);
-            const double dh = rlf64(ev.dh, p);
+            const int32_t dl_i = (int32_t)uni((uint32_t)ev_dl);
+            const int32_t dg_i = (int32_t)uni((uint32_t)ev_dg);
+            const uint32_t vL = uni(ev_vL), vG = uni(ev_vG);
+            const uint32_t iL = uni(ev_iL), iG = uni(ev_iG);
+            const double dh = rlf64(ev_dh, 0);
             uint32_t cval[D];
+            {
+                uint32_t f = t;
 #pragma unroll
-            for (int d = 0; d < D; ++d) cval[d] = rl32(dec_c[d], p);
+                for (int d = 0; d < D; ++d) {
+                    cval[d] = f / stride[d] - 1;
+                    f %= stride[d];
+                }
+            }
             // dynamics.rs:73-98 — contact counts, pre-move lattice
             if (lane < NE) {
                 const uint32_t nbe = sig.ld((int32_t)t + eoff) & kIdMask;
@@ -867,52 +823,188 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 
             if constexpr (SG) __threadfence();  // the next evaluation reads sigma from HBM
             __syncthreads();
             energy += dh;
-            p = np;
             // the lattice changed: every later attempt is evaluated again
             need_eval = true;
-            continue;
         }
-        p = np;
         if (todo == 0) break;
-        if (p >= 64) {  // slide the window by one half
-            if (!h1_valid) {
+
+        // ---- decode a block of windows (the lattice plays no part)
+        if (need_block) {
+            const uint32_t room = kFill - base;
+            const uint32_t nk = room >= 128 ? min((uint32_t)KB, (room - 64) / 64) : 0u;
+            if (nk == 0) {
                 CPM_REFILL(base + p);
                 continue;
             }
+            if (!hr_ready) {
+#pragma unroll
+                for (int j = 0; j <= KB; ++j)
+                    hr[j] = base + 64 * j + 64 <= kFill ? draws[base + 64 * j + lane] : 0ull;
+            }
+            Half h0, h1;
+            uint64_t m0[D + 1], m1[D + 1];
+            prep_half<D>(h0, hr[0], range, zone, m0);
+#pragma unroll
+            for (int k = 0; k < KB; ++k) {
+                if (k < (int)nk) {
+                    prep_half<D>(h1, hr[k + 1], range, zone, m1);
+                    CPM_DECODE(k);
+                    h0 = h1;
+#pragma unroll
+                    for (int c = 0; c <= D; ++c) m0[c] = m1[c];
+                }
+            }
+            nvalid = nk;
+            // the next block's draws, in flight while this block is walked
+            hr_ready = nk == (uint32_t)KB;
+            if (hr_ready) {
+                hr[0] = hr[KB];
+#pragma unroll
+                for (int j = 1; j <= KB; ++j) {
+                    const uint32_t at = base + 64 * (KB + j);
+                    hr[j] = at + 64 <= kFill ? draws[at + lane] : 0ull;
+                }
+            }
+            need_block = false;
+            need_eval = true;
+        }
+
+        // ---- evaluate the block's non-null attempts, 64 per pass, packed
+        // onto consecutive lanes through the LDS stage
+        if (need_eval) {
+            need_eval = false;
+            uint64_t nnm[KB];
+            uint32_t pre[KB + 1];
+            pre[0] = 0;
+#pragma unroll
+            for (int k = 0; k < KB; ++k) {
+                bool nn = false;
+                if (k < (int)nvalid) {
+                    const uint32_t Lw = sig.ld(st_t[k]), Gw = sig.ld(st_s[k]);
+                    nn = (st_m[k] & kMv) && Lw != Gw && (k > 0 || lane >= p);
+                }
+                st_m[k] = (st_m[k] & (kMe | kMv)) | (nn ? kMn : 0u);
+                nnm[k] = uni64(__ballot(nn));
+                pre[k + 1] = pre[k] + (uint32_t)__popcll(nnm[k]);
+            }
+            const uint32_t total = pre[KB];
+            for (uint32_t q0 = 0; q0 < total; q0 += 64) {
+                uint32_t cidx[KB];
+#pragma unroll
+                for (int k = 0; k < KB; ++k) {
+                    // index within this pass of the lane's attempt (>= 64: not in it)
+                    const uint32_t r = __builtin_amdgcn_mbcnt_hi(
+                        (uint32_t)(nnm[k] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)nnm[k], 0u));
+                    const uint32_t c = pre[k] + r - q0;
+                    cidx[k] = ((nnm[k] >> lane) & 1) && pre[k] + r >= q0 ? c : 64u;
+                    if (pre[k + 1] > q0 && pre[k] < q0 + 64 && cidx[k] < 64)
+                        stage[cidx[k]] = make_uint4(st_t[k], st_s[k], (uint32_t)st_u[k],
+                                                    (uint32_t)(st_u[k] >> 32));
+                }
+                __syncthreads();
+                const uint32_t n_here = min(total - q0, 64u);
+                uint4 in = make_uint4(t_safe, t_safe, 0u, 0u);
+                if (lane < n_here) in = stage[lane];
+                __syncthreads();  // the stage is rewritten by the next pass
+                uint32_t ev_Lw, ev_Gw, ev_vL, ev_vG, ev_iL, ev_iG, ev_f;
+                int32_t ev_dl, ev_dg;
+                double ev_dh;
+                CPM_EVAL(in.x, in.y, (uint64_t)in.w << 32 | in.z, lane < n_here);
+#pragma unroll
+                for (int k = 0; k < KB; ++k) {
+                    if (pre[k + 1] > q0 && pre[k] < q0 + 64) {
+                        const uint32_t f = bperm(ev_f, cidx[k] & 63);
+                        if (cidx[k] < 64) st_m[k] |= f;
+                    }
+                }
+            }
+        }
+
+        // ---- walk the chain through slot 0 up to the first accepted copy,
+        // the window's end or the MCS target, in one step: lane k finds the
+        // k-th attempt position x = next^k(p) by pointer doubling.  An
+        // attempt takes at least D + 1 >= 3 draws, so at most 22 attempts
+        // start in a 64-draw window and 5 doublings suffice.
+        const uint32_t m_own = st_m[0];
+        {
+            const uint32_t e = m_own & kMe, u1 = (m_own & kMu) ? 1u : 0u;
+            const bool okb = (m_own & kMv) && (!u1 || e < 128);
+            uint32_t jb = okb ? e + u1 : 127u;  // next^(2^b); >= 64 = leaves
+            uint32_t x = p;
+#pragma unroll
+            for (int b = 0; b < 5; ++b) {
+                const uint32_t y = bperm(jb, x & 63);
+                if (((lane >> b) & 1) && x < 64) x = y;
+                if (b < 4) {
+                    const uint32_t z = bperm(jb, jb & 63);
+                    if (jb < 64) jb = z;
+                }
+            }
+            const uint32_t fx = bperm(m_own, x & 63);  // meta of the lane's attempt
+            const bool okx = (fx & kMv) && (!(fx & kMu) || (fx & kMe) < 128);
+            const bool vk = lane < 32 && x < 64 && okx;
+            const uint32_t k_inv = (uint32_t)__builtin_ctzll(uni64(__ballot(!vk)));
+            const uint64_t acc_k = uni64(__ballot(vk && (fx & kMa)));
+            const uint32_t k_acc = acc_k ? (uint32_t)__builtin_ctzll(acc_k) : 64u;
+            // attempts left before the MCS target: (todo - 1) * n_sites + (n_sites - ain)
+            const uint64_t left = (todo - 1) * (uint64_t)n_sites + (n_sites - ain);
+            uint32_t n_att = min(k_inv, left < 64 ? (uint32_t)left : 64u);
+            const bool hit = k_acc < n_att;
+            if (hit) n_att = k_acc + 1;
+            {
+                const bool w = lane < n_att;
+                c_att += n_att;
+                c_nn += __popcll(uni64(__ballot(w && (fx & kMn))));
+                c_top += __popcll(uni64(__ballot(w && (fx & kMp))));
+                c_en += __popcll(uni64(__ballot(w && (fx & kMg))));
+                c_acc += hit ? 1u : 0u;
+                c_tie += __popcll(uni64(__ballot(w && (fx & kMt))));
+            }
+            // monte_carlo.rs:330-356 — MCS accounting
+            ain += n_att;
+            while (ain >= n_sites) {
+                ain -= n_sites;
+                ++mcs;
+                --todo;
+            }
+            const uint32_t np = rl32(x, n_att);  // the position after the walked attempts
+            if (hit) {
+                commit_p = rl32(x, k_acc);
+                pending = true;
+                p = np;
+                continue;
+            }
+            p = np;
+        }
+        if (todo == 0) break;
+        if (p >= 64) {  // slot 0 is walked: the next window becomes slot 0
+#pragma unroll
+            for (int k = 0; k + 1 < KB; ++k) {
+                st_t[k] = st_t[k + 1];
+                st_s[k] = st_s[k + 1];
+                st_m[k] = st_m[k + 1];
+                st_u[k] = st_u[k + 1];
+            }
             base += 64;
             p -= 64;
-            h0 = h1;
-#pragma unroll
-            for (int c = 0; c <= D; ++c) m0[c] = m1[c];
-            h1_valid = base + 128 <= kFill;
-            prep_half<D>(h1, h1_valid ? w_next : 0ull, range, zone, m1);
-            w_next = base + 192 <= kFill ? draws[base + 128 + lane] : 0ull;
-            need_decode = true;
+            if (--nvalid == 0) need_block = true;
             continue;
         }
-        // The attempt at p does not fit in the window: restart the window
-        // at p (refilling the buffer if the window would run past its end).
-        // An attempt needing more than 128 draws even from a window of its
-        // own is reported, never looped on.
-        if (p == 0 && h1_valid) {
+        // The attempt at p does not fit in its window: restart the block at
+        // it.  An attempt needing more than 128 draws even from a window of
+        // its own is reported, never looped on.
+        if (p == 0) {
             error = true;
             break;
         }
-        if (base + p + 192 > kFill) {
-            CPM_REFILL(base + p);
-            continue;
-        }
         base += p;
         p = 0;
-        h1_valid = true;
-        prep_half<D>(h0, draws[base + lane], range, zone, m0);
-        prep_half<D>(h1, draws[base + 64 + lane], range, zone, m1);
-        w_next = base + 192 <= kFill ? draws[base + 128 + lane] : 0ull;
-        need_decode = true;
+        hr_ready = false;
+        need_block = true;
     }
     base += p;  // buffer position of the next unconsumed draw
 #undef CPM_DECODE
-#undef CPM_EVALUATE
+#undef CPM_EVAL
 #undef CPM_REFILL
     __syncthreads();
     // the consumer's RNG state: the segment's start state (a vector load:
@@ -1847,11 +1939,11 @@ static bool lds_eligible(const Engine* e, const Model& m, uint32_t* bytes, bool*
     const uint32_t T = (uint32_t)m.cfg.types.size();
     const LdsLayout in = lds_layout((uint32_t)m.n_padded, (uint32_t)m.n_cells, T, true);
     const LdsLayout out = lds_layout((uint32_t)m.n_padded, (uint32_t)m.n_cells, T, false);
-    const bool fits = in.total + kLdsStaticReserve <= e->max_lds;
+    const bool fits = in.total + kStageBytes + kLdsStaticReserve <= e->max_lds;
     if (gsig) *gsig = !fits;
-    *bytes = fits ? in.total : out.total;
+    *bytes = (fits ? in.total : out.total) + kStageBytes;
     if (!gsig) return fits;  // callers that need sigma in LDS
-    return out.total + kLdsStaticReserve <= e->max_lds;
+    return out.total + kStageBytes + kLdsStaticReserve <= e->max_lds;
 }
 
 // LDS bytes a launch reserves for replicas needing `bytes` (grouping key).
@@ -1878,7 +1970,8 @@ static void upload(Engine* e, Model* m) {
     d->lds_bytes = lds_bytes;
     d->lds_bytes_hbm = lds_layout((uint32_t)m->n_padded, (uint32_t)m->n_cells,
                                   (uint32_t)m->cfg.types.size(), false)
-                           .total;
+                           .total +
+                       kStageBytes;
     if (lds) {
         // the LDS kernel addresses the full shell from the strides and maps
         // the energy stencil into it at compile time
