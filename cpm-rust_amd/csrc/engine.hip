@@ -579,13 +579,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 
     uint4* const stage = (uint4*)(smem + Ly.total);  // one evaluation pass's attempts
     uint32_t base = 0, p = 0;
     uint64_t hr[KB + 1];
-    uint32_t st_t[KB], st_s[KB], st_m[KB];  // per slot and lane: target, source, meta
-    uint64_t st_u[KB];                      // the draw after the attempt (its u when dH > 0)
+    uint32_t st_t[KB], st_m[KB];  // per slot and lane: target, meta (copy index in bits 16-20)
+    uint64_t st_u[KB];            // the draw after the attempt (its u when dH > 0)
 #pragma unroll
     for (int j = 0; j <= KB; ++j) hr[j] = 0;
 #pragma unroll
     for (int k = 0; k < KB; ++k) {
-        st_t[k] = st_s[k] = t_safe;
+        st_t[k] = t_safe;
         st_m[k] = 0;
         st_u[k] = 0;
     }
@@ -626,10 +626,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 
                 kv = v;                                                                      \
             pos = q + 1;                                                                     \
         }                                                                                    \
-        const uint32_t off = bperm(coff, kv);                                                \
         st_t[k] = valid ? t : t_safe;                                                        \
-        st_s[k] = valid ? (uint32_t)((int32_t)t + (int32_t)off) : t_safe;                    \
-        st_m[k] = min(pos, (uint32_t)kMe) | (valid ? kMv : 0u);                               \
+        st_m[k] = min(pos, (uint32_t)kMe) | (valid ? kMv | kv << 16 : 0u);                   \
         const uint32_t ue = min(pos, 127u), ul = ue & 63;                                    \
         const uint32_t lo0 = bperm((uint32_t)h0.w, ul), hi0 = bperm((uint32_t)(h0.w >> 32), ul); \
         const uint32_t lo1 = bperm((uint32_t)h1.w, ul), hi1 = bperm((uint32_t)(h1.w >> 32), ul); \
@@ -727,7 +725,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 
             uint32_t ev_Lw, ev_Gw, ev_vL, ev_vG, ev_iL, ev_iG, ev_f;
             int32_t ev_dl, ev_dg;
             double ev_dh;
-            CPM_EVAL(t, rl32(st_s[0], commit_p), rl64(st_u[0], commit_p), true);
+            const uint32_t s = t + rl32(coff, (rl32(st_m[0], commit_p) >> 16) & 31);
+            CPM_EVAL(t, s, rl64(st_u[0], commit_p), true);
             (void)ev_f;
             const uint32_t Lw = uni(ev_Lw), Gw = uni(ev_Gw);
             const uint32_t L = Lw & kIdMask, G = Gw & kIdMask;
@@ -874,32 +873,39 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 
         if (need_eval) {
             need_eval = false;
             uint64_t nnm[KB];
-            uint32_t pre[KB + 1];
+            uint32_t pre[KB + 1], src[KB];
             pre[0] = 0;
 #pragma unroll
             for (int k = 0; k < KB; ++k) {
                 bool nn = false;
+                // source site: target + offset of the copy index (a void
+                // decode keeps its interior stand-in as both)
+                src[k] = st_t[k] + ((st_m[k] & kMv) ? bperm(coff, (st_m[k] >> 16) & 31) : 0u);
                 if (k < (int)nvalid) {
-                    const uint32_t Lw = sig.ld(st_t[k]), Gw = sig.ld(st_s[k]);
+                    const uint32_t Lw = sig.ld(st_t[k]), Gw = sig.ld(src[k]);
                     nn = (st_m[k] & kMv) && Lw != Gw && (k > 0 || lane >= p);
                 }
-                st_m[k] = (st_m[k] & (kMe | kMv)) | (nn ? kMn : 0u);
+                st_m[k] = (st_m[k] & (kMe | kMv | 0x1F0000u)) | (nn ? kMn : 0u);
                 nnm[k] = uni64(__ballot(nn));
                 pre[k + 1] = pre[k] + (uint32_t)__popcll(nnm[k]);
             }
             const uint32_t total = pre[KB];
+            // index within the pass starting at q0 of the lane's attempt in
+            // slot k (>= 64: not in it)
+#define CPM_CIDX(k, q0)                                                                      \
+    (((nnm[k] >> lane) & 1) &&                                                               \
+             pre[k] + __builtin_amdgcn_mbcnt_hi((uint32_t)(nnm[k] >> 32),                    \
+                                                __builtin_amdgcn_mbcnt_lo((uint32_t)nnm[k], 0u)) >= (q0) \
+         ? pre[k] + __builtin_amdgcn_mbcnt_hi((uint32_t)(nnm[k] >> 32),                      \
+                                              __builtin_amdgcn_mbcnt_lo((uint32_t)nnm[k], 0u)) - (q0) \
+         : 64u)
             for (uint32_t q0 = 0; q0 < total; q0 += 64) {
-                uint32_t cidx[KB];
 #pragma unroll
                 for (int k = 0; k < KB; ++k) {
-                    // index within this pass of the lane's attempt (>= 64: not in it)
-                    const uint32_t r = __builtin_amdgcn_mbcnt_hi(
-                        (uint32_t)(nnm[k] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)nnm[k], 0u));
-                    const uint32_t c = pre[k] + r - q0;
-                    cidx[k] = ((nnm[k] >> lane) & 1) && pre[k] + r >= q0 ? c : 64u;
-                    if (pre[k + 1] > q0 && pre[k] < q0 + 64 && cidx[k] < 64)
-                        stage[cidx[k]] = make_uint4(st_t[k], st_s[k], (uint32_t)st_u[k],
-                                                    (uint32_t)(st_u[k] >> 32));
+                    const uint32_t c = CPM_CIDX(k, q0);
+                    if (pre[k + 1] > q0 && pre[k] < q0 + 64 && c < 64)
+                        stage[c] = make_uint4(st_t[k], src[k], (uint32_t)st_u[k],
+                                              (uint32_t)(st_u[k] >> 32));
                 }
                 __syncthreads();
                 const uint32_t n_here = min(total - q0, 64u);
@@ -913,11 +919,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 
 #pragma unroll
                 for (int k = 0; k < KB; ++k) {
                     if (pre[k + 1] > q0 && pre[k] < q0 + 64) {
-                        const uint32_t f = bperm(ev_f, cidx[k] & 63);
-                        if (cidx[k] < 64) st_m[k] |= f;
+                        const uint32_t c = CPM_CIDX(k, q0);
+                        const uint32_t f = bperm(ev_f, c & 63);
+                        if (c < 64) st_m[k] |= f;
                     }
                 }
             }
+#undef CPM_CIDX
         }
 
         // ---- walk the chain through slot 0 up to the first accepted copy,
@@ -981,7 +989,6 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 
 #pragma unroll
             for (int k = 0; k + 1 < KB; ++k) {
                 st_t[k] = st_t[k + 1];
-                st_s[k] = st_s[k + 1];
                 st_m[k] = st_m[k + 1];
                 st_u[k] = st_u[k + 1];
             }
@@ -2270,13 +2277,20 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
                 groups.push_back({cut[g][k], k});
                 k = cut[g][k];
             }
-            // largest footprints first: the biggest lattices carry the
-            // longest serial chains, so their workgroups are dispatched first
+            // largest footprints first, and within a launch the longest
+            // chains first: the dispatcher spreads consecutive workgroups
+            // over the CUs and SIMDs, so the replicas that set the launch's
+            // length do not share a SIMD while others are left idle
             for (auto [a, b] : groups) {
                 const size_t first = start_k[a], last = start_k[b - 1] + count_k[b - 1];
                 launches.push_back({key, size_k[b - 1], (uint32_t)order.size(),
                                     (uint32_t)(last - first)});
+                const size_t at = order.size();
                 for (size_t j = first; j < last; ++j) order.push_back(cls[j].second);
+                std::stable_sort(order.begin() + at, order.end(), [&](uint32_t x, uint32_t y) {
+                    return (double)todo[x] * (double)models[x]->n_sites >
+                           (double)todo[y] * (double)models[y]->n_sites;
+                });
             }
         }
     }
@@ -2524,9 +2538,9 @@ static uint64_t batch_signature(Engine* e, Model* const* models, size_t n, const
 
 // Exact replay with an autotuned sigma placement.  Results do not depend on
 // the placement or on how the MCS are split into launches, so the first
-// large enough call of a new batch signature is split: a slice of its MCS
-// under each fill candidate, timed, then the rest under the fastest, which
-// later calls with that signature reuse.
+// large enough call of a new batch signature is split: two slices of its
+// MCS under each fill candidate, timed, then the rest under the fastest,
+// which later calls with that signature reuse.
 void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* todo, int mode,
                   cpm_counters* counters) {
     if (mode == CPM_MODE_SUBLATTICE) {
@@ -2541,7 +2555,9 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
     auto it = e->tuned_fill.find(sig);
     uint64_t max_todo = 0;
     for (size_t i = 0; i < n; ++i) max_todo = std::max(max_todo, todo[i]);
-    const uint64_t slice = max_todo / 32;  // per candidate: ~3 % of the call
+    // two slices per candidate, the second round in reverse order so that
+    // the lattices' drift during the call (early MCS accept more) cancels
+    const uint64_t slice = max_todo / (4 * kNumFillCandidates);  // 25 % of the call in all
     if (it != e->tuned_fill.end() || slice == 0 || n < 2) {
         run_exact(e, models, n, todo,
                   it != e->tuned_fill.end() ? kFillCandidates[it->second] : kLdsPlacementFill,
@@ -2549,20 +2565,17 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
         return;
     }
     std::vector<uint64_t> left(todo, todo + n), part(n);
-    int best = -1;
-    double best_rate = 0.0;
-    for (int c = 0; c < kNumFillCandidates; ++c) {
+    double att[kNumFillCandidates] = {}, ms[kNumFillCandidates] = {};
+    for (int r = 0; r < 2 * kNumFillCandidates; ++r) {
+        const int c = r < kNumFillCandidates ? r : 2 * kNumFillCandidates - 1 - r;
         for (size_t i = 0; i < n; ++i) {
             part[i] = std::min(left[i], slice);
             left[i] -= part[i];
         }
         cpm_counters cc{};
         run_exact(e, models, n, part.data(), kFillCandidates[c], &cc);
-        const double rate = cc.kernel_ms > 0 ? (double)cc.attempts / cc.kernel_ms : 0.0;
-        if (best < 0 || rate > best_rate) {
-            best = c;
-            best_rate = rate;
-        }
+        att[c] += (double)cc.attempts;
+        ms[c] += cc.kernel_ms;
         if (counters) {
             counters->attempts += cc.attempts;
             counters->non_null += cc.non_null;
@@ -2575,6 +2588,9 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
             counters->replay_launch_ms += cc.replay_launch_ms;
         }
     }
+    int best = 0;
+    for (int c = 1; c < kNumFillCandidates; ++c)
+        if (att[c] * ms[best] > att[best] * ms[c]) best = c;
     e->tuned_fill[sig] = best;
     bool more = false;
     for (size_t i = 0; i < n; ++i) more |= left[i] != 0;
