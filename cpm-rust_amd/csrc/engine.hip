@@ -291,6 +291,14 @@ __global__ void __launch_bounds__(64) k_exact_replay(Rep* __restrict__ reps,
 // x^(g * kSegLen) mod p(x): the jump from a fill's start to segment g
 __constant__ uint64_t c_jump[kSegs][4];
 
+// Orders one wave's LDS and global accesses around a point where its lanes
+// exchange data (a workgroup barrier minus the rendezvous, so that it is
+// safe inside code only one wave of a workgroup runs).
+__device__ __forceinline__ void wsync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // The wave writes kFill consecutive draws of the stream whose state at the
 // buffer start is s0: lane g jumps to draw g * kSegLen (sum of T^i s0 over
 // the polynomial's terms) and steps through its segment.  Segment start
@@ -311,7 +319,7 @@ __device__ void wave_fill(uint64_t* __restrict__ draws, uint64_t* __restrict__ s
     uint64_t* __restrict__ out = draws + (size_t)lane * kSegLen;
     for (uint32_t i = 0; i < kSegLen; ++i) out[i] = xo_next(s);
     __threadfence();  // the wave reads other lanes' segments next
-    __syncthreads();
+    wsync();
 }
 
 __device__ __forceinline__ uint32_t rl32(uint32_t v, uint32_t lane) {
@@ -454,6 +462,11 @@ constexpr uint32_t kMt = 1u << 14;  // u within 4 ulp of exp(-dH)
 // LDS the wave path adds after the replica's tables: one evaluation pass's
 // 64 staged attempts (target, source, u draw).
 constexpr uint32_t kStageBytes = 64 * 16;
+// LDS the pipelined kernel adds to the single-wave one: the second wave's
+// stage and two block buffers of KB slots (16 bytes per lane).
+__host__ __device__ constexpr uint32_t pipe_extra_bytes(int D) {
+    return kStageBytes + 2u * (D == 2 ? 3u : 2u) * 64u * 16u;
+}
 
 // Packed sigma words, in LDS or (G) in HBM.  The HBM copy is read through
 // global-address-space loads (flat loads would also count against the LDS
@@ -482,15 +495,17 @@ struct SigmaRef<true> {
 // accepted copy, evaluates that attempt again on every lane, commits it,
 // re-evaluates the rest of the block and goes on.  NE is the energy stencil
 // size; NT = 3^D - 1 the full shell.
-template <int D, int NE, bool SG>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 ? 4 : 1))) k_lds_engine(Rep* __restrict__ reps,
+template <int D, int NE, bool SG, bool PW>
+__global__ void __launch_bounds__(PW ? 128 : 64) __attribute__((amdgpu_waves_per_eu(D == 2 ? (PW ? 3 : 4) : 1))) k_lds_engine(Rep* __restrict__ reps,
                                                    const uint32_t* __restrict__ idx,
                                                    const uint64_t* __restrict__ sp3d) {
     constexpr int NT = D == 2 ? 8 : 26;
+    constexpr uint32_t NTH = PW ? 128 : 64;
     extern __shared__ __align__(16) unsigned char smem[];
     Rep& R = reps[idx[blockIdx.x]];
     if (R.done) return;
-    const uint32_t lane = threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = PW ? threadIdx.x >> 6 : 0u;  // pipelined: 0 walks, 1 decodes/evaluates
     const uint32_t T = R.n_types, T1 = T + 1, n_cells = R.n_cells;
     const uint32_t n_padded = R.n_padded, n_sites = R.n_sites;
     const LdsLayout Ly = lds_layout(n_padded, n_cells, T, !SG);
@@ -509,17 +524,17 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 
 
     // ---- stage the replica into LDS
     const uint32_t* __restrict__ gtyp = R.type_index;
-    for (uint32_t i = lane; i < n_padded; i += 64) {
+    for (uint32_t i = threadIdx.x; i < n_padded; i += NTH) {
         const uint32_t id = R.sigma[i];
         sig.st(i, id ? id | ((gtyp[id - 1] + 1) << 12) : 0u);
     }
     if constexpr (SG) __threadfence();
-    for (uint32_t i = lane; i < n_cells; i += 64) {
+    for (uint32_t i = threadIdx.x; i < n_cells; i += NTH) {
         vol[i] = R.volume[i];
         itf[i] = R.interface[i];
     }
-    for (uint32_t i = lane; i < T1 * T1; i += 64) J[i] = R.params[2 * T + i];
-    for (uint32_t i = lane; i < T; i += 64) {
+    for (uint32_t i = threadIdx.x; i < T1 * T1; i += NTH) J[i] = R.params[2 * T + i];
+    for (uint32_t i = threadIdx.x; i < T; i += NTH) {
         lv[i] = R.params[i];
         li[i] = R.params[T + i];
         tv[i] = R.targets[i];
@@ -566,7 +581,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 
     // ---- the stream buffer, written by this wave from the call's RNG state
     uint64_t* const gdraws = R.draws;
     uint64_t* const segs = R.draws + kFill;  // start state of each segment
-    {
+    if (wave == (PW ? 1u : 0u)) {
         const uint64_t s0[4] = {R.rng[0], R.rng[1], R.rng[2], R.rng[3]};
         wave_fill(gdraws, segs, s0, lane);
     }
@@ -576,7 +591,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 
     // it; the decode of the window in slot k reads the 128 draws from its
     // start, halves hr[k] and hr[k + 1] of the block.
     constexpr int KB = D == 2 ? 3 : 2;
-    uint4* const stage = (uint4*)(smem + Ly.total);  // one evaluation pass's attempts
+    static_assert(pipe_extra_bytes(D) == kStageBytes + 2u * KB * 64u * 16u, "block buffers");
+    uint4* const stage = (uint4*)(smem + Ly.total + wave * kStageBytes);  // one pass's attempts
     uint32_t base = 0, p = 0;
     uint64_t hr[KB + 1];
     uint32_t st_t[KB], st_m[KB];  // per slot and lane: target, meta (copy index in bits 16-20)
@@ -591,6 +607,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 
     }
     uint32_t nvalid = 0, commit_p = 0;
     bool need_block = true, need_eval = false, hr_ready = false, pending = false;
+    bool walk_hit = false;
 
     // The buffer is restarted at the segment holding position `at` (its
     // draws regenerated from the segment's start state).
@@ -600,7 +617,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 
         /* a vector load: the segment states were written by vector stores */                \
         const uint64_t mine = segs[g * 4 + (lane & 3)];                                       \
         const uint64_t s0[4] = {rl64(mine, 0), rl64(mine, 1), rl64(mine, 2), rl64(mine, 3)}; \
-        __syncthreads();                                                                     \
+        wsync();                                                                             \
         wave_fill(gdraws, segs, s0, lane);                                                   \
         base = at - g * kSegLen;                                                             \
         p = 0;                                                                               \
@@ -715,183 +732,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 
                (go ? kMg : 0u) | (tie ? kMt : 0u);                                           \
     } while (0)
 
-    for (;;) {
-        // ---- an accepted attempt (slot 0, position commit_p): evaluated
-        // again on every lane, then committed (dynamics.rs:73-224)
-        if (pending) {
-            pending = false;
-            dirty = true;
-            const uint32_t t = rl32(st_t[0], commit_p);
-            uint32_t ev_Lw, ev_Gw, ev_vL, ev_vG, ev_iL, ev_iG, ev_f;
-            int32_t ev_dl, ev_dg;
-            double ev_dh;
-            const uint32_t s = t + rl32(coff, (rl32(st_m[0], commit_p) >> 16) & 31);
-            CPM_EVAL(t, s, rl64(st_u[0], commit_p), true);
-            (void)ev_f;
-            const uint32_t Lw = uni(ev_Lw), Gw = uni(ev_Gw);
-            const uint32_t L = Lw & kIdMask, G = Gw & kIdMask;
-            const int32_t dl_i = (int32_t)uni((uint32_t)ev_dl);
-            const int32_t dg_i = (int32_t)uni((uint32_t)ev_dg);
-            const uint32_t vL = uni(ev_vL), vG = uni(ev_vG);
-            const uint32_t iL = uni(ev_iL), iG = uni(ev_iG);
-            const double dh = rlf64(ev_dh, 0);
-            uint32_t cval[D];
-            {
-                uint32_t f = t;
-#pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    cval[d] = f / stride[d] - 1;
-                    f %= stride[d];
-                }
-            }
-            // dynamics.rs:73-98 — contact counts, pre-move lattice
-            if (lane < NE) {
-                const uint32_t nbe = sig.ld((int32_t)t + eoff) & kIdMask;
-                if (nbe != L)
-                    atomicSub(&R.contact[(uint64_t)min(L, nbe) * ncell1 + max(L, nbe)], 1u);
-                if (nbe != G)
-                    atomicAdd(&R.contact[(uint64_t)min(G, nbe) * ncell1 + max(G, nbe)], 1u);
-            }
-            __syncthreads();  // every lane has read t's neighbourhood
-            if (lane == 0) {
-                sig.st(t, Gw);
-                if (L) {
-                    vol[L - 1] = vL - 1;
-                    itf[L - 1] = (uint32_t)((int32_t)iL + dl_i);
-                }
-                if (G) {
-                    vol[G - 1] = vG + 1;
-                    itf[G - 1] = (uint32_t)((int32_t)iG + dg_i);
-                }
-                // dynamics.rs:125-183 — unwrapped position sums
-                if (L) {
-                    int64_t* sp = R.sum_pos + (uint64_t)(L - 1) * D;
-                    int64_t* so = R.sum_outer + (uint64_t)(L - 1) * D * D;
-                    for (int d = 0; d < D; ++d) sp[d] -= (int64_t)cval[d];
-                    for (int d = 0; d < D; ++d)
-                        for (int e2 = 0; e2 < D; ++e2)
-                            so[d * D + e2] -= (int64_t)cval[d] * (int64_t)cval[e2];
-                }
-                if (G) {
-                    int64_t* sp = R.sum_pos + (uint64_t)(G - 1) * D;
-                    int64_t* so = R.sum_outer + (uint64_t)(G - 1) * D * D;
-                    const uint32_t v = vG + 1;
-                    const uint32_t vbg = max(v > 0 ? v - 1 : 0u, 1u);
-                    double cen[D];
-                    for (int d = 0; d < D; ++d) cen[d] = (double)sp[d] / (double)vbg;
-                    int64_t img[D];
-                    for (int d = 0; d < D; ++d) {
-                        int64_t best = (int64_t)cval[d];
-                        if (periodic) {
-                            double bd = fabs((double)best - cen[d]);
-                            const int64_t lo = (int64_t)cval[d] - (int64_t)dims[d];
-                            const int64_t hi = (int64_t)cval[d] + (int64_t)dims[d];
-                            const double dlo = fabs((double)lo - cen[d]);
-                            if (dlo < bd) {
-                                best = lo;
-                                bd = dlo;
-                            }
-                            if (fabs((double)hi - cen[d]) < bd) best = hi;
-                        }
-                        img[d] = best;
-                        sp[d] += best;
-                    }
-                    for (int d = 0; d < D; ++d)
-                        for (int e2 = 0; e2 < D; ++e2) so[d * D + e2] += img[d] * img[e2];
-                }
-                // dynamics.rs:206-224 — rewrite t's halo mirrors
-                if (periodic) {
-                    uint32_t opt[D][3], nopt[D], total = 1;
-                    for (int d = 0; d < D; ++d) {
-                        nopt[d] = 1;
-                        opt[d][0] = cval[d] + 1;
-                        if (cval[d] == 0) opt[d][nopt[d]++] = dims[d] + 1;
-                        if (cval[d] == dims[d] - 1) opt[d][nopt[d]++] = 0;
-                        total *= nopt[d];
-                    }
-                    for (uint32_t q = 1; q < total; ++q) {
-                        uint32_t r = q, f = 0;
-                        for (int d = D - 1; d >= 0; --d) {
-                            f += opt[d][r % nopt[d]] * stride[d];
-                            r /= nopt[d];
-                        }
-                        sig.st(f, Gw);
-                    }
-                }
-            }
-            if constexpr (SG) __threadfence();  // the next evaluation reads sigma from HBM
-            __syncthreads();
-            energy += dh;
-            // the lattice changed: every later attempt is evaluated again
-            need_eval = true;
-        }
-        if (todo == 0) break;
-
-        // ---- decode a block of windows (the lattice plays no part)
-        if (need_block) {
-            const uint32_t room = kFill - base;
-            const uint32_t nk = room >= 128 ? min((uint32_t)KB, (room - 64) / 64) : 0u;
-            if (nk == 0) {
-                CPM_REFILL(base + p);
-                continue;
-            }
-            if (!hr_ready) {
-#pragma unroll
-                for (int j = 0; j <= KB; ++j)
-                    hr[j] = base + 64 * j + 64 <= kFill ? draws[base + 64 * j + lane] : 0ull;
-            }
-            Half h0, h1;
-            uint64_t m0[D + 1], m1[D + 1];
-            prep_half<D>(h0, hr[0], range, zone, m0);
-#pragma unroll
-            for (int k = 0; k < KB; ++k) {
-                if (k < (int)nk) {
-                    prep_half<D>(h1, hr[k + 1], range, zone, m1);
-                    CPM_DECODE(k);
-                    h0 = h1;
-#pragma unroll
-                    for (int c = 0; c <= D; ++c) m0[c] = m1[c];
-                }
-            }
-            nvalid = nk;
-            // the next block's draws, in flight while this block is walked
-            hr_ready = nk == (uint32_t)KB;
-            if (hr_ready) {
-                hr[0] = hr[KB];
-#pragma unroll
-                for (int j = 1; j <= KB; ++j) {
-                    const uint32_t at = base + 64 * (KB + j);
-                    hr[j] = at + 64 <= kFill ? draws[at + lane] : 0ull;
-                }
-            }
-            need_block = false;
-            need_eval = true;
-        }
-
-        // ---- evaluate the block's non-null attempts, 64 per pass, packed
-        // onto consecutive lanes through the LDS stage
-        if (need_eval) {
-            need_eval = false;
-            uint64_t nnm[KB];
-            uint32_t pre[KB + 1], src[KB];
-            pre[0] = 0;
-#pragma unroll
-            for (int k = 0; k < KB; ++k) {
-                bool nn = false;
-                // source site: target + offset of the copy index (a void
-                // decode keeps its interior stand-in as both)
-                src[k] = st_t[k] + ((st_m[k] & kMv) ? bperm(coff, (st_m[k] >> 16) & 31) : 0u);
-                if (k < (int)nvalid) {
-                    const uint32_t Lw = sig.ld(st_t[k]), Gw = sig.ld(src[k]);
-                    nn = (st_m[k] & kMv) && Lw != Gw && (k > 0 || lane >= p);
-                }
-                st_m[k] = (st_m[k] & (kMe | kMv | 0x1F0000u)) | (nn ? kMn : 0u);
-                nnm[k] = uni64(__ballot(nn));
-                pre[k + 1] = pre[k] + (uint32_t)__popcll(nnm[k]);
-            }
-            const uint32_t total = pre[KB];
-            // index within the pass starting at q0 of the lane's attempt in
-            // slot k (>= 64: not in it)
+    // ---- the steps of the loop, shared by the single-wave and the
+    // pipelined kernels.
+    // index within the pass starting at q0 of the lane's attempt in
+    // slot k (>= 64: not in it)
 #define CPM_CIDX(k, q0)                                                                      \
     (((nnm[k] >> lane) & 1) &&                                                               \
              pre[k] + __builtin_amdgcn_mbcnt_hi((uint32_t)(nnm[k] >> 32),                    \
@@ -899,121 +743,450 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 
          ? pre[k] + __builtin_amdgcn_mbcnt_hi((uint32_t)(nnm[k] >> 32),                      \
                                               __builtin_amdgcn_mbcnt_lo((uint32_t)nnm[k], 0u)) - (q0) \
          : 64u)
-            for (uint32_t q0 = 0; q0 < total; q0 += 64) {
-#pragma unroll
-                for (int k = 0; k < KB; ++k) {
-                    const uint32_t c = CPM_CIDX(k, q0);
-                    if (pre[k + 1] > q0 && pre[k] < q0 + 64 && c < 64)
-                        stage[c] = make_uint4(st_t[k], src[k], (uint32_t)st_u[k],
-                                              (uint32_t)(st_u[k] >> 32));
-                }
-                __syncthreads();
-                const uint32_t n_here = min(total - q0, 64u);
-                uint4 in = make_uint4(t_safe, t_safe, 0u, 0u);
-                if (lane < n_here) in = stage[lane];
-                __syncthreads();  // the stage is rewritten by the next pass
-                uint32_t ev_Lw, ev_Gw, ev_vL, ev_vG, ev_iL, ev_iG, ev_f;
-                int32_t ev_dl, ev_dg;
-                double ev_dh;
-                CPM_EVAL(in.x, in.y, (uint64_t)in.w << 32 | in.z, lane < n_here);
-#pragma unroll
-                for (int k = 0; k < KB; ++k) {
-                    if (pre[k + 1] > q0 && pre[k] < q0 + 64) {
-                        const uint32_t c = CPM_CIDX(k, q0);
-                        const uint32_t f = bperm(ev_f, c & 63);
-                        if (c < 64) st_m[k] |= f;
-                    }
-                }
-            }
-#undef CPM_CIDX
-        }
+    // An accepted attempt (slot 0, position commit_p): evaluated again on
+    // every lane, then committed (dynamics.rs:73-224).
+#define CPM_DO_COMMIT()                                                                         \
+    do {                                                                                        \
+        const uint32_t t = rl32(st_t[0], commit_p);                                             \
+        uint32_t ev_Lw, ev_Gw, ev_vL, ev_vG, ev_iL, ev_iG, ev_f;                                \
+        int32_t ev_dl, ev_dg;                                                                   \
+        double ev_dh;                                                                           \
+        const uint32_t s = t + rl32(coff, (rl32(st_m[0], commit_p) >> 16) & 31);                \
+        CPM_EVAL(t, s, rl64(st_u[0], commit_p), true);                                          \
+        (void)ev_f;                                                                             \
+        const uint32_t Lw = uni(ev_Lw), Gw = uni(ev_Gw);                                        \
+        const uint32_t L = Lw & kIdMask, G = Gw & kIdMask;                                      \
+        const int32_t dl_i = (int32_t)uni((uint32_t)ev_dl);                                     \
+        const int32_t dg_i = (int32_t)uni((uint32_t)ev_dg);                                     \
+        const uint32_t vL = uni(ev_vL), vG = uni(ev_vG);                                        \
+        const uint32_t iL = uni(ev_iL), iG = uni(ev_iG);                                        \
+        const double dh = rlf64(ev_dh, 0);                                                      \
+        uint32_t cval[D];                                                                       \
+        {                                                                                       \
+            uint32_t f = t;                                                                     \
+        _Pragma("unroll")                                                                       \
+            for (int d = 0; d < D; ++d) {                                                       \
+                cval[d] = f / stride[d] - 1;                                                    \
+                f %= stride[d];                                                                 \
+            }                                                                                   \
+        }                                                                                       \
+        /* dynamics.rs:73-98 — contact counts, pre-move lattice */                              \
+        if (lane < NE) {                                                                        \
+            const uint32_t nbe = sig.ld((int32_t)t + eoff) & kIdMask;                           \
+            if (nbe != L)                                                                       \
+                atomicSub(&R.contact[(uint64_t)min(L, nbe) * ncell1 + max(L, nbe)], 1u);        \
+            if (nbe != G)                                                                       \
+                atomicAdd(&R.contact[(uint64_t)min(G, nbe) * ncell1 + max(G, nbe)], 1u);        \
+        }                                                                                       \
+        wsync();  /* every lane has read t's neighbourhood */                                   \
+        if (lane == 0) {                                                                        \
+            sig.st(t, Gw);                                                                      \
+            if (L) {                                                                            \
+                vol[L - 1] = vL - 1;                                                            \
+                itf[L - 1] = (uint32_t)((int32_t)iL + dl_i);                                    \
+            }                                                                                   \
+            if (G) {                                                                            \
+                vol[G - 1] = vG + 1;                                                            \
+                itf[G - 1] = (uint32_t)((int32_t)iG + dg_i);                                    \
+            }                                                                                   \
+            /* dynamics.rs:125-183 — unwrapped position sums */                                 \
+            if (L) {                                                                            \
+                int64_t* sp = R.sum_pos + (uint64_t)(L - 1) * D;                                \
+                int64_t* so = R.sum_outer + (uint64_t)(L - 1) * D * D;                          \
+                for (int d = 0; d < D; ++d) sp[d] -= (int64_t)cval[d];                          \
+                for (int d = 0; d < D; ++d)                                                     \
+                    for (int e2 = 0; e2 < D; ++e2)                                              \
+                        so[d * D + e2] -= (int64_t)cval[d] * (int64_t)cval[e2];                 \
+            }                                                                                   \
+            if (G) {                                                                            \
+                int64_t* sp = R.sum_pos + (uint64_t)(G - 1) * D;                                \
+                int64_t* so = R.sum_outer + (uint64_t)(G - 1) * D * D;                          \
+                const uint32_t v = vG + 1;                                                      \
+                const uint32_t vbg = max(v > 0 ? v - 1 : 0u, 1u);                               \
+                double cen[D];                                                                  \
+                for (int d = 0; d < D; ++d) cen[d] = (double)sp[d] / (double)vbg;               \
+                int64_t img[D];                                                                 \
+                for (int d = 0; d < D; ++d) {                                                   \
+                    int64_t best = (int64_t)cval[d];                                            \
+                    if (periodic) {                                                             \
+                        double bd = fabs((double)best - cen[d]);                                \
+                        const int64_t lo = (int64_t)cval[d] - (int64_t)dims[d];                 \
+                        const int64_t hi = (int64_t)cval[d] + (int64_t)dims[d];                 \
+                        const double dlo = fabs((double)lo - cen[d]);                           \
+                        if (dlo < bd) {                                                         \
+                            best = lo;                                                          \
+                            bd = dlo;                                                           \
+                        }                                                                       \
+                        if (fabs((double)hi - cen[d]) < bd) best = hi;                          \
+                    }                                                                           \
+                    img[d] = best;                                                              \
+                    sp[d] += best;                                                              \
+                }                                                                               \
+                for (int d = 0; d < D; ++d)                                                     \
+                    for (int e2 = 0; e2 < D; ++e2) so[d * D + e2] += img[d] * img[e2];          \
+            }                                                                                   \
+            /* dynamics.rs:206-224 — rewrite t's halo mirrors */                                \
+            if (periodic) {                                                                     \
+                uint32_t opt[D][3], nopt[D], total = 1;                                         \
+                for (int d = 0; d < D; ++d) {                                                   \
+                    nopt[d] = 1;                                                                \
+                    opt[d][0] = cval[d] + 1;                                                    \
+                    if (cval[d] == 0) opt[d][nopt[d]++] = dims[d] + 1;                          \
+                    if (cval[d] == dims[d] - 1) opt[d][nopt[d]++] = 0;                          \
+                    total *= nopt[d];                                                           \
+                }                                                                               \
+                for (uint32_t q = 1; q < total; ++q) {                                          \
+                    uint32_t r = q, f = 0;                                                      \
+                    for (int d = D - 1; d >= 0; --d) {                                          \
+                        f += opt[d][r % nopt[d]] * stride[d];                                   \
+                        r /= nopt[d];                                                           \
+                    }                                                                           \
+                    sig.st(f, Gw);                                                              \
+                }                                                                               \
+            }                                                                                   \
+        }                                                                                       \
+        if constexpr (SG) __threadfence();  /* the next evaluation reads sigma from HBM */      \
+        wsync();                                                                                \
+        energy += dh;                                                                           \
+        /* the lattice changed: every later attempt is evaluated again */                       \
+        need_eval = true;                                                                       \
+    } while (0)
+    // Decode a block of nk windows from `base` (the lattice plays no part).
+#define CPM_DO_BLOCK(nk_)                                                                       \
+    do {                                                                                        \
+        const uint32_t nkb = (nk_);                                                                 \
+        if (!hr_ready) {                                                                        \
+        _Pragma("unroll")                                                                       \
+            for (int j = 0; j <= KB; ++j)                                                       \
+                hr[j] = base + 64 * j + 64 <= kFill ? draws[base + 64 * j + lane] : 0ull;       \
+        }                                                                                       \
+        Half h0, h1;                                                                            \
+        uint64_t m0[D + 1], m1[D + 1];                                                          \
+        prep_half<D>(h0, hr[0], range, zone, m0);                                               \
+        _Pragma("unroll")                                                                       \
+        for (int k = 0; k < KB; ++k) {                                                          \
+            if (k < (int)nkb) {                                                                  \
+                prep_half<D>(h1, hr[k + 1], range, zone, m1);                                   \
+                CPM_DECODE(k);                                                                  \
+                h0 = h1;                                                                        \
+        _Pragma("unroll")                                                                       \
+                for (int c = 0; c <= D; ++c) m0[c] = m1[c];                                     \
+            }                                                                                   \
+        }                                                                                       \
+        nvalid = nkb;                                                                           \
+        /* the next block's draws, in flight while this block is walked */                      \
+        hr_ready = nkb == (uint32_t)KB;                                                          \
+        if (hr_ready) {                                                                         \
+            hr[0] = hr[KB];                                                                     \
+        _Pragma("unroll")                                                                       \
+            for (int j = 1; j <= KB; ++j) {                                                     \
+                const uint32_t at = base + 64 * (KB + j);                                       \
+                hr[j] = at + 64 <= kFill ? draws[at + lane] : 0ull;                             \
+            }                                                                                   \
+        }                                                                                       \
+        need_block = false;                                                                     \
+        need_eval = true;                                                                       \
+    } while (0)
+    // Evaluate the block's non-null attempts, 64 per pass, packed onto
+    // consecutive lanes through the LDS stage.
+#define CPM_DO_EVAL()                                                                           \
+    do {                                                                                        \
+        uint64_t nnm[KB];                                                                       \
+        uint32_t pre[KB + 1], src[KB];                                                          \
+        pre[0] = 0;                                                                             \
+        _Pragma("unroll")                                                                       \
+        for (int k = 0; k < KB; ++k) {                                                          \
+            bool nn = false;                                                                    \
+            /* source site: target + offset of the copy index (a void */                        \
+            /* decode keeps its interior stand-in as both) */                                   \
+            src[k] = st_t[k] + ((st_m[k] & kMv) ? bperm(coff, (st_m[k] >> 16) & 31) : 0u);      \
+            if (k < (int)nvalid) {                                                              \
+                const uint32_t Lw = sig.ld(st_t[k]), Gw = sig.ld(src[k]);                       \
+                nn = (st_m[k] & kMv) && Lw != Gw && (k > 0 || lane >= p);                       \
+            }                                                                                   \
+            st_m[k] = (st_m[k] & (kMe | kMv | 0x1F0000u)) | (nn ? kMn : 0u);                    \
+            nnm[k] = uni64(__ballot(nn));                                                       \
+            pre[k + 1] = pre[k] + (uint32_t)__popcll(nnm[k]);                                   \
+        }                                                                                       \
+        const uint32_t total = pre[KB];                                                         \
+        for (uint32_t q0 = 0; q0 < total; q0 += 64) {                                           \
+        _Pragma("unroll")                                                                       \
+            for (int k = 0; k < KB; ++k) {                                                      \
+                const uint32_t c = CPM_CIDX(k, q0);                                             \
+                if (pre[k + 1] > q0 && pre[k] < q0 + 64 && c < 64)                              \
+                    stage[c] = make_uint4(st_t[k], src[k], (uint32_t)st_u[k],                   \
+                                          (uint32_t)(st_u[k] >> 32));                           \
+            }                                                                                   \
+            wsync();                                                                            \
+            const uint32_t n_here = min(total - q0, 64u);                                       \
+            uint4 in = make_uint4(t_safe, t_safe, 0u, 0u);                                      \
+            if (lane < n_here) in = stage[lane];                                                \
+            wsync();  /* the stage is rewritten by the next pass */                             \
+            uint32_t ev_Lw, ev_Gw, ev_vL, ev_vG, ev_iL, ev_iG, ev_f;                            \
+            int32_t ev_dl, ev_dg;                                                               \
+            double ev_dh;                                                                       \
+            CPM_EVAL(in.x, in.y, (uint64_t)in.w << 32 | in.z, lane < n_here);                   \
+        _Pragma("unroll")                                                                       \
+            for (int k = 0; k < KB; ++k) {                                                      \
+                if (pre[k + 1] > q0 && pre[k] < q0 + 64) {                                      \
+                    const uint32_t c = CPM_CIDX(k, q0);                                         \
+                    const uint32_t f = bperm(ev_f, c & 63);                                     \
+                    if (c < 64) st_m[k] |= f;                                                   \
+                }                                                                               \
+            }                                                                                   \
+        }                                                                                       \
+    } while (0)
+    // Walk the chain through slot 0 up to the first accepted copy, the
+    // window's end or the MCS target, in one step: lane k finds the k-th
+    // attempt position x = next^k(p) by pointer doubling.  An attempt takes
+    // at least D + 1 >= 3 draws, so at most 22 attempts start in a 64-draw
+    // window and 5 doublings suffice.  walk_hit: it stopped at an accepted
+    // copy (now pending, with p past it).
+#define CPM_DO_WALK()                                                                           \
+    do {                                                                                        \
+        const uint32_t m_own = st_m[0];                                                         \
+        {                                                                                       \
+            const uint32_t e = m_own & kMe, u1 = (m_own & kMu) ? 1u : 0u;                       \
+            const bool okb = (m_own & kMv) && (!u1 || e < 128);                                 \
+            uint32_t jb = okb ? e + u1 : 127u;  /* next^(2^b); >= 64 = leaves */                \
+            uint32_t x = p;                                                                     \
+        _Pragma("unroll")                                                                       \
+            for (int b = 0; b < 5; ++b) {                                                       \
+                const uint32_t y = bperm(jb, x & 63);                                           \
+                if (((lane >> b) & 1) && x < 64) x = y;                                         \
+                if (b < 4) {                                                                    \
+                    const uint32_t z = bperm(jb, jb & 63);                                      \
+                    if (jb < 64) jb = z;                                                        \
+                }                                                                               \
+            }                                                                                   \
+            const uint32_t fx = bperm(m_own, x & 63);  /* meta of the lane's attempt */         \
+            const bool okx = (fx & kMv) && (!(fx & kMu) || (fx & kMe) < 128);                   \
+            const bool vk = lane < 32 && x < 64 && okx;                                         \
+            const uint32_t k_inv = (uint32_t)__builtin_ctzll(uni64(__ballot(!vk)));             \
+            const uint64_t acc_k = uni64(__ballot(vk && (fx & kMa)));                           \
+            const uint32_t k_acc = acc_k ? (uint32_t)__builtin_ctzll(acc_k) : 64u;              \
+            /* attempts left before the MCS target: (todo - 1) * n_sites + (n_sites - ain) */   \
+            const uint64_t left = (todo - 1) * (uint64_t)n_sites + (n_sites - ain);             \
+            uint32_t n_att = min(k_inv, left < 64 ? (uint32_t)left : 64u);                      \
+            const bool hit = k_acc < n_att;                                                     \
+            if (hit) n_att = k_acc + 1;                                                         \
+            {                                                                                   \
+                const bool w = lane < n_att;                                                    \
+                c_att += n_att;                                                                 \
+                c_nn += __popcll(uni64(__ballot(w && (fx & kMn))));                             \
+                c_top += __popcll(uni64(__ballot(w && (fx & kMp))));                            \
+                c_en += __popcll(uni64(__ballot(w && (fx & kMg))));                             \
+                c_acc += hit ? 1u : 0u;                                                         \
+                c_tie += __popcll(uni64(__ballot(w && (fx & kMt))));                            \
+            }                                                                                   \
+            /* monte_carlo.rs:330-356 — MCS accounting */                                       \
+            ain += n_att;                                                                       \
+            while (ain >= n_sites) {                                                            \
+                ain -= n_sites;                                                                 \
+                ++mcs;                                                                          \
+                --todo;                                                                         \
+            }                                                                                   \
+            const uint32_t np = rl32(x, n_att);  /* the position after the walked attempts */   \
+            p = np;                                                                             \
+            walk_hit = hit;                                                                     \
+            if (hit) {                                                                          \
+                commit_p = rl32(x, k_acc);                                                      \
+                pending = true;                                                                 \
+            }                                                                                   \
+        }                                                                                       \
+    } while (0)
+    // Slot 0 is walked: the next window becomes slot 0.
+#define CPM_SHIFT_SLOTS()                                                                       \
+    do {                                                                                        \
+        _Pragma("unroll")                                                                       \
+        for (int k = 0; k + 1 < KB; ++k) {                                                      \
+            st_t[k] = st_t[k + 1];                                                              \
+            st_m[k] = st_m[k + 1];                                                              \
+            st_u[k] = st_u[k + 1];                                                              \
+        }                                                                                       \
+        base += 64;                                                                             \
+        p -= 64;                                                                                \
+        --nvalid;                                                                               \
+    } while (0)
+    // Pipelined producer: decode and evaluate the block at pb into block
+    // buffer b (none when the stream buffer must first be refilled and the
+    // walker may still need it).
+#define CPM_PRODUCE(b_, idle_)                                                                   \
+    do {                                                                                         \
+        const uint32_t bb = (b_);                                                                  \
+        const bool walker_idle = (idle_);                                                        \
+        uint32_t room = kFill - pb;                                                              \
+        uint32_t nk = room >= 128 ? min((uint32_t)KB, (room - 64) / 64) : 0u;                    \
+        if (nk == 0 && walker_idle) {                                                            \
+            CPM_REFILL(pb);                                                                      \
+            pb = base;                                                                           \
+            room = kFill - pb;                                                                   \
+            nk = min((uint32_t)KB, (room - 64) / 64);                                            \
+        }                                                                                        \
+        if (nk) {                                                                                \
+            base = pb;                                                                           \
+            p = 0;                                                                               \
+            CPM_DO_BLOCK(nk);                                                                    \
+            CPM_DO_EVAL();                                                                       \
+            need_eval = false;                                                                   \
+        _Pragma("unroll")                                                                        \
+            for (int k = 0; k < KB; ++k)                                                         \
+                blk[(bb * KB + k) * 64 + lane] =                                                  \
+                    make_uint4(st_t[k], st_m[k], (uint32_t)st_u[k], (uint32_t)(st_u[k] >> 32));  \
+            pb += 64 * nk;                                                                       \
+        }                                                                                        \
+        if (lane == 0) {                                                                         \
+            s_nv[bb] = nk;                                                                        \
+            s_base[bb] = base;                                                                    \
+        }                                                                                        \
+    } while (0)
 
-        // ---- walk the chain through slot 0 up to the first accepted copy,
-        // the window's end or the MCS target, in one step: lane k finds the
-        // k-th attempt position x = next^k(p) by pointer doubling.  An
-        // attempt takes at least D + 1 >= 3 draws, so at most 22 attempts
-        // start in a 64-draw window and 5 doublings suffice.
-        const uint32_t m_own = st_m[0];
-        {
-            const uint32_t e = m_own & kMe, u1 = (m_own & kMu) ? 1u : 0u;
-            const bool okb = (m_own & kMv) && (!u1 || e < 128);
-            uint32_t jb = okb ? e + u1 : 127u;  // next^(2^b); >= 64 = leaves
-            uint32_t x = p;
-#pragma unroll
-            for (int b = 0; b < 5; ++b) {
-                const uint32_t y = bperm(jb, x & 63);
-                if (((lane >> b) & 1) && x < 64) x = y;
-                if (b < 4) {
-                    const uint32_t z = bperm(jb, jb & 63);
-                    if (jb < 64) jb = z;
+    if constexpr (!PW) {
+        for (;;) {
+            if (pending) {
+                pending = false;
+                dirty = true;
+                CPM_DO_COMMIT();
+            }
+            if (todo == 0) break;
+            if (need_block) {
+                const uint32_t room = kFill - base;
+                const uint32_t nk = room >= 128 ? min((uint32_t)KB, (room - 64) / 64) : 0u;
+                if (nk == 0) {
+                    CPM_REFILL(base + p);
+                    continue;
                 }
+                CPM_DO_BLOCK(nk);
             }
-            const uint32_t fx = bperm(m_own, x & 63);  // meta of the lane's attempt
-            const bool okx = (fx & kMv) && (!(fx & kMu) || (fx & kMe) < 128);
-            const bool vk = lane < 32 && x < 64 && okx;
-            const uint32_t k_inv = (uint32_t)__builtin_ctzll(uni64(__ballot(!vk)));
-            const uint64_t acc_k = uni64(__ballot(vk && (fx & kMa)));
-            const uint32_t k_acc = acc_k ? (uint32_t)__builtin_ctzll(acc_k) : 64u;
-            // attempts left before the MCS target: (todo - 1) * n_sites + (n_sites - ain)
-            const uint64_t left = (todo - 1) * (uint64_t)n_sites + (n_sites - ain);
-            uint32_t n_att = min(k_inv, left < 64 ? (uint32_t)left : 64u);
-            const bool hit = k_acc < n_att;
-            if (hit) n_att = k_acc + 1;
-            {
-                const bool w = lane < n_att;
-                c_att += n_att;
-                c_nn += __popcll(uni64(__ballot(w && (fx & kMn))));
-                c_top += __popcll(uni64(__ballot(w && (fx & kMp))));
-                c_en += __popcll(uni64(__ballot(w && (fx & kMg))));
-                c_acc += hit ? 1u : 0u;
-                c_tie += __popcll(uni64(__ballot(w && (fx & kMt))));
+            if (need_eval) {
+                need_eval = false;
+                CPM_DO_EVAL();
             }
-            // monte_carlo.rs:330-356 — MCS accounting
-            ain += n_att;
-            while (ain >= n_sites) {
-                ain -= n_sites;
-                ++mcs;
-                --todo;
-            }
-            const uint32_t np = rl32(x, n_att);  // the position after the walked attempts
-            if (hit) {
-                commit_p = rl32(x, k_acc);
-                pending = true;
-                p = np;
+            CPM_DO_WALK();
+            if (walk_hit) continue;
+            if (todo == 0) break;
+            if (p >= 64) {
+                CPM_SHIFT_SLOTS();
+                if (nvalid == 0) need_block = true;
                 continue;
             }
-            p = np;
-        }
-        if (todo == 0) break;
-        if (p >= 64) {  // slot 0 is walked: the next window becomes slot 0
-#pragma unroll
-            for (int k = 0; k + 1 < KB; ++k) {
-                st_t[k] = st_t[k + 1];
-                st_m[k] = st_m[k + 1];
-                st_u[k] = st_u[k + 1];
+            // The attempt at p does not fit in its window: restart the block
+            // at it.  An attempt needing more than 128 draws even from a
+            // window of its own is reported, never looped on.
+            if (p == 0) {
+                error = true;
+                break;
             }
-            base += 64;
-            p -= 64;
-            if (--nvalid == 0) need_block = true;
-            continue;
+            base += p;
+            p = 0;
+            hr_ready = false;
+            need_block = true;
         }
-        // The attempt at p does not fit in its window: restart the block at
-        // it.  An attempt needing more than 128 draws even from a window of
-        // its own is reported, never looped on.
-        if (p == 0) {
-            error = true;
-            break;
+    } else {
+        // Pipelined: wave 1 decodes and evaluates block i + 1 while wave 0
+        // walks block i, one workgroup barrier per block.  The walker
+        // commits as above and evaluates again whatever its own commits made
+        // stale (the rest of its block, and the next block when it committed
+        // while that one was being evaluated).  Shared words, by iteration
+        // parity: the walker's command for the next iteration (go on,
+        // restart at s_pos, done) and whether it committed; each block
+        // buffer's start and window count.  The stream buffer is refilled
+        // only while the walker has no block, so the walker's final position
+        // always lies in the current buffer.
+        __shared__ uint32_t s_cmd[2], s_pos[2], s_cmt[2], s_nv[2], s_base[2];
+        constexpr uint32_t kGo = 0, kRestart = 1, kDone = 2;
+        uint4* const blk = (uint4*)(smem + Ly.total + 2 * kStageBytes);  // [2][KB][64]
+        uint32_t pb = 0;  // producer: where its next block starts
+        if (wave == 1) {
+            CPM_PRODUCE(0u, true);
+        } else if (lane == 0) {
+            s_cmd[0] = kGo;
+            s_cmt[0] = 0;
         }
-        base += p;
-        p = 0;
-        hr_ready = false;
-        need_block = true;
+        __syncthreads();
+        bool stale = false;
+        for (uint32_t it = 0;; ++it) {
+            const uint32_t cur = it & 1, nxt = cur ^ 1;
+            const uint32_t cmd = s_cmd[cur];
+            if (cmd == kDone) break;
+            if (wave == 1) {
+                if (cmd == kRestart) {
+                    pb = s_pos[cur];
+                    hr_ready = false;
+                }
+                if constexpr (SG) {
+                    if (s_cmt[cur]) __threadfence();  // sigma words the walker rewrote
+                }
+                CPM_PRODUCE(nxt, cmd == kRestart || s_nv[cur] == 0);
+            } else {
+                uint32_t next_cmd = kGo, next_pos = 0;
+                bool committed = false;
+                if (cmd != kRestart && s_nv[cur] > 0) {
+                    nvalid = s_nv[cur];
+                    base = s_base[cur];
+#pragma unroll
+                    for (int k = 0; k < KB; ++k) {
+                        const uint4 v = blk[(cur * KB + k) * 64 + lane];
+                        st_t[k] = v.x;
+                        st_m[k] = v.y;
+                        st_u[k] = (uint64_t)v.w << 32 | v.z;
+                    }
+                    need_eval = stale;
+                    for (;;) {
+                        if (pending) {
+                            pending = false;
+                            dirty = true;
+                            committed = true;
+                            CPM_DO_COMMIT();
+                        }
+                        if (todo == 0) break;
+                        if (need_eval) {
+                            need_eval = false;
+                            CPM_DO_EVAL();
+                        }
+                        CPM_DO_WALK();
+                        if (walk_hit) continue;
+                        if (todo == 0) break;
+                        if (p >= 64) {
+                            CPM_SHIFT_SLOTS();
+                            if (nvalid == 0) break;
+                            continue;
+                        }
+                        if (p == 0) {
+                            error = true;
+                            break;
+                        }
+                        next_cmd = kRestart;
+                        next_pos = base + p;
+                        p = 0;
+                        break;
+                    }
+                }
+                if (todo == 0 || error) next_cmd = kDone;
+                stale = committed;
+                if (lane == 0) {
+                    s_cmd[nxt] = next_cmd;
+                    s_pos[nxt] = next_pos;
+                    s_cmt[nxt] = committed ? 1u : 0u;
+                }
+            }
+            __syncthreads();
+        }
+        if (wave == 1) return;
+        __threadfence();  // the producer's last refill of the segment states
     }
     base += p;  // buffer position of the next unconsumed draw
 #undef CPM_DECODE
 #undef CPM_EVAL
 #undef CPM_REFILL
-    __syncthreads();
+#undef CPM_CIDX
+#undef CPM_DO_COMMIT
+#undef CPM_DO_BLOCK
+#undef CPM_DO_EVAL
+#undef CPM_DO_WALK
+#undef CPM_SHIFT_SLOTS
+#undef CPM_PRODUCE
+    wsync();
     // the consumer's RNG state: the segment's start state (a vector load:
     // it was written by vector stores) stepped to the cursor
     const uint32_t g_end = min(base / kSegLen, kSegs - 1);
@@ -1049,11 +1222,14 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D == 2 
     }
 }
 
-// key = 1000 * (sigma in HBM) + 100 * D + energy stencil size
-#define CPM_WAVE_KERNELS(X)                                                            \
-    X(204, 2, 4, false) X(208, 2, 8, false) X(306, 3, 6, false) X(318, 3, 18, false)   \
-    X(326, 3, 26, false) X(1204, 2, 4, true) X(1208, 2, 8, true) X(1306, 3, 6, true)   \
-    X(1318, 3, 18, true) X(1326, 3, 26, true)
+// key = 2000 * (pipelined) + 1000 * (sigma in HBM) + 100 * D + energy stencil size
+#define CPM_WAVE_KERNELS_PW(X, pw, k0)                                                 \
+    X(k0 + 204, 2, 4, false, pw) X(k0 + 208, 2, 8, false, pw)                          \
+    X(k0 + 306, 3, 6, false, pw) X(k0 + 318, 3, 18, false, pw)                         \
+    X(k0 + 326, 3, 26, false, pw) X(k0 + 1204, 2, 4, true, pw)                         \
+    X(k0 + 1208, 2, 8, true, pw) X(k0 + 1306, 3, 6, true, pw)                          \
+    X(k0 + 1318, 3, 18, true, pw) X(k0 + 1326, 3, 26, true, pw)
+#define CPM_WAVE_KERNELS(X) CPM_WAVE_KERNELS_PW(X, false, 0) CPM_WAVE_KERNELS_PW(X, true, 2000)
 
 // ===========================================================================
 // Sublattice-parallel mode: halo-disjoint checkerboard colours updated in
@@ -1767,6 +1943,9 @@ struct Device {
 // LDS kept free of the dynamic allocation for the kernels' static LDS
 // (the sublattice kernel's counters and the compiler's own use)
 constexpr uint32_t kLdsStaticReserve = 1024;
+// Replicas whose chain (MCS x sites) is at least this share of the batch's
+// longest run on the pipelined kernel.
+constexpr double kPipeShare = 0.75;
 // Share of the chip's LDS the sigma placement plans to fill (workgroups of
 // mixed sizes do not pack a CU's LDS perfectly): the default, and the
 // candidates the autotuner tries (0: sigma in HBM for every replica; 8:
@@ -1836,7 +2015,7 @@ Engine* engine_create(int device) {
     int n_cu = 0;
     HIP_OK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device));
     e->n_cu = (uint32_t)n_cu;
-#define CPM_FN(k, d, ne, gs) (const void*)k_lds_engine<d, ne, gs>,
+#define CPM_FN(k, d, ne, gs, pw) (const void*)k_lds_engine<d, ne, gs, pw>,
     for (auto fn : {CPM_WAVE_KERNELS(CPM_FN)(const void*)k_sublattice<2, false, kSlThreads>,
                     (const void*)k_sublattice<3, false, kSlThreads>})
 #undef CPM_FN
@@ -2139,10 +2318,10 @@ static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint
 
 static void launch_lds_engine(int key, uint32_t lds, uint32_t count, hipStream_t st, Rep* reps,
                               const uint32_t* idx, const uint64_t* sp3d) {
-    const dim3 g(count), b(64);
+    const dim3 g(count), b(key >= 2000 ? 128 : 64);
     switch (key) {
-#define CPM_CASE(k, d, ne, gs)                                                                \
-    case k: hipLaunchKernelGGL((k_lds_engine<d, ne, gs>), g, b, lds, st, reps, idx, sp3d); break;
+#define CPM_CASE(k, d, ne, gs, pw)                                                            \
+    case k: hipLaunchKernelGGL((k_lds_engine<d, ne, gs, pw>), g, b, lds, st, reps, idx, sp3d); break;
         CPM_WAVE_KERNELS(CPM_CASE)
 #undef CPM_CASE
         default: throw Error(CPM_ERR_DEVICE, "no exact-replay kernel for this stencil");
@@ -2168,6 +2347,38 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
     // from the start and the largest lattices' serial chains set the time);
     // the rest keep sigma in HBM (read through L2) and use LDS for their
     // per-cell tables only.
+    // The replicas with the longest chains (and a lone replica) run on the
+    // pipelined kernel, a second wave decoding and evaluating ahead of the
+    // walking one, while the batch leaves SIMDs without a wave: where every
+    // SIMD is busy the second wave would only take issue slots from others.
+    std::vector<char> piped(n, 0);
+    {
+        double max_work = 0.0;
+        size_t waves = 0;
+        std::vector<uint32_t> cand;
+        for (size_t i = 0; i < n; ++i) {
+            const Model* m = models[i];
+            if (!todo[i] || !m->dev->lds) continue;
+            ++waves;
+            max_work = std::max(max_work, (double)todo[i] * (double)m->n_sites);
+            if (m->dev->lds_bytes_hbm + pipe_extra_bytes(m->D) + kLdsStaticReserve <= e->max_lds)
+                cand.push_back((uint32_t)i);
+        }
+        auto work = [&](uint32_t i) { return (double)todo[i] * (double)models[i]->n_sites; };
+        std::stable_sort(cand.begin(), cand.end(),
+                         [&](uint32_t a, uint32_t b) { return work(a) > work(b); });
+        const size_t simds = 4 * (size_t)e->n_cu;
+        for (uint32_t i : cand) {
+            if (work(i) < kPipeShare * max_work || waves + 1 > simds) break;
+            piped[i] = 1;
+            ++waves;
+        }
+    }
+    auto lds_need = [&](size_t i, bool hbm) {
+        const Device* d = models[i]->dev;
+        return (hbm ? d->lds_bytes_hbm : d->lds_bytes) +
+               (piped[i] ? pipe_extra_bytes(models[i]->D) : 0u);
+    };
     std::vector<char> in_hbm(n, 0);
     {
         std::vector<uint32_t> cand;
@@ -2176,17 +2387,17 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
             const Model* m = models[i];
             if (!todo[i] || !m->dev->lds) continue;
             in_hbm[i] = 1;
-            demand += (double)lds_class(e, m->dev->lds_bytes_hbm);
-            if (!m->dev->gsig) cand.push_back((uint32_t)i);
+            demand += (double)lds_class(e, lds_need(i, true));
+            if (!m->dev->gsig && lds_need(i, false) + kLdsStaticReserve <= e->max_lds)
+                cand.push_back((uint32_t)i);
         }
         std::stable_sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) {
             return models[a]->dev->lds_bytes < models[b]->dev->lds_bytes;
         });
         const double capacity = fill * (double)e->n_cu * (double)e->max_lds;
         for (uint32_t i : cand) {
-            const Device* d = models[i]->dev;
             const double extra =
-                (double)lds_class(e, d->lds_bytes) - (double)lds_class(e, d->lds_bytes_hbm);
+                (double)lds_class(e, lds_need(i, false)) - (double)lds_class(e, lds_need(i, true));
             if (demand + extra > capacity) break;
             demand += extra;
             in_hbm[i] = 0;
@@ -2210,7 +2421,8 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
         reps[i] = R;
         if (!todo[i]) continue;
         if (m->dev->lds)
-            by_kernel[(in_hbm[i] ? 1000 : 0) + m->D * 100 + m->n_energy].push_back((uint32_t)i);
+            by_kernel[(piped[i] ? 2000 : 0) + (in_hbm[i] ? 1000 : 0) + m->D * 100 + m->n_energy]
+                .push_back((uint32_t)i);
         else
             global_idx.push_back((uint32_t)i);
     }
@@ -2233,10 +2445,8 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
             Engine::kClassStreams > n_kernels ? Engine::kClassStreams - n_kernels : 0;
         for (auto& [key, v] : by_kernel) {
             std::vector<std::pair<uint32_t, uint32_t>> cls(v.size());  // (class bytes, replica)
-            for (size_t j = 0; j < v.size(); ++j) {
-                const Device* d = models[v[j]]->dev;
-                cls[j] = {lds_class(e, key >= 1000 ? d->lds_bytes_hbm : d->lds_bytes), v[j]};
-            }
+            for (size_t j = 0; j < v.size(); ++j)
+                cls[j] = {lds_class(e, lds_need(v[j], in_hbm[v[j]] != 0)), v[j]};
             std::stable_sort(cls.begin(), cls.end(),
                              [](auto& a, auto& b) { return a.first < b.first; });
             // distinct classes with counts
