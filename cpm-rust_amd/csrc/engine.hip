@@ -608,6 +608,14 @@ __global__ void __launch_bounds__(PW ? 128 : 64) __attribute__((amdgpu_waves_per
     uint32_t nvalid = 0, commit_p = 0;
     bool need_block = true, need_eval = false, hr_ready = false, pending = false;
     bool walk_hit = false;
+    // Copies committed since the block's evaluation (padded coordinates and
+    // the two cells), so that only attempts they can have changed are
+    // evaluated again; clog_all: evaluate everything (a copy on a periodic
+    // boundary, whose halo mirrors also changed, or a full log).
+    constexpr uint32_t kClog = 8;
+    __shared__ uint32_t s_clog[kClog][4];
+    uint32_t clog_n = 0;
+    bool clog_all = false, eval_sel = false;
 
     // The buffer is restarted at the segment holding position `at` (its
     // draws regenerated from the segment's start state).
@@ -770,6 +778,23 @@ __global__ void __launch_bounds__(PW ? 128 : 64) __attribute__((amdgpu_waves_per
                 f %= stride[d];                                                                 \
             }                                                                                   \
         }                                                                                       \
+        /* log the copy for the selective re-evaluation */                                      \
+        {                                                                                       \
+            bool edge = false;                                                                  \
+        _Pragma("unroll")                                                                       \
+            for (int d = 0; d < D; ++d) edge = edge || cval[d] == 0 || cval[d] + 1 == dims[d];  \
+            if ((periodic && edge) || clog_n == kClog) {                                        \
+                clog_all = true;                                                                \
+            } else {                                                                            \
+                if (lane == 0) {                                                                \
+                    s_clog[clog_n][0] = cval[0] + 1;                                            \
+                    s_clog[clog_n][1] = cval[1 % D] + 1;                                        \
+                    s_clog[clog_n][2] = D == 3 ? cval[D - 1] + 1 : 0u;                          \
+                    s_clog[clog_n][3] = L | G << 16;                                            \
+                }                                                                               \
+                ++clog_n;                                                                       \
+            }                                                                                   \
+        }                                                                                       \
         /* dynamics.rs:73-98 — contact counts, pre-move lattice */                              \
         if (lane < NE) {                                                                        \
             const uint32_t nbe = sig.ld((int32_t)t + eoff) & kIdMask;                           \
@@ -900,12 +925,34 @@ __global__ void __launch_bounds__(PW ? 128 : 64) __attribute__((amdgpu_waves_per
             /* source site: target + offset of the copy index (a void */                        \
             /* decode keeps its interior stand-in as both) */                                   \
             src[k] = st_t[k] + ((st_m[k] & kMv) ? bperm(coff, (st_m[k] >> 16) & 31) : 0u);      \
+            bool want = false;                                                                  \
             if (k < (int)nvalid) {                                                              \
                 const uint32_t Lw = sig.ld(st_t[k]), Gw = sig.ld(src[k]);                       \
                 nn = (st_m[k] & kMv) && Lw != Gw && (k > 0 || lane >= p);                       \
+                want = nn;                                                                      \
+                if (eval_sel && nn && (st_m[k] & kMn) && !clog_all) {                           \
+                    /* still non-null and evaluated before: again only if a */                  \
+                    /* logged copy lies in its 3^D box or changed its cells */                  \
+                    const uint32_t Li = Lw & kIdMask, Gi = Gw & kIdMask;                        \
+                    const uint32_t c0 = st_t[k] / stride[0], rem = st_t[k] - c0 * stride[0];    \
+                    const uint32_t c1 = D == 3 ? rem / stride[1 % D] : rem;                     \
+                    const uint32_t c2 = D == 3 ? rem - c1 * stride[1 % D] : 0u;                 \
+                    bool aff = false;                                                           \
+                    for (uint32_t j = 0; j < clog_n; ++j) {                                     \
+                        const uint32_t cells = s_clog[j][3];                                    \
+                        const uint32_t cl = cells & 0xFFFF, cg = cells >> 16;                   \
+                        const bool box = c0 + 1 - s_clog[j][0] <= 2u &&                         \
+                                         c1 + 1 - s_clog[j][1] <= 2u &&                         \
+                                         (D == 2 || c2 + 1 - s_clog[j][2] <= 2u);               \
+                        aff = aff || box || (Li && (Li == cl || Li == cg)) ||                   \
+                              (Gi && (Gi == cl || Gi == cg));                                   \
+                    }                                                                           \
+                    want = aff;                                                                 \
+                }                                                                               \
             }                                                                                   \
-            st_m[k] = (st_m[k] & (kMe | kMv | 0x1F0000u)) | (nn ? kMn : 0u);                    \
-            nnm[k] = uni64(__ballot(nn));                                                       \
+            if (want || !nn)                                                                    \
+                st_m[k] = (st_m[k] & (kMe | kMv | 0x1F0000u)) | (nn ? kMn : 0u);                \
+            nnm[k] = uni64(__ballot(want));                                                     \
             pre[k + 1] = pre[k] + (uint32_t)__popcll(nnm[k]);                                   \
         }                                                                                       \
         const uint32_t total = pre[KB];                                                         \
@@ -1027,6 +1074,7 @@ __global__ void __launch_bounds__(PW ? 128 : 64) __attribute__((amdgpu_waves_per
             base = pb;                                                                           \
             p = 0;                                                                               \
             CPM_DO_BLOCK(nk);                                                                    \
+            eval_sel = false;                                                                    \
             CPM_DO_EVAL();                                                                       \
             need_eval = false;                                                                   \
         _Pragma("unroll")                                                                        \
@@ -1047,6 +1095,7 @@ __global__ void __launch_bounds__(PW ? 128 : 64) __attribute__((amdgpu_waves_per
                 pending = false;
                 dirty = true;
                 CPM_DO_COMMIT();
+                eval_sel = true;
             }
             if (todo == 0) break;
             if (need_block) {
@@ -1057,6 +1106,9 @@ __global__ void __launch_bounds__(PW ? 128 : 64) __attribute__((amdgpu_waves_per
                     continue;
                 }
                 CPM_DO_BLOCK(nk);
+                clog_n = 0;
+                clog_all = false;
+                eval_sel = false;
             }
             if (need_eval) {
                 need_eval = false;
@@ -1085,9 +1137,9 @@ __global__ void __launch_bounds__(PW ? 128 : 64) __attribute__((amdgpu_waves_per
     } else {
         // Pipelined: wave 1 decodes and evaluates block i + 1 while wave 0
         // walks block i, one workgroup barrier per block.  The walker
-        // commits as above and evaluates again whatever its own commits made
-        // stale (the rest of its block, and the next block when it committed
-        // while that one was being evaluated).  Shared words, by iteration
+        // commits as above and evaluates again the attempts its own commits
+        // can have changed (in the rest of its block, and in the next block
+        // for the copies it made while that one was being evaluated).  Shared words, by iteration
         // parity: the walker's command for the next iteration (go on,
         // restart at s_pos, done) and whether it committed; each block
         // buffer's start and window count.  The stream buffer is refilled
@@ -1104,7 +1156,6 @@ __global__ void __launch_bounds__(PW ? 128 : 64) __attribute__((amdgpu_waves_per
             s_cmt[0] = 0;
         }
         __syncthreads();
-        bool stale = false;
         for (uint32_t it = 0;; ++it) {
             const uint32_t cur = it & 1, nxt = cur ^ 1;
             const uint32_t cmd = s_cmd[cur];
@@ -1131,7 +1182,12 @@ __global__ void __launch_bounds__(PW ? 128 : 64) __attribute__((amdgpu_waves_per
                         st_m[k] = v.y;
                         st_u[k] = (uint64_t)v.w << 32 | v.z;
                     }
-                    need_eval = stale;
+                    // the producer evaluated this block while the walker made
+                    // the logged copies: evaluate again what they touched
+                    eval_sel = true;
+                    if (clog_n > 0 || clog_all) CPM_DO_EVAL();
+                    clog_n = 0;
+                    clog_all = false;
                     for (;;) {
                         if (pending) {
                             pending = false;
@@ -1161,9 +1217,11 @@ __global__ void __launch_bounds__(PW ? 128 : 64) __attribute__((amdgpu_waves_per
                         p = 0;
                         break;
                     }
+                } else {
+                    clog_n = 0;  // nothing changed while this iteration's block was made
+                    clog_all = false;
                 }
                 if (todo == 0 || error) next_cmd = kDone;
-                stale = committed;
                 if (lane == 0) {
                     s_cmd[nxt] = next_cmd;
                     s_pos[nxt] = next_pos;
