@@ -65,7 +65,7 @@ class CpmCounters(C.Structure):
 
 class CpmServerOptions(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_active", C.c_uint32),
-                ("quantum_attempts", C.c_uint64)]
+                ("quantum_attempts", C.c_uint64), ("round_ms", C.c_double)]
 
 
 class CpmRequestInfo(C.Structure):

@@ -55,12 +55,16 @@ class Server:
 
     hbm_budget: device bytes requests may hold at once (None: 80 % of the
     free device memory); max_active: requests on the device at once;
-    quantum_attempts: copy attempts per request per round.
+    round_ms: target round length — an exact-replay request runs the MCS
+    its chain does in that time at the speed measured in its last round;
+    quantum_attempts: copy attempts per request per round before that
+    speed is known, and in sublattice mode.
     """
 
     def __init__(self, hbm_budget: int | None = None, max_active: int = 4096,
-                 quantum_attempts: int = 1 << 18):
-        opt = N.CpmServerOptions(int(hbm_budget or 0), int(max_active), int(quantum_attempts))
+                 quantum_attempts: int = 1 << 18, round_ms: float = 30.0):
+        opt = N.CpmServerOptions(int(hbm_budget or 0), int(max_active), int(quantum_attempts),
+                                 float(round_ms))
         self.h = C.c_void_p()
         N.check(N.lib().cpm_serve_create(N.engine(), C.byref(opt), C.byref(self.h)))
         self._lock = threading.Lock()

@@ -377,6 +377,7 @@ cpm_status cpm_serve_create(cpm_engine* engine, const cpm_server_options* option
             o.hbm_budget = options->hbm_budget_bytes;
             o.max_active = options->max_active;
             o.quantum = options->quantum_attempts;
+            o.round_s = options->round_ms * 1e-3;
         }
         *out = new cpm_server{serve_create(engine->e, o)};
     });

@@ -79,6 +79,7 @@ struct Rep {
     uint32_t error;           // an attempt needed more draws than a window holds
     uint64_t fill_state[4];   // sublattice mode: [0] keys its randomness
     unsigned long long cnt[6];
+    uint64_t t_run;           // wave path: 100 MHz ticks the replica's chain ran in this call
 };
 
 __constant__ uint32_t c_sp2d[8];  // 256-bit 2D simple-point table
@@ -504,6 +505,7 @@ __global__ void __launch_bounds__(PW ? 128 : 64) __attribute__((amdgpu_waves_per
     extern __shared__ __align__(16) unsigned char smem[];
     Rep& R = reps[idx[blockIdx.x]];
     if (R.done) return;
+    const uint64_t t_begin = __builtin_amdgcn_s_memrealtime();
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = PW ? threadIdx.x >> 6 : 0u;  // pipelined: 0 walks, 1 decodes/evaluates
     const uint32_t T = R.n_types, T1 = T + 1, n_cells = R.n_cells;
@@ -1269,6 +1271,7 @@ __global__ void __launch_bounds__(PW ? 128 : 64) __attribute__((amdgpu_waves_per
         R.cnt[3] += c_en;
         R.cnt[4] += c_acc;
         R.cnt[5] += c_tie;
+        R.t_run = __builtin_amdgcn_s_memrealtime() - t_begin;
         R.done = 1;
         if (error) {
             R.error = 1;
@@ -2010,6 +2013,7 @@ constexpr double kPipeShare = 0.75;
 // sigma in LDS wherever it fits).
 constexpr double kLdsPlacementFill = 0.75;
 constexpr double kFillCandidates[] = {0.0, 0.5, 0.75, 1.0, 8.0};
+constexpr uint64_t kTuneMinSliceMcs = 4;  // tune only calls of at least 80 MCS (4 per slice)
 constexpr int kNumFillCandidates = sizeof(kFillCandidates) / sizeof(kFillCandidates[0]);
 
 struct Engine {
@@ -2476,6 +2480,7 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
         R.done = todo[i] == 0;
         R.error = 0;
         for (int k = 0; k < 6; ++k) R.cnt[k] = 0;
+        R.t_run = 0;
         reps[i] = R;
         if (!todo[i]) continue;
         if (m->dev->lds)
@@ -2629,6 +2634,9 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
             m->energy = reps[i].energy;
             m->mcs = reps[i].mcs;
             m->device_dirty = true;
+            // the chain's own speed in this call (wave path), for schedulers
+            if (reps[i].t_run > 0 && reps[i].cnt[0] > 0)
+                m->attempt_rate = (double)reps[i].cnt[0] / ((double)reps[i].t_run * 1e-8);
         }
         add_counters(counters, reps[i]);
     }
@@ -2826,7 +2834,9 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
     // two slices per candidate, the second round in reverse order so that
     // the lattices' drift during the call (early MCS accept more) cancels
     const uint64_t slice = max_todo / (4 * kNumFillCandidates);  // 25 % of the call in all
-    if (it != e->tuned_fill.end() || slice == 0 || n < 2) {
+    // short calls (serving rounds, sampling intervals of a few MCS) are not
+    // worth the extra launches: they run with the default placement
+    if (it != e->tuned_fill.end() || slice < kTuneMinSliceMcs || n < 2) {
         run_exact(e, models, n, todo,
                   it != e->tuned_fill.end() ? kFillCandidates[it->second] : kLdsPlacementFill,
                   counters);

@@ -75,6 +75,9 @@ struct Model {
     // the same result however its MCS are split into calls
     uint64_t sublattice_key = 0;
     bool has_sublattice_key = false;
+    // attempts per second of this model's exact-replay chain in its last
+    // wave-path call (0 until measured)
+    double attempt_rate = 0.0;
 
     // the device copy is authoritative while `device_dirty` (host arrays
     // stale) — see engine.hip

@@ -2,9 +2,12 @@
 //
 //   admit:  queued requests, in arrival order, while the HBM budget and the
 //           active-request limit allow;
-//   round:  every active request advances by min(quantum / n_sites, MCS
-//           left in its current schedule step) MCS, all of them in one
-//           batched engine call per mode;
+//   round:  every active request advances by min(its share, MCS left in
+//           its current schedule step) MCS, all of them in one batched
+//           engine call per mode; the share is the MCS its exact-replay
+//           chain runs in round_s at the speed measured in its last round
+//           (quantum attempts before that, and in sublattice mode), so that
+//           no request waits for a slower one at the end of each round;
 //   finish: requests whose step completed take their trace point or
 //           readout sample; requests whose schedule is complete get their
 //           RunResult, release their device memory and wake any waiter.
@@ -97,7 +100,11 @@ void Server::round() {
     std::vector<uint64_t> todo(n);
     for (size_t i = 0; i < n; ++i) {
         Request* r = active[i];
-        const uint64_t per = std::max<uint64_t>(1, opt.quantum / std::max<uint64_t>(1, r->m->n_sites));
+        const double sites = (double)std::max<uint64_t>(1, r->m->n_sites);
+        const double share = r->mode == CPM_MODE_EXACT_REPLAY && r->m->attempt_rate > 0
+                                 ? r->m->attempt_rate * opt.round_s
+                                 : (double)opt.quantum;
+        const uint64_t per = std::max<uint64_t>(1, (uint64_t)(share / sites));
         todo[i] = std::min(per, r->sch[r->step].mcs - r->in_step);
     }
     cpm_counters cnt{};
@@ -183,6 +190,7 @@ Server* serve_create(Engine* e, const ServeOptions& o) {
     if (!s->opt.hbm_budget) s->opt.hbm_budget = (uint64_t)(0.8 * (double)device_free_bytes(e));
     if (!s->opt.max_active) s->opt.max_active = 4096;
     if (!s->opt.quantum) s->opt.quantum = 1ull << 18;
+    if (!(s->opt.round_s > 0)) s->opt.round_s = 0.03;
     s->st.hbm_budget = s->opt.hbm_budget;
     s->t0 = Clock::now();
     s->worker = std::thread([s] { s->loop(); });

@@ -49,13 +49,14 @@ def test_request_result_is_the_same_alone_and_in_a_mix(mode):
         alone = srv.submit(target, **WINDOWS, include_lattice=True, mode=mode).result(120)
     same_result(direct, alone)
 
-    # the mix: requests of other sizes and both modes, a small quantum so
-    # every request is split into many rounds, and a budget that makes
-    # later arrivals wait for memory
+    # the mix: requests of other sizes and both modes, a small quantum and
+    # round length so every request is split into many rounds (of
+    # different lengths once its speed is measured), and a budget that
+    # makes later arrivals wait for memory
     others = [(sim_of(100 + i, side=s), m) for i, (s, m) in enumerate(
         [(48, "exact_replay"), (64, "sublattice"), (32, "exact_replay"), (56, "sublattice"),
          (40, "exact_replay"), (72, "exact_replay"), (36, "sublattice"), (48, "exact_replay")])]
-    with cpm.Server(quantum_attempts=20000, hbm_budget=6 << 20) as srv:
+    with cpm.Server(quantum_attempts=20000, round_ms=0.5, hbm_budget=6 << 20) as srv:
         reqs = [srv.submit(s, **WINDOWS, mode=m) for s, m in others[:4]]
         time.sleep(0.05)  # the target arrives while the first ones run
         mine = srv.submit(target, **WINDOWS, include_lattice=True, mode=mode)
