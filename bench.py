@@ -113,9 +113,7 @@ def copy_calibration(mib: int = 256, reps: int = 4):
 
 def serving_trace(cpm, n_requests: int = 1500, rate: float = 80.0, seed: int = 7,
                   quantum: int = 1 << 17, max_active: int = 4096):
-    """The serving workload: a Poisson stream (mean `rate` requests/s, below
-    the ~95 requests/s the device sustains on this mix, so that latency is
-    service time rather than an ever-growing queue) of
+    """The serving workload: a Poisson stream (mean `rate` requests/s) of
     mixed run requests — 70 % 2D periodic lattices of side U{64..256}, 30 %
     3D periodic lattices of side U{32..40} (the LDS-resident range) — with
     2-4 cell types, V* = 40, I* = 30 (2D) / 120 (3D), lambda_V = 5,
