@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <set>
 #include <mutex>
 #include <vector>
 
@@ -2442,7 +2443,8 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
                (piped[i] ? pipe_extra_bytes(models[i]->D) : 0u);
     };
     std::vector<char> in_hbm(n, 0);
-    {
+    auto place = [&]() {
+        std::fill(in_hbm.begin(), in_hbm.end(), 0);
         std::vector<uint32_t> cand;
         double demand = 0.0;
         for (size_t i = 0; i < n; ++i) {
@@ -2463,6 +2465,23 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
             if (demand + extra > capacity) break;
             demand += extra;
             in_hbm[i] = 0;
+        }
+    };
+    place();
+    // Kernels of more instantiations than there are class streams would
+    // share streams and run one after another: then no replica is pipelined.
+    {
+        std::set<int> keys;
+        bool any_piped = false;
+        for (size_t i = 0; i < n; ++i)
+            if (todo[i] && models[i]->dev->lds) {
+                keys.insert((piped[i] ? 2000 : 0) + (in_hbm[i] ? 1000 : 0) + models[i]->D * 100 +
+                            models[i]->n_energy);
+                any_piped |= piped[i] != 0;
+            }
+        if (any_piped && keys.size() > (size_t)Engine::kClassStreams) {
+            std::fill(piped.begin(), piped.end(), 0);
+            place();
         }
     }
     // partition: global-path replicas, and wave-path replicas by kernel
