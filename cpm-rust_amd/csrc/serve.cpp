@@ -6,8 +6,9 @@
 //           its current schedule step) MCS, all of them in one batched
 //           engine call per mode; the share is the MCS its exact-replay
 //           chain runs in round_s at the speed measured in its last round
-//           (quantum attempts before that, and in sublattice mode), so that
-//           no request waits for a slower one at the end of each round;
+//           (half the mean speed of its dimension's active requests before
+//           that; quantum attempts with none, and in sublattice mode), so
+//           that no request waits for a slower one at the end of a round;
 //   finish: requests whose step completed take their trace point or
 //           readout sample; requests whose schedule is complete get their
 //           RunResult, release their device memory and wake any waiter.
@@ -98,12 +99,25 @@ void Server::round() {
     const double t_start = now();
     const size_t n = active.size();
     std::vector<uint64_t> todo(n);
+    // a request new to the device runs at half the mean speed measured for
+    // active requests of its dimension (a fresh lattice accepts more copies)
+    double rate_sum[4] = {0, 0, 0, 0};
+    int rate_n[4] = {0, 0, 0, 0};
+    for (Request* r : active)
+        if (r->mode == CPM_MODE_EXACT_REPLAY && r->m->attempt_rate > 0) {
+            rate_sum[r->m->D] += r->m->attempt_rate;
+            ++rate_n[r->m->D];
+        }
     for (size_t i = 0; i < n; ++i) {
         Request* r = active[i];
         const double sites = (double)std::max<uint64_t>(1, r->m->n_sites);
-        const double share = r->mode == CPM_MODE_EXACT_REPLAY && r->m->attempt_rate > 0
-                                 ? r->m->attempt_rate * opt.round_s
-                                 : (double)opt.quantum;
+        double share = (double)opt.quantum;
+        if (r->mode == CPM_MODE_EXACT_REPLAY) {
+            if (r->m->attempt_rate > 0)
+                share = r->m->attempt_rate * opt.round_s;
+            else if (rate_n[r->m->D])
+                share = 0.5 * rate_sum[r->m->D] / rate_n[r->m->D] * opt.round_s;
+        }
         const uint64_t per = std::max<uint64_t>(1, (uint64_t)(share / sites));
         todo[i] = std::min(per, r->sch[r->step].mcs - r->in_step);
     }
