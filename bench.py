@@ -6,9 +6,11 @@ summaries every 500 MCS).
 A step is one sampling interval (500 MCS) of every replica in the batch,
 advanced on the device by the batched exact-replay kernel with all state
 resident in HBM.  Beside it the script measures the plain
-one-replica-per-launch kernel on a 64x64 replica, the dominant kernel's
-HBM roofline fraction from its algorithmic bytes, and the CPU oracle on the
-host cores (a bounded sample of the same workload).
+one-replica-per-launch kernel on a 64x64 replica, 4096 64x64 replicas
+batched (the batched-vs-plain ratio on that shape), the dominant kernel's
+HBM and LDS roofline fractions from its algorithmic bytes (the LDS peak from
+a microbenchmark run first), single replicas, a serving trace, and the CPU
+oracle on the host cores (a bounded sample of the same workload).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--replicas N]
 """
@@ -30,19 +32,20 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 DOMINANT_KERNEL = "k_lds_engine<2, 8, *, *>"  # exact replay, 2D, Moore stencil, any sigma placement and pipelining
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_lds_engine_2d_run_batch4096_pmc.txt")
+LDS_TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_lds_engine_2d_run_batch4096_lds_pmc.txt")
 SIDES = (48, 64, 96, 128)
 INTERVAL = 500
 MASTER_SEED = 42
 
 
-def draw_batch(n: int, seed: int = 2026):
+def draw_batch(n: int, seed: int = 2026, sides=SIDES):
     """The seeded prior box of the batched workload: J entries U[1,20]
     (J[medium][medium] = 0), lambda_V U[1,20], lambda_I U[0.5,5],
-    V* U[30,80], I* U[20,60] (rounded), side uniform over SIDES."""
+    V* U[30,80], I* U[20,60] (rounded), side uniform over `sides`."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(n):
-        side = int(rng.choice(SIDES))
+        side = int(rng.choice(sides))
         j = rng.uniform(1, 20, size=6)
         J = [[0.0, j[0], j[1]], [j[0], j[2], j[3]], [j[1], j[3], j[4]]]
         lv, li = rng.uniform(1, 20, 2), rng.uniform(0.5, 5, 2)
@@ -73,19 +76,66 @@ def algorithmic_bytes(c: dict) -> float:
     return 8.0 * c["attempts"] + 28.0 * c["topology"] + 56.0 * c["energy"] + 356.0 * c["accepted"]
 
 
+def _summary_value(path: str, key: str):
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith(key + ":"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def measured_traffic():
     """HBM bytes per launch of the dominant kernel, from the committed
     rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE passes, each scaled
     by the same counter's reading on a device copy of known size); None if
     the summary is absent."""
-    try:
-        with open(TRAFFIC_SUMMARY) as f:
-            for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":", 1)[1])
-    except OSError:
-        pass
+    return _summary_value(TRAFFIC_SUMMARY, "traffic_bytes_per_launch")
+
+
+def measured_lds_traffic():
+    """LDS bytes per launch of the dominant kernel (SQ_INSTS_LDS_LOAD_ and
+    _STORE_BANDWIDTH, in 64-byte units fixed on the LDS microbenchmark),
+    from the committed rocprofv3 --pmc summary; None if it is absent."""
+    return _summary_value(LDS_TRAFFIC_SUMMARY, "lds_traffic_bytes_per_launch")
+
+
+def lds_peak_tbps():
+    """Aggregate LDS read bandwidth of this GPU in TB/s, measured by the
+    microbenchmark built beside the engine library (every thread of 8
+    workgroups per CU streaming 16-byte LDS reads); None if it is absent."""
+    import subprocess
+
+    exe = os.path.join(ROOT, "cpm-rust_amd", "cpm", "lds_bandwidth")
+    if not os.path.exists(exe):
+        return None
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120, check=True).stdout
+    for line in out.splitlines():
+        if line.startswith("lds_read_tbps:"):
+            return float(line.split(":", 1)[1])
     return None
+
+
+def batch_64(cpm, n: int, interval: int, plain_rate: float, threads: int):
+    """The batched-vs-plain comparison on its stated shape: `n` replicas of
+    2D periodic 64x64 lattices (same prior box, side fixed at 64), one
+    sampling interval warm and one timed through run_batch's engine."""
+    params = draw_batch(n, seed=2027, sides=(64,))
+    from concurrent.futures import ThreadPoolExecutor
+
+    sims = [make_sim(cpm, p, cpm.derive_seed_u64(MASTER_SEED + 1, p["index"])) for p in params]
+    with ThreadPoolExecutor(threads) as ex:
+        reps = list(ex.map(lambda s: cpm.Replica(s, 0, 10000, INTERVAL), sims))
+    cpm.Replica.run_mcs_many(reps, interval)
+    t0 = time.perf_counter()
+    c = cpm.Replica.run_mcs_many(reps, interval)
+    rate = c["attempts"] / (time.perf_counter() - t0)
+    print(f"[bench] {n} x 64x64 batched: {rate / 1e9:.2f} G attempts/s, "
+          f"{rate / plain_rate:.0f}x the plain kernel", flush=True)
+    return {"replicas": n, "interval_mcs": interval, "attempts_per_s": rate,
+            "plain_attempts_per_s": plain_rate, "batched_vs_plain": rate / plain_rate}
 
 
 def copy_calibration(mib: int = 256, reps: int = 4):
@@ -313,6 +363,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-mcs", type=int, default=20)
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-batch64", action="store_true",
+                    help="skip the 64x64 batched-vs-plain comparison")
+    ap.add_argument("--no-lds-peak", action="store_true",
+                    help="skip the LDS microbenchmark (a child process; leave it out under a "
+                         "profiler that has already opened the GPU)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device copy (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -321,6 +376,10 @@ def main():
               "index range with no exchange, which this build does not run; use --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+
+    lds_peak = None if args.no_lds_peak else lds_peak_tbps()
+    if lds_peak is not None:
+        print(f"[bench] LDS read peak (microbenchmark): {lds_peak:.1f} TB/s", flush=True)
 
     import cpm
 
@@ -373,6 +432,9 @@ def main():
     step_s = wall / args.steps
     sims_per_hour = len(reps) * 3600.0 / (t_init + 20 * step_s)
 
+    b64 = None if args.no_batch64 else batch_64(cpm, args.replicas, args.interval, plain_rate,
+                                                 min(os.cpu_count() or 1, 16))
+
     single = single_replica_rates(cpm)
     print("[bench] single replicas: " + ", ".join(
         f"{k} exact {v['exact_replay_attempts_per_s'] / 1e6:.1f} M/s, sublattice "
@@ -412,6 +474,15 @@ def main():
                      "kernel": DOMINANT_KERNEL, "launches": int(launches),
                      "avg_launch_ms": avg_launch_s * 1e3,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
+        # the same algorithmic bytes against the LDS read peak measured on
+        # this GPU: every lattice and cell table is LDS-resident in the
+        # batched kernel, so its reads and writes are LDS traffic
+        "lds_roofline": None if lds_peak is None else {
+            "bound": "lds", "achieved": achieved, "peak": lds_peak * 1e3, "unit": "GB/s",
+            "frac": achieved / (lds_peak * 1e3), "traffic": measured_lds_traffic(),
+            "kernel": DOMINANT_KERNEL,
+            "peak_source": "LDS read microbenchmark on this GPU"},
+        "batch_64x64": b64,
         "cpu_baseline": cpu,
         "serving": serving,
         "single_replica": single,

@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "cpm")
 LIB = os.path.join(OUT_DIR, "libcpm_engine.so")
+LDS_BW = os.path.join(OUT_DIR, "lds_bandwidth")
 OBJ_DIR = os.path.join(HERE, "_build")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -65,6 +66,9 @@ def build(verbose: bool = False) -> str:
     _run([HIPCC, "-shared", "--offload-arch=gfx950", "-fPIC", "-pthread", "-Wl,--no-undefined",
           *objs, "-o", tmp])
     os.replace(tmp, LIB)
+    # the LDS read-bandwidth microbenchmark (the LDS roofline's peak)
+    _run([HIPCC, "-O3", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
+          "-packed-fp32-ops", os.path.join(CSRC, "lds_bandwidth.hip"), "-o", LDS_BW])
     return LIB
 
 

@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <set>
@@ -2831,6 +2832,23 @@ static uint64_t batch_signature(Engine* e, Model* const* models, size_t n, const
     return dims | (lg << 8) | (q << 16);
 }
 
+// A fixed sigma placement fill from the environment (CPM_SIGMA_FILL, a
+// number > 0), or a negative value when the autotuner decides.  Profilers
+// that serialise kernels (counter collection) change the tuner's timings;
+// fixing the fill keeps a profiled run's launches those of an unprofiled one.
+static double fixed_fill() {
+    static const double v = [] {
+        const char* s = std::getenv("CPM_SIGMA_FILL");
+        if (!s || !*s) return -1.0;
+        char* end = nullptr;
+        const double f = std::strtod(s, &end);
+        if (end == s || *end || !(f > 0.0))
+            throw Error(CPM_ERR_ARG, "CPM_SIGMA_FILL must be a number > 0");
+        return f;
+    }();
+    return v;
+}
+
 // Exact replay with an autotuned sigma placement.  Results do not depend on
 // the placement or on how the MCS are split into launches, so the first
 // large enough call of a new batch signature is split: two slices of its
@@ -2846,6 +2864,10 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
     std::lock_guard<std::mutex> lock(e->mu);
     HIP_OK(hipSetDevice(e->device));
     for (size_t i = 0; i < n; ++i) upload(e, models[i]);
+    if (fixed_fill() > 0.0) {
+        run_exact(e, models, n, todo, fixed_fill(), counters);
+        return;
+    }
     const uint64_t sig = batch_signature(e, models, n, todo);
     auto it = e->tuned_fill.find(sig);
     uint64_t max_todo = 0;
