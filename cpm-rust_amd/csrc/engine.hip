@@ -2009,14 +2009,21 @@ constexpr uint32_t kLdsStaticReserve = 1024;
 // Replicas whose chain (MCS x sites) is at least this share of the batch's
 // longest run on the pipelined kernel.
 constexpr double kPipeShare = 0.75;
-// Share of the chip's LDS the sigma placement plans to fill (workgroups of
-// mixed sizes do not pack a CU's LDS perfectly): the default, and the
-// candidates the autotuner tries (0: sigma in HBM for every replica; 8:
-// sigma in LDS wherever it fits).
-constexpr double kLdsPlacementFill = 0.75;
-constexpr double kFillCandidates[] = {0.0, 0.5, 0.75, 1.0, 8.0};
-constexpr uint64_t kTuneMinSliceMcs = 4;  // tune only calls of at least 80 MCS (4 per slice)
-constexpr int kNumFillCandidates = sizeof(kFillCandidates) / sizeof(kFillCandidates[0]);
+// A sigma placement: the share of the chip's LDS it plans to fill
+// (workgroups of mixed sizes do not pack a CU's LDS perfectly; 0: sigma in
+// HBM for every replica, 8: in LDS wherever it fits) and which replicas get
+// LDS first — the smallest lattices (most replicas resident in LDS) or the
+// longest chains (MCS x sites: the replicas that set the call's time).
+struct Placement {
+    double fill;
+    bool longest_first;
+};
+constexpr Placement kDefaultPlacement = {0.75, false};
+// the candidates the autotuner tries
+constexpr Placement kPlacements[] = {{0.0, false},  {0.5, false}, {0.75, false}, {1.0, false},
+                                     {8.0, false},  {0.75, true}, {0.9, true},   {1.0, true}};
+constexpr int kNumPlacements = sizeof(kPlacements) / sizeof(kPlacements[0]);
+constexpr uint64_t kTuneMinSliceMcs = 4;  // tune only calls of at least 4 MCS per slice
 
 struct Engine {
     int device = 0;
@@ -2035,8 +2042,8 @@ struct Engine {
     uint32_t max_lds = 0;
     uint32_t n_cu = 0;
     std::vector<hipEvent_t> launch_ev;  // start/stop pairs around replay launches
-    // autotuned sigma placement: batch signature -> best fill candidate
-    std::map<uint64_t, int> tuned_fill;
+    // autotuned sigma placement: batch signature -> best candidate
+    std::map<uint64_t, int> tuned_placement;
     uint32_t* grid_idx = nullptr;       // grid sublattice: replica per workgroup range
     size_t grid_idx_cap = 0;
     unsigned long long* grid_counters = nullptr;  // grid sublattice: round counters
@@ -2342,15 +2349,27 @@ void sync_to_host(Model* m) {
     if (!m->dev || !m->device_dirty) return;
     const Layout L = layout_of(*m, m->dev->lds);
     const size_t n = m->n_cells, D = m->D;
-    auto get = [&](void* dst, size_t off, size_t bytes) {
-        if (bytes) HIP_OK(hipMemcpy(dst, m->dev->base + off, bytes, hipMemcpyDeviceToHost));
+    const size_t contact_bytes = (n + 1) * (n + 1) * 4;
+    // sigma through the position sums are adjacent in the allocation: one
+    // copy for them (each copy call costs more than a few KiB of traffic),
+    // and the contact matrix straight into place when it is large
+    const bool contact_apart = contact_bytes > (1u << 20);
+    const size_t end = contact_apart ? L.contact : L.contact + contact_bytes;
+    std::vector<uint8_t> stage(end - L.sigma);
+    HIP_OK(hipMemcpy(stage.data(), m->dev->base + L.sigma, stage.size(), hipMemcpyDeviceToHost));
+    auto put = [&](void* dst, size_t off, size_t bytes) {
+        if (bytes) std::memcpy(dst, stage.data() + (off - L.sigma), bytes);
     };
-    get(m->sigma.data(), L.sigma, m->n_padded * 4);
-    get(m->volume.data(), L.volume, n * 4);
-    get(m->interface.data(), L.interface, n * 4);
-    get(m->sum_pos.data(), L.sum_pos, n * D * 8);
-    get(m->sum_outer.data(), L.sum_outer, n * D * D * 8);
-    get(m->contact.data(), L.contact, (n + 1) * (n + 1) * 4);
+    put(m->sigma.data(), L.sigma, m->n_padded * 4);
+    put(m->volume.data(), L.volume, n * 4);
+    put(m->interface.data(), L.interface, n * 4);
+    put(m->sum_pos.data(), L.sum_pos, n * D * 8);
+    put(m->sum_outer.data(), L.sum_outer, n * D * D * 8);
+    if (contact_apart)
+        HIP_OK(hipMemcpy(m->contact.data(), m->dev->base + L.contact, contact_bytes,
+                         hipMemcpyDeviceToHost));
+    else
+        put(m->contact.data(), L.contact, contact_bytes);
     m->device_dirty = false;
 }
 
@@ -2395,7 +2414,7 @@ static void launch_lds_engine(int key, uint32_t lds, uint32_t count, hipStream_t
 // One exact-replay call with a given sigma placement fill (caller holds the
 // engine lock).
 static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t* todo,
-                      double fill, cpm_counters* counters) {
+                      Placement placement, cpm_counters* counters) {
     bool any3d = false;
     for (size_t i = 0; i < n; ++i) {
         upload(e, models[i]);
@@ -2406,8 +2425,9 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
     ensure_buf(e->idx_dev, e->idx_cap, n);
     std::vector<Rep> reps(n);
     // Sigma placement for this batch: a lattice too big for LDS keeps sigma
-    // in HBM; the others keep it in LDS smallest first, while the whole
-    // batch's LDS still fits the chip at once (so every replica is resident
+    // in HBM; the others keep it in LDS in the placement's order (smallest
+    // lattices or longest chains first) while the whole batch's LDS still
+    // fits the placement's share of the chip (so every replica is resident
     // from the start and the largest lattices' serial chains set the time);
     // the rest keep sigma in HBM (read through L2) and use LDS for their
     // per-cell tables only.
@@ -2456,14 +2476,25 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
             if (!m->dev->gsig && lds_need(i, false) + kLdsStaticReserve <= e->max_lds)
                 cand.push_back((uint32_t)i);
         }
-        std::stable_sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) {
-            return models[a]->dev->lds_bytes < models[b]->dev->lds_bytes;
-        });
-        const double capacity = fill * (double)e->n_cu * (double)e->max_lds;
+        if (placement.longest_first)
+            std::stable_sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) {
+                return (double)todo[a] * (double)models[a]->n_sites >
+                       (double)todo[b] * (double)models[b]->n_sites;
+            });
+        else
+            std::stable_sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) {
+                return models[a]->dev->lds_bytes < models[b]->dev->lds_bytes;
+            });
+        const double capacity = placement.fill * (double)e->n_cu * (double)e->max_lds;
         for (uint32_t i : cand) {
             const double extra =
                 (double)lds_class(e, lds_need(i, false)) - (double)lds_class(e, lds_need(i, true));
-            if (demand + extra > capacity) break;
+            if (demand + extra > capacity) {
+                // smallest first: nothing later fits either; longest first:
+                // a later, smaller lattice may still fit
+                if (placement.longest_first) continue;
+                break;
+            }
             demand += extra;
             in_hbm[i] = 0;
         }
@@ -2832,19 +2863,24 @@ static uint64_t batch_signature(Engine* e, Model* const* models, size_t n, const
     return dims | (lg << 8) | (q << 16);
 }
 
-// A fixed sigma placement fill from the environment (CPM_SIGMA_FILL, a
-// number > 0), or a negative value when the autotuner decides.  Profilers
-// that serialise kernels (counter collection) change the tuner's timings;
-// fixing the fill keeps a profiled run's launches those of an unprofiled one.
-static double fixed_fill() {
-    static const double v = [] {
-        const char* s = std::getenv("CPM_SIGMA_FILL");
-        if (!s || !*s) return -1.0;
+// A fixed sigma placement from the environment, CPM_SIGMA_PLACEMENT =
+// "smallest:<share>" or "longest:<share>" (share > 0), which skips the
+// autotuner; fill 0 when unset.  Profilers that serialise kernels (counter
+// collection) change the tuner's timings; fixing the placement keeps a
+// profiled run's launches those of an unprofiled one.
+static Placement fixed_placement() {
+    static const Placement v = [] {
+        const char* s = std::getenv("CPM_SIGMA_PLACEMENT");
+        if (!s || !*s) return Placement{0.0, false};
+        const char* colon = std::strchr(s, ':');
+        const std::string order = colon ? std::string(s, colon) : std::string();
         char* end = nullptr;
-        const double f = std::strtod(s, &end);
-        if (end == s || *end || !(f > 0.0))
-            throw Error(CPM_ERR_ARG, "CPM_SIGMA_FILL must be a number > 0");
-        return f;
+        const double f = colon ? std::strtod(colon + 1, &end) : 0.0;
+        if ((order != "smallest" && order != "longest") || end == colon + 1 || *end || !(f > 0.0))
+            throw Error(CPM_ERR_ARG,
+                        "CPM_SIGMA_PLACEMENT must be smallest:<share> or longest:<share> with "
+                        "share > 0");
+        return Placement{f, order == "longest"};
     }();
     return v;
 }
@@ -2852,7 +2888,7 @@ static double fixed_fill() {
 // Exact replay with an autotuned sigma placement.  Results do not depend on
 // the placement or on how the MCS are split into launches, so the first
 // large enough call of a new batch signature is split: two slices of its
-// MCS under each fill candidate, timed, then the rest under the fastest,
+// MCS under each placement candidate, timed, then the rest under the fastest,
 // which later calls with that signature reuse.
 void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* todo, int mode,
                   cpm_counters* counters) {
@@ -2864,35 +2900,35 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
     std::lock_guard<std::mutex> lock(e->mu);
     HIP_OK(hipSetDevice(e->device));
     for (size_t i = 0; i < n; ++i) upload(e, models[i]);
-    if (fixed_fill() > 0.0) {
-        run_exact(e, models, n, todo, fixed_fill(), counters);
+    if (fixed_placement().fill > 0.0) {
+        run_exact(e, models, n, todo, fixed_placement(), counters);
         return;
     }
     const uint64_t sig = batch_signature(e, models, n, todo);
-    auto it = e->tuned_fill.find(sig);
+    auto it = e->tuned_placement.find(sig);
     uint64_t max_todo = 0;
     for (size_t i = 0; i < n; ++i) max_todo = std::max(max_todo, todo[i]);
     // two slices per candidate, the second round in reverse order so that
     // the lattices' drift during the call (early MCS accept more) cancels
-    const uint64_t slice = max_todo / (4 * kNumFillCandidates);  // 25 % of the call in all
+    const uint64_t slice = max_todo / (4 * kNumPlacements);  // 25 % of the call in all
     // short calls (serving rounds, sampling intervals of a few MCS) are not
     // worth the extra launches: they run with the default placement
-    if (it != e->tuned_fill.end() || slice < kTuneMinSliceMcs || n < 2) {
+    if (it != e->tuned_placement.end() || slice < kTuneMinSliceMcs || n < 2) {
         run_exact(e, models, n, todo,
-                  it != e->tuned_fill.end() ? kFillCandidates[it->second] : kLdsPlacementFill,
+                  it != e->tuned_placement.end() ? kPlacements[it->second] : kDefaultPlacement,
                   counters);
         return;
     }
     std::vector<uint64_t> left(todo, todo + n), part(n);
-    double att[kNumFillCandidates] = {}, ms[kNumFillCandidates] = {};
-    for (int r = 0; r < 2 * kNumFillCandidates; ++r) {
-        const int c = r < kNumFillCandidates ? r : 2 * kNumFillCandidates - 1 - r;
+    double att[kNumPlacements] = {}, ms[kNumPlacements] = {};
+    for (int r = 0; r < 2 * kNumPlacements; ++r) {
+        const int c = r < kNumPlacements ? r : 2 * kNumPlacements - 1 - r;
         for (size_t i = 0; i < n; ++i) {
             part[i] = std::min(left[i], slice);
             left[i] -= part[i];
         }
         cpm_counters cc{};
-        run_exact(e, models, n, part.data(), kFillCandidates[c], &cc);
+        run_exact(e, models, n, part.data(), kPlacements[c], &cc);
         att[c] += (double)cc.attempts;
         ms[c] += cc.kernel_ms;
         if (counters) {
@@ -2908,12 +2944,12 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
         }
     }
     int best = 0;
-    for (int c = 1; c < kNumFillCandidates; ++c)
+    for (int c = 1; c < kNumPlacements; ++c)
         if (att[c] * ms[best] > att[best] * ms[c]) best = c;
-    e->tuned_fill[sig] = best;
+    e->tuned_placement[sig] = best;
     bool more = false;
     for (size_t i = 0; i < n; ++i) more |= left[i] != 0;
-    if (more) run_exact(e, models, n, left.data(), kFillCandidates[best], counters);
+    if (more) run_exact(e, models, n, left.data(), kPlacements[best], counters);
 }
 
 void run_mcs(Engine* e, Model* const* models, size_t n, uint64_t n_mcs, int mode,

@@ -631,12 +631,12 @@ static void label(const Model& m, Sample& s) {
         if (comps[id] > 1) s.fragmented.push_back(id);
 }
 
-Sample take_sample(const Model& m, bool include_lattice) {
+Sample take_sample(const Model& m, bool include_lattice, uint64_t mcs, double energy) {
     Sample s;
     const int D = m.D;
     const size_t n = m.n_cells;
-    s.mcs = m.mcs;
-    s.energy = m.energy;
+    s.mcs = mcs;
+    s.energy = energy;
     s.ids.resize(n);
     s.volume = m.volume;
     s.interface = m.interface;

@@ -114,7 +114,13 @@ struct Sample {
     bool has_lattice = false;
 };
 
-Sample take_sample(const Model& m, bool include_lattice);
+// A readout of the model's host copy; mcs and energy name the state it was
+// synced at (the two scalars change with every device call, the host copy
+// only when it is synced again).
+Sample take_sample(const Model& m, bool include_lattice, uint64_t mcs, double energy);
+inline Sample take_sample(const Model& m, bool include_lattice) {
+    return take_sample(m, include_lattice, m.mcs, m.energy);
+}
 bool not_equilibrated(const std::vector<double>& trace);
 
 struct Result {

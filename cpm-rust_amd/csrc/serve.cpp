@@ -9,9 +9,11 @@
 //           (half the mean speed of its dimension's active requests before
 //           that; quantum attempts with none, and in sublattice mode), so
 //           that no request waits for a slower one at the end of a round;
-//   finish: requests whose step completed take their trace point or
-//           readout sample; requests whose schedule is complete get their
-//           RunResult, release their device memory and wake any waiter.
+//   finish: requests whose step completed take their trace point; those
+//           with a readout sample or a complete schedule copy their state
+//           to the host, and finished ones release their device memory;
+//           the readouts themselves (samples, RunResult, waking waiters) run
+//           on a second host thread while the device runs the next round.
 //
 // A request's result is independent of the batch it runs in and of how
 // its MCS are split into rounds: exact replay reproduces the reference's
@@ -22,6 +24,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <future>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -64,10 +67,15 @@ struct Server {
     bool stop = false;
     ServerStats st;
     std::thread worker;
+    // the previous round's readouts, running beside the current round
+    std::future<void> readouts;
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
     void loop();
     void round();
+    void wait_readouts() {
+        if (readouts.valid()) readouts.get();  // the task reports its own failures
+    }
 };
 
 void Server::loop() {
@@ -75,7 +83,11 @@ void Server::loop() {
         {
             std::unique_lock<std::mutex> lk(mu);
             cv_work.wait(lk, [&] { return stop || !queue.empty() || !active.empty(); });
-            if (stop) return;
+            if (stop) {
+                lk.unlock();
+                wait_readouts();
+                return;
+            }
             // admission in arrival order; a request that does not fit waits
             // for memory to free up (it fits the budget on its own)
             while (!queue.empty() && active.size() < opt.max_active) {
@@ -139,6 +151,11 @@ void Server::round() {
     }
     const double t_device = now();
     std::vector<Request*> finished, sampled;
+    std::vector<uint64_t> at_mcs;
+    std::vector<double> at_energy;
+    // the readouts of the previous round read the host copies that this
+    // round's syncs overwrite, and may still append to results
+    wait_readouts();
     if (failure.empty()) {
         for (size_t i = 0; i < n; ++i) {
             Request* r = active[i];
@@ -147,21 +164,19 @@ void Server::round() {
             if (r->in_step < r->sch[r->step].mcs) continue;
             const Step& s = r->sch[r->step];
             if (s.trace) r->trace.push_back(r->m->energy);
-            if (s.sample) sampled.push_back(r);
+            if (s.sample) {
+                sampled.push_back(r);
+                at_mcs.push_back(r->m->mcs);
+                at_energy.push_back(r->m->energy);
+            }
             r->in_step = 0;
             if (++r->step == r->sch.size()) finished.push_back(r);
         }
         try {
-            parallel_for(sampled.size(), [&](size_t j) {
-                Request* r = sampled[j];
-                sync_to_host(r->m.get());
-                r->result.samples.push_back(take_sample(*r->m, r->include_lattice));
-            });
+            parallel_for(sampled.size(), [&](size_t j) { sync_to_host(sampled[j]->m.get()); });
             parallel_for(finished.size(), [&](size_t j) {
-                Request* r = finished[j];
-                sync_to_host(r->m.get());
-                finish_result(*r->m, r->trace, r->result);
-                device_release(r->m.get());
+                sync_to_host(finished[j]->m.get());
+                device_release(finished[j]->m.get());
             });
         } catch (const std::exception& ex) {
             failure = ex.what();
@@ -173,28 +188,73 @@ void Server::round() {
     st.attempts += cnt.attempts;
     st.busy_s += t_end - t_start;
     st.device_s += t_device - t_start;
-    if (!failure.empty()) finished = active;  // every active request fails
-    for (Request* r : finished) {
-        if (failure.empty()) {
-            r->info.state = CPM_REQ_DONE;
-            st.completed += 1;
-        } else {
+    if (!failure.empty()) {
+        // every active request fails
+        for (Request* r : active) {
             r->info.state = CPM_REQ_FAILED;
             r->info.error = failure;
+            r->info.done_s = t_end;
             device_release(r->m.get());
+            r->m.reset();
+            in_use -= r->info.footprint;
             st.failed += 1;
         }
-        r->info.done_s = t_end;
-        r->m.reset();
-        in_use -= r->info.footprint;
+        active.clear();
+        st.active = 0;
+        st.hbm_in_use = in_use;
+        cv_done.notify_all();
+        return;
     }
+    // finished requests leave the device now (their memory is free for the
+    // next admission) and complete when their readouts do
+    for (Request* r : finished) in_use -= r->info.footprint;
     std::vector<Request*> still;
     for (Request* r : active)
-        if (r->info.state == CPM_REQ_ACTIVE) still.push_back(r);
+        if (r->step < r->sch.size()) still.push_back(r);
     active.swap(still);
     st.active = active.size();
     st.hbm_in_use = in_use;
-    if (!finished.empty()) cv_done.notify_all();
+    if (sampled.empty() && finished.empty()) return;
+    readouts = std::async(std::launch::async, [this, sampled = std::move(sampled),
+                                               finished = std::move(finished),
+                                               at_mcs = std::move(at_mcs),
+                                               at_energy = std::move(at_energy)] {
+        std::string err;
+        try {
+            parallel_for(sampled.size(), [&](size_t j) {
+                Request* r = sampled[j];
+                r->result.samples.push_back(
+                    take_sample(*r->m, r->include_lattice, at_mcs[j], at_energy[j]));
+            });
+            parallel_for(finished.size(), [&](size_t j) {
+                Request* r = finished[j];
+                finish_result(*r->m, r->trace, r->result);
+                r->m.reset();
+            });
+        } catch (const std::exception& ex) {
+            err = ex.what();
+        }
+        std::lock_guard<std::mutex> lk2(mu);
+        const double t_done = now();
+        if (!err.empty())
+            // a failed readout fails the requests it was for; a sampled
+            // request that is still running fails when it finishes
+            for (Request* r : sampled)
+                if (r->info.state == CPM_REQ_ACTIVE && r->info.error.empty()) r->info.error = err;
+        for (Request* r : finished) {
+            if (err.empty() && r->info.error.empty()) {
+                r->info.state = CPM_REQ_DONE;
+                st.completed += 1;
+            } else {
+                r->info.state = CPM_REQ_FAILED;
+                if (r->info.error.empty()) r->info.error = err;
+                st.failed += 1;
+            }
+            r->info.done_s = t_done;
+            r->m.reset();
+        }
+        if (!finished.empty()) cv_done.notify_all();
+    });
 }
 
 Server* serve_create(Engine* e, const ServeOptions& o) {
