@@ -30,7 +30,7 @@ sys.path.insert(0, os.path.join(ROOT, "cpm-rust_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-DOMINANT_KERNEL = "k_lds_engine<2, 8, *, *>"  # exact replay, 2D, Moore stencil, any sigma placement and pipelining
+DOMINANT_KERNEL = "k_lds_engine<2, 8, *, *, false>"  # exact replay, 2D, Moore stencil, any sigma placement and pipelining, no time budget
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_lds_engine_2d_run_batch4096_pmc.txt")
 LDS_TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_lds_engine_2d_run_batch4096_lds_pmc.txt")
 SIDES = (48, 64, 96, 128)

@@ -55,10 +55,10 @@ class Server:
 
     hbm_budget: device bytes requests may hold at once (None: 80 % of the
     free device memory); max_active: requests on the device at once;
-    round_ms: target round length — an exact-replay request runs the MCS
-    its chain does in that time at the speed measured in its last round;
-    quantum_attempts: copy attempts per request per round before that
-    speed is known, and in sublattice mode.
+    round_ms: round length — every exact-replay chain runs that long on
+    the device per round (stopping mid-MCS if need be; the next round
+    resumes it), never past its next sample; quantum_attempts: copy
+    attempts per sublattice-mode request per round.
     """
 
     def __init__(self, hbm_budget: int | None = None, max_active: int = 4096,

@@ -82,6 +82,9 @@ struct Rep {
     uint64_t fill_state[4];   // sublattice mode: [0] keys its randomness
     unsigned long long cnt[6];
     uint64_t t_run;           // wave path: 100 MHz ticks the replica's chain ran in this call
+    uint64_t budget;          // wave path: 100 MHz ticks the chain may run (0: no limit, at
+                              // most 40 s); a chain out of time stops after a walked window,
+                              // mid-MCS if so
 };
 
 __constant__ uint32_t c_sp2d[8];  // 256-bit 2D simple-point table
@@ -325,6 +328,12 @@ __device__ void wave_fill(uint64_t* __restrict__ draws, uint64_t* __restrict__ s
     wsync();
 }
 
+// Whether a chain with a time budget (100 MHz ticks, 0: none) has used it.
+__device__ __forceinline__ bool out_of_time(const uint32_t& budget_lds, uint64_t t_begin) {
+    const uint32_t b = __builtin_amdgcn_readfirstlane(*(const volatile uint32_t*)&budget_lds);
+    return b && (uint32_t)(__builtin_amdgcn_s_memrealtime() - t_begin) >= b;
+}
+
 __device__ __forceinline__ uint32_t rl32(uint32_t v, uint32_t lane) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
 }
@@ -498,7 +507,7 @@ struct SigmaRef<true> {
 // accepted copy, evaluates that attempt again on every lane, commits it,
 // re-evaluates the rest of the block and goes on.  NE is the energy stencil
 // size; NT = 3^D - 1 the full shell.
-template <int D, int NE, bool SG, bool PW>
+template <int D, int NE, bool SG, bool PW, bool TB>
 __global__ void __launch_bounds__(PW ? 128 : 64) __attribute__((amdgpu_waves_per_eu(D == 2 ? (PW ? 3 : 4) : 1))) k_lds_engine(Rep* __restrict__ reps,
                                                    const uint32_t* __restrict__ idx,
                                                    const uint64_t* __restrict__ sp3d) {
@@ -578,6 +587,12 @@ __global__ void __launch_bounds__(PW ? 128 : 64) __attribute__((amdgpu_waves_per
     uint64_t todo = R.todo;
     uint32_t ain = R.attempt_in_mcs;
     uint64_t mcs = R.mcs;
+    // TB (a time-limited call): the time budget in 100 MHz ticks, at most
+    // 40 s, kept in LDS and read once per walked window, so that it holds no
+    // scalar register.  Calls without a budget run the TB = false kernels,
+    // whose loop carries no check at all.
+    __shared__ uint32_t s_budget;
+    if (TB && threadIdx.x == 0) s_budget = (uint32_t)min(R.budget, (uint64_t)4000000000u);
     double energy = R.energy;
     uint32_t c_att = 0, c_nn = 0, c_top = 0, c_en = 0, c_acc = 0, c_tie = 0;
     bool dirty = false, error = false;
@@ -1121,6 +1136,9 @@ __global__ void __launch_bounds__(PW ? 128 : 64) __attribute__((amdgpu_waves_per
             CPM_DO_WALK();
             if (walk_hit) continue;
             if (todo == 0) break;
+            if constexpr (TB) {
+                if (out_of_time(s_budget, t_begin)) break;
+            }
             if (p >= 64) {
                 CPM_SHIFT_SLOTS();
                 if (nvalid == 0) need_block = true;
@@ -1175,7 +1193,7 @@ __global__ void __launch_bounds__(PW ? 128 : 64) __attribute__((amdgpu_waves_per
                 CPM_PRODUCE(nxt, cmd == kRestart || s_nv[cur] == 0);
             } else {
                 uint32_t next_cmd = kGo, next_pos = 0;
-                bool committed = false;
+                bool committed = false, timed_out = false;
                 if (cmd != kRestart && s_nv[cur] > 0) {
                     nvalid = s_nv[cur];
                     base = s_base[cur];
@@ -1207,6 +1225,12 @@ __global__ void __launch_bounds__(PW ? 128 : 64) __attribute__((amdgpu_waves_per
                         CPM_DO_WALK();
                         if (walk_hit) continue;
                         if (todo == 0) break;
+                        if constexpr (TB) {
+                            if (out_of_time(s_budget, t_begin)) {
+                                timed_out = true;
+                                break;
+                            }
+                        }
                         if (p >= 64) {
                             CPM_SHIFT_SLOTS();
                             if (nvalid == 0) break;
@@ -1225,7 +1249,7 @@ __global__ void __launch_bounds__(PW ? 128 : 64) __attribute__((amdgpu_waves_per
                     clog_n = 0;  // nothing changed while this iteration's block was made
                     clog_all = false;
                 }
-                if (todo == 0 || error) next_cmd = kDone;
+                if (todo == 0 || error || timed_out) next_cmd = kDone;
                 if (lane == 0) {
                     s_cmd[nxt] = next_cmd;
                     s_pos[nxt] = next_pos;
@@ -1285,14 +1309,17 @@ __global__ void __launch_bounds__(PW ? 128 : 64) __attribute__((amdgpu_waves_per
     }
 }
 
-// key = 2000 * (pipelined) + 1000 * (sigma in HBM) + 100 * D + energy stencil size
-#define CPM_WAVE_KERNELS_PW(X, pw, k0)                                                 \
-    X(k0 + 204, 2, 4, false, pw) X(k0 + 208, 2, 8, false, pw)                          \
-    X(k0 + 306, 3, 6, false, pw) X(k0 + 318, 3, 18, false, pw)                         \
-    X(k0 + 326, 3, 26, false, pw) X(k0 + 1204, 2, 4, true, pw)                         \
-    X(k0 + 1208, 2, 8, true, pw) X(k0 + 1306, 3, 6, true, pw)                          \
-    X(k0 + 1318, 3, 18, true, pw) X(k0 + 1326, 3, 26, true, pw)
-#define CPM_WAVE_KERNELS(X) CPM_WAVE_KERNELS_PW(X, false, 0) CPM_WAVE_KERNELS_PW(X, true, 2000)
+// key = 4000 * (time-limited) + 2000 * (pipelined) + 1000 * (sigma in HBM) + 100 * D +
+// energy stencil size
+#define CPM_WAVE_KERNELS_PW(X, pw, tb, k0)                                             \
+    X(k0 + 204, 2, 4, false, pw, tb) X(k0 + 208, 2, 8, false, pw, tb)                  \
+    X(k0 + 306, 3, 6, false, pw, tb) X(k0 + 318, 3, 18, false, pw, tb)                 \
+    X(k0 + 326, 3, 26, false, pw, tb) X(k0 + 1204, 2, 4, true, pw, tb)                 \
+    X(k0 + 1208, 2, 8, true, pw, tb) X(k0 + 1306, 3, 6, true, pw, tb)                  \
+    X(k0 + 1318, 3, 18, true, pw, tb) X(k0 + 1326, 3, 26, true, pw, tb)
+#define CPM_WAVE_KERNELS(X)                                                              \
+    CPM_WAVE_KERNELS_PW(X, false, false, 0) CPM_WAVE_KERNELS_PW(X, true, false, 2000)  \
+    CPM_WAVE_KERNELS_PW(X, false, true, 4000) CPM_WAVE_KERNELS_PW(X, true, true, 6000)
 
 // ===========================================================================
 // Sublattice-parallel mode: halo-disjoint checkerboard colours updated in
@@ -2086,7 +2113,7 @@ Engine* engine_create(int device) {
     int n_cu = 0;
     HIP_OK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device));
     e->n_cu = (uint32_t)n_cu;
-#define CPM_FN(k, d, ne, gs, pw) (const void*)k_lds_engine<d, ne, gs, pw>,
+#define CPM_FN(k, d, ne, gs, pw, tb) (const void*)k_lds_engine<d, ne, gs, pw, tb>,
     for (auto fn : {CPM_WAVE_KERNELS(CPM_FN)(const void*)k_sublattice<2, false, kSlThreads>,
                     (const void*)k_sublattice<3, false, kSlThreads>})
 #undef CPM_FN
@@ -2401,10 +2428,12 @@ static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint
 
 static void launch_lds_engine(int key, uint32_t lds, uint32_t count, hipStream_t st, Rep* reps,
                               const uint32_t* idx, const uint64_t* sp3d) {
-    const dim3 g(count), b(key >= 2000 ? 128 : 64);
+    const dim3 g(count), b(key % 4000 >= 2000 ? 128 : 64);
     switch (key) {
-#define CPM_CASE(k, d, ne, gs, pw)                                                            \
-    case k: hipLaunchKernelGGL((k_lds_engine<d, ne, gs, pw>), g, b, lds, st, reps, idx, sp3d); break;
+#define CPM_CASE(k, d, ne, gs, pw, tb)                                                        \
+    case k:                                                                                   \
+        hipLaunchKernelGGL((k_lds_engine<d, ne, gs, pw, tb>), g, b, lds, st, reps, idx, sp3d); \
+        break;
         CPM_WAVE_KERNELS(CPM_CASE)
 #undef CPM_CASE
         default: throw Error(CPM_ERR_DEVICE, "no exact-replay kernel for this stencil");
@@ -2414,7 +2443,7 @@ static void launch_lds_engine(int key, uint32_t lds, uint32_t count, hipStream_t
 // One exact-replay call with a given sigma placement fill (caller holds the
 // engine lock).
 static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t* todo,
-                      Placement placement, cpm_counters* counters) {
+                      Placement placement, cpm_counters* counters, double budget_s = 0.0) {
     bool any3d = false;
     for (size_t i = 0; i < n; ++i) {
         upload(e, models[i]);
@@ -2452,8 +2481,11 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
         std::stable_sort(cand.begin(), cand.end(),
                          [&](uint32_t a, uint32_t b) { return work(a) > work(b); });
         const size_t simds = 4 * (size_t)e->n_cu;
+        // a time-limited call runs every chain for the same time: any of
+        // them gains from a SIMD of its own
+        const double share = budget_s > 0 ? 0.0 : kPipeShare;
         for (uint32_t i : cand) {
-            if (work(i) < kPipeShare * max_work || waves + 1 > simds) break;
+            if (work(i) < share * max_work || waves + 1 > simds) break;
             piped[i] = 1;
             ++waves;
         }
@@ -2517,7 +2549,7 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
         }
     }
     // partition: global-path replicas, and wave-path replicas by kernel
-    // instantiation (1000 * sigma in HBM + 100 * D + energy stencil size)
+    // instantiation (the key of launch_lds_engine)
     std::vector<uint32_t> order, global_idx;
     std::map<int, std::vector<uint32_t>> by_kernel;
     for (size_t i = 0; i < n; ++i) {
@@ -2527,15 +2559,20 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
         R.energy = m->energy;
         R.mcs = m->mcs;
         R.todo = todo[i];
-        R.attempt_in_mcs = 0;
+        R.attempt_in_mcs = m->attempt_in_mcs;
         R.done = todo[i] == 0;
         R.error = 0;
         for (int k = 0; k < 6; ++k) R.cnt[k] = 0;
         R.t_run = 0;
+        // 100 MHz ticks (s_memrealtime); the lane path runs whole MCS only
+        R.budget = budget_s > 0 && m->dev->lds ? std::max<uint64_t>(1, (uint64_t)(budget_s * 1e8)) : 0;
+        if (m->attempt_in_mcs && !m->dev->lds)
+            throw Error(CPM_ERR_DEVICE, "a lane-path model stopped mid-MCS");
         reps[i] = R;
         if (!todo[i]) continue;
         if (m->dev->lds)
-            by_kernel[(piped[i] ? 2000 : 0) + (in_hbm[i] ? 1000 : 0) + m->D * 100 + m->n_energy]
+            by_kernel[(budget_s > 0 ? 4000 : 0) + (piped[i] ? 2000 : 0) + (in_hbm[i] ? 1000 : 0) +
+                      m->D * 100 + m->n_energy]
                 .push_back((uint32_t)i);
         else
             global_idx.push_back((uint32_t)i);
@@ -2684,6 +2721,7 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
             for (int k = 0; k < 4; ++k) m->rng.s[k] = reps[i].rng[k];
             m->energy = reps[i].energy;
             m->mcs = reps[i].mcs;
+            m->attempt_in_mcs = reps[i].attempt_in_mcs;
             m->device_dirty = true;
             // the chain's own speed in this call (wave path), for schedulers
             if (reps[i].t_run > 0 && reps[i].cnt[0] > 0)
@@ -2891,15 +2929,24 @@ static Placement fixed_placement() {
 // MCS under each placement candidate, timed, then the rest under the fastest,
 // which later calls with that signature reuse.
 void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* todo, int mode,
-                  cpm_counters* counters) {
+                  cpm_counters* counters, double budget_s) {
+    if (mode != CPM_MODE_EXACT_REPLAY && mode != CPM_MODE_SUBLATTICE)
+        throw Error(CPM_ERR_ARG, "unknown mode");
+    for (size_t i = 0; i < n; ++i)
+        if (todo[i] && models[i]->attempt_in_mcs && mode != CPM_MODE_EXACT_REPLAY)
+            throw Error(CPM_ERR_ARG, "a model stopped mid-MCS continues in exact replay only");
     if (mode == CPM_MODE_SUBLATTICE) {
         run_sublattice(e, models, n, todo, counters);
         return;
     }
-    if (mode != CPM_MODE_EXACT_REPLAY) throw Error(CPM_ERR_ARG, "unknown mode");
     std::lock_guard<std::mutex> lock(e->mu);
     HIP_OK(hipSetDevice(e->device));
     for (size_t i = 0; i < n; ++i) upload(e, models[i]);
+    if (budget_s > 0) {  // a time-limited call: no tuning slices
+        const Placement pl = fixed_placement().fill > 0.0 ? fixed_placement() : kDefaultPlacement;
+        run_exact(e, models, n, todo, pl, counters, budget_s);
+        return;
+    }
     if (fixed_placement().fill > 0.0) {
         run_exact(e, models, n, todo, fixed_placement(), counters);
         return;

@@ -78,6 +78,10 @@ struct Model {
     // attempts per second of this model's exact-replay chain in its last
     // wave-path call (0 until measured)
     double attempt_rate = 0.0;
+    // attempts done in the MCS in progress: non-zero only between the
+    // time-limited calls of a serving loop (a chain out of time stops
+    // mid-MCS; the next call resumes the same MCS)
+    uint32_t attempt_in_mcs = 0;
 
     // the device copy is authoritative while `device_dirty` (host arrays
     // stale) — see engine.hip

@@ -2,13 +2,12 @@
 //
 //   admit:  queued requests, in arrival order, while the HBM budget and the
 //           active-request limit allow;
-//   round:  every active request advances by min(its share, MCS left in
-//           its current schedule step) MCS, all of them in one batched
-//           engine call per mode; the share is the MCS its exact-replay
-//           chain runs in round_s at the speed measured in its last round
-//           (half the mean speed of its dimension's active requests before
-//           that; quantum attempts with none, and in sublattice mode), so
-//           that no request waits for a slower one at the end of a round;
+//   round:  every active request advances to the end of its current
+//           schedule step, all of them in one batched engine call per mode;
+//           in exact replay each chain also stops after round_s on the
+//           device's clock (mid-MCS if need be; the next round resumes it),
+//           so every chain runs the whole round and none waits for a
+//           slower one; in sublattice mode a round is quantum attempts;
 //   finish: requests whose step completed take their trace point; those
 //           with a readout sample or a complete schedule copy their state
 //           to the host, and finished ones release their device memory;
@@ -110,28 +109,20 @@ void Server::loop() {
 void Server::round() {
     const double t_start = now();
     const size_t n = active.size();
-    std::vector<uint64_t> todo(n);
-    // a request new to the device runs at half the mean speed measured for
-    // active requests of its dimension (a fresh lattice accepts more copies)
-    double rate_sum[4] = {0, 0, 0, 0};
-    int rate_n[4] = {0, 0, 0, 0};
-    for (Request* r : active)
-        if (r->mode == CPM_MODE_EXACT_REPLAY && r->m->attempt_rate > 0) {
-            rate_sum[r->m->D] += r->m->attempt_rate;
-            ++rate_n[r->m->D];
-        }
+    // Exact replay: every request runs to the end of its schedule step or
+    // for round_s, whichever comes first (the chain stops on the device's
+    // clock, mid-MCS if need be).  Sublattice mode: quantum attempts.
+    std::vector<uint64_t> todo(n), mcs_before(n);
     for (size_t i = 0; i < n; ++i) {
         Request* r = active[i];
-        const double sites = (double)std::max<uint64_t>(1, r->m->n_sites);
-        double share = (double)opt.quantum;
+        const uint64_t left = r->sch[r->step].mcs - r->in_step;
+        mcs_before[i] = r->m->mcs;
         if (r->mode == CPM_MODE_EXACT_REPLAY) {
-            if (r->m->attempt_rate > 0)
-                share = r->m->attempt_rate * opt.round_s;
-            else if (rate_n[r->m->D])
-                share = 0.5 * rate_sum[r->m->D] / rate_n[r->m->D] * opt.round_s;
+            todo[i] = left;
+        } else {
+            const double sites = (double)std::max<uint64_t>(1, r->m->n_sites);
+            todo[i] = std::min(left, std::max<uint64_t>(1, (uint64_t)((double)opt.quantum / sites)));
         }
-        const uint64_t per = std::max<uint64_t>(1, (uint64_t)(share / sites));
-        todo[i] = std::min(per, r->sch[r->step].mcs - r->in_step);
     }
     cpm_counters cnt{};
     std::string failure;
@@ -144,7 +135,9 @@ void Server::round() {
                     ms.push_back(active[i]->m.get());
                     td.push_back(todo[i]);
                 }
-            if (!ms.empty()) run_mcs_each(e, ms.data(), ms.size(), td.data(), mode, &cnt);
+            if (!ms.empty())
+                run_mcs_each(e, ms.data(), ms.size(), td.data(), mode, &cnt,
+                             mode == CPM_MODE_EXACT_REPLAY ? opt.round_s : 0.0);
         }
     } catch (const std::exception& ex) {
         failure = ex.what();
@@ -159,8 +152,9 @@ void Server::round() {
     if (failure.empty()) {
         for (size_t i = 0; i < n; ++i) {
             Request* r = active[i];
-            r->in_step += todo[i];
-            r->info.mcs_done += todo[i];
+            const uint64_t done = r->m->mcs - mcs_before[i];  // whole MCS completed
+            r->in_step += done;
+            r->info.mcs_done += done;
             if (r->in_step < r->sch[r->step].mcs) continue;
             const Step& s = r->sch[r->step];
             if (s.trace) r->trace.push_back(r->m->energy);

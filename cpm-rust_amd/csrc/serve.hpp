@@ -17,8 +17,8 @@ struct Server;
 struct ServeOptions {
     uint64_t hbm_budget = 0;   // bytes of device memory requests may hold
     uint32_t max_active = 0;   // requests on the device at once
-    uint64_t quantum = 0;      // attempts per request per round until its speed is known
-    double round_s = 0;        // target length of a round once it is
+    uint64_t quantum = 0;      // attempts per sublattice-mode request per round
+    double round_s = 0;        // device time of an exact-replay chain per round
 };
 
 struct RequestInfo {

@@ -256,23 +256,23 @@ cpm_status cpm_engine_sp_tables(cpm_engine* engine, uint8_t* table2d, uint64_t* 
  * (crates/cpm-core/src/output.rs:434-529) and returns the same RunResult.
  * A server owns a worker thread that admits queued requests in arrival
  * order while their device footprint fits the HBM budget, and advances all
- * admitted requests together in rounds (continuous batching): each round
- * gives every request the whole MCS its chain runs in round_ms at its
- * measured speed (quantum_attempts attempts before that is known), never
- * past its next sample, so requests that arrive while others run join at
- * the next round and leave when done.  A request's result is the same
- * alone and in any mix: exact replay follows the reference's stream
- * whatever the chunking, and sublattice randomness is indexed by absolute
- * MCS.
+ * admitted requests together in rounds (continuous batching): in each
+ * round every exact-replay chain runs for round_ms on the device's clock
+ * (stopping mid-MCS if need be; the next round resumes it) and a
+ * sublattice-mode request for quantum_attempts attempts, never past its
+ * next sample, so requests that arrive while others run join at the next
+ * round and leave when done.  A request's result is the same alone and in
+ * any mix: exact replay follows the reference's stream whatever the
+ * chunking, and sublattice randomness is indexed by absolute MCS.
  */
 typedef struct cpm_server cpm_server;
 typedef struct {
     uint64_t hbm_budget_bytes; /* 0: 80 % of the device's free memory      */
     uint32_t max_active;       /* requests on the device at once; 0: 4096  */
-    uint64_t quantum_attempts; /* per request per round until its speed is   */
-                               /* known (and in sublattice mode); 0: 2^18   */
-    double round_ms;           /* target round length for exact replay, at  */
-                               /* each request's measured speed; 0: 30 ms   */
+    uint64_t quantum_attempts; /* per sublattice-mode request per round;    */
+                               /* 0: 2^18                                   */
+    double round_ms;           /* device time of an exact-replay chain per  */
+                               /* round; 0: 30 ms                           */
 } cpm_server_options;
 enum { CPM_REQ_QUEUED = 0, CPM_REQ_ACTIVE = 1, CPM_REQ_DONE = 2, CPM_REQ_FAILED = 3 };
 typedef struct {
