@@ -127,6 +127,21 @@ def test_batch_composition_does_not_change_results():
         assert_same_state(rep.snapshot(), ref.snapshot())
 
 
+def test_tuned_placement_slices_match_oracle():
+    """A batch call long enough for the sigma placement autotuner (128 MCS,
+    two slices per candidate placement: sigma in HBM for every replica, in
+    LDS smallest lattices first or longest chains first) equals the
+    uninterrupted reference loop for every replica."""
+    pairs = [build_case(c[1]) for c in CASES[:5]]
+    reps = [cpm.Replica(s) for s, _ in pairs]
+    print("[case] tuner batch of 5 replicas, 128 MCS", flush=True)
+    cpm.Replica.run_mcs_many(reps, 128)
+    for (sim, cfg), rep in zip(pairs, reps):
+        ref = O.Model(cfg)
+        ref.run_mcs(128)
+        assert_same_state(rep.snapshot(), ref.snapshot())
+
+
 def test_plain_launches_match_batched():
     sim, _ = build_case(CASES[0][1])
     a, b = cpm.Replica(sim), cpm.Replica(sim)
