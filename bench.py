@@ -7,7 +7,8 @@ A step is one sampling interval (500 MCS) of every replica in the batch,
 advanced on the device by the batched exact-replay kernel with all state
 resident in HBM.  Beside it the script measures the plain
 one-replica-per-launch kernel on a 64x64 replica, 4096 64x64 replicas
-batched (the batched-vs-plain ratio on that shape), the dominant kernel's
+batched (the batched-vs-plain ratio on that shape), the run_batch workload
+end to end at its full 10k MCS (measured sims/hour), the dominant kernel's
 HBM and LDS roofline fractions from its algorithmic bytes (the LDS peak from
 a microbenchmark run first), single replicas, a serving trace, and the CPU
 oracle on the host cores (a bounded sample of the same workload).
@@ -136,6 +137,26 @@ def batch_64(cpm, n: int, interval: int, plain_rate: float, threads: int):
           f"{rate / plain_rate:.0f}x the plain kernel", flush=True)
     return {"replicas": n, "interval_mcs": interval, "attempts_per_s": rate,
             "plain_attempts_per_s": plain_rate, "batched_vs_plain": rate / plain_rate}
+
+
+def run_batch_full(cpm, n: int, mcs: int, interval: int, threads: int):
+    """The run_batch workload end to end: `n` configurations of the prior box
+    (seeds derive_seed_u64(42, i)), each initialised on the host, run for
+    `mcs` MCS with a readout sample every `interval` MCS, and returned as
+    RunResults — configuration, initialisation, device run and readouts all
+    inside the timed region."""
+    params = draw_batch(n)
+    sims = [make_sim(cpm, p, cpm.derive_seed_u64(MASTER_SEED, p["index"])) for p in params]
+    cnt = {}
+    t0 = time.perf_counter()
+    results = cpm.run_batch_configs(sims, 0, mcs, interval, counters=cnt)
+    wall = time.perf_counter() - t0
+    samples = sum(len(r.samples) for r in results)
+    print(f"[bench] run_batch end to end: {n} simulations x {mcs} MCS in {wall:.1f} s "
+          f"({n * 3600.0 / wall:.0f} sims/hour)", flush=True)
+    return {"simulations": n, "mcs": mcs, "sampling_interval_mcs": interval, "wall_s": wall,
+            "sims_per_hour": n * 3600.0 / wall, "attempts_per_s": cnt["attempts"] / wall,
+            "samples": samples}
 
 
 def copy_calibration(mib: int = 256, reps: int = 4):
@@ -365,6 +386,9 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-batch64", action="store_true",
                     help="skip the 64x64 batched-vs-plain comparison")
+    ap.add_argument("--no-run-batch", action="store_true",
+                    help="skip the end-to-end run_batch (10k MCS per simulation)")
+    ap.add_argument("--run-batch-mcs", type=int, default=10000)
     ap.add_argument("--no-lds-peak", action="store_true",
                     help="skip the LDS microbenchmark (a child process; leave it out under a "
                          "profiler that has already opened the GPU)")
@@ -430,10 +454,17 @@ def main():
     # run_batch sims/hour projected from the measured step (20 intervals per
     # simulation) plus the measured host initialisation
     step_s = wall / args.steps
-    sims_per_hour = len(reps) * 3600.0 / (t_init + 20 * step_s)
+    n_reps = len(reps)
+    sims_per_hour = n_reps * 3600.0 / (t_init + 20 * step_s)
 
     b64 = None if args.no_batch64 else batch_64(cpm, args.replicas, args.interval, plain_rate,
                                                  min(os.cpu_count() or 1, 16))
+
+    full = None
+    if not args.no_run_batch:
+        del reps  # the step replicas' device memory
+        full = run_batch_full(cpm, args.replicas, args.run_batch_mcs, args.interval,
+                              min(os.cpu_count() or 1, 16))
 
     single = single_replica_rates(cpm)
     print("[bench] single replicas: " + ", ".join(
@@ -461,13 +492,14 @@ def main():
         "vs_baseline": None,
         "dtype": "int32+f64",
         "data": "synthetic",
-        "config": {"workload": f"run_batch: {len(reps)} theta-replicas, 2D periodic sides "
+        "config": {"workload": f"run_batch: {n_reps} theta-replicas, 2D periodic sides "
                                f"{{48,64,96,128}}, 2 types, phi=0.7, {args.interval} MCS per step",
-                   "replicas": len(reps), "interval_mcs": args.interval,
+                   "replicas": n_reps, "interval_mcs": args.interval,
                    "parallelism": "1 GPU, replicas batched in one launch"},
         "plain_attempts_per_s": plain_rate,
         "batched_vs_plain": value / plain_rate,
         "sims_per_hour_projected": sims_per_hour,
+        "run_batch": full,
         "counters": {k: totals[k] for k in totals},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),

@@ -8,6 +8,7 @@
 #include <memory>
 #include <string>
 #include <thread>
+#include <future>
 #include <vector>
 
 #include "../../include/cpm_engine.h"
@@ -69,23 +70,39 @@ void run_lockstep(Engine* e, std::vector<Model*>& ms, bool include_lattice, int 
     std::vector<std::vector<double>> traces(n);
     out.assign(n, Result());
     std::vector<uint64_t> todo(n);
+    // the readouts of step k run on a second host thread while the device
+    // runs step k + 1; they read the host copies synced after step k (with
+    // that step's MCS and energy), so the next sync waits for them
+    std::future<void> readouts;
     for (size_t k = 0; k < steps; ++k) {
         for (size_t i = 0; i < n; ++i) todo[i] = k < sch[i].size() ? sch[i][k].mcs : 0;
         run_mcs_each(e, ms.data(), n, todo.data(), mode, counters);
+        if (readouts.valid()) readouts.get();
         std::vector<size_t> sampled;
+        std::vector<uint64_t> at_mcs;
+        std::vector<double> at_energy;
         for (size_t i = 0; i < n; ++i) {
             if (k >= sch[i].size()) continue;
             if (sch[i][k].trace) traces[i].push_back(ms[i]->energy);
             if (sch[i][k].sample) {
-                sync_to_host(ms[i]);
                 sampled.push_back(i);
+                at_mcs.push_back(ms[i]->mcs);
+                at_energy.push_back(ms[i]->energy);
             }
         }
-        parallel_for(sampled.size(), [&](size_t j) {
-            size_t i = sampled[j];
-            out[i].samples.push_back(take_sample(*ms[i], include_lattice));
+        parallel_for(sampled.size(), [&](size_t j) { sync_to_host(ms[sampled[j]]); });
+        readouts = std::async(std::launch::async, [&out, &ms, include_lattice,
+                                                   sampled = std::move(sampled),
+                                                   at_mcs = std::move(at_mcs),
+                                                   at_energy = std::move(at_energy)] {
+            parallel_for(sampled.size(), [&](size_t j) {
+                const size_t i = sampled[j];
+                out[i].samples.push_back(
+                    take_sample(*ms[i], include_lattice, at_mcs[j], at_energy[j]));
+            });
         });
     }
+    if (readouts.valid()) readouts.get();
     parallel_for(n, [&](size_t i) {
         sync_to_host(ms[i]);
         finish_result(*ms[i], traces[i], out[i]);
