@@ -314,10 +314,13 @@ class Status:
 
 
 def _np(ptr, n, dtype, shape=None):
+    """A NumPy copy of n elements at a C pointer (one C-level copy: batch
+    results hold tens of thousands of samples, so the per-call cost counts)."""
+    dtype = np.dtype(dtype)
     if n == 0:
         out = np.zeros(0, dtype=dtype)
     else:
-        out = np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+        out = np.frombuffer(C.string_at(ptr, n * dtype.itemsize), dtype=dtype).copy()
     return out.reshape(shape) if shape is not None else out
 
 
