@@ -10,8 +10,9 @@ one-replica-per-launch kernel on a 64x64 replica, 4096 64x64 replicas
 batched (the batched-vs-plain ratio on that shape), the run_batch workload
 end to end at its full 10k MCS (measured sims/hour), the dominant kernel's
 HBM and LDS roofline fractions from its algorithmic bytes (the LDS peak from
-a microbenchmark run first), single replicas, a serving trace, and the CPU
-oracle on the host cores (a bounded sample of the same workload).
+a microbenchmark run first), single replicas, serving traces at 80 and 200
+requests/s, and the CPU oracle on the host cores (a bounded sample of the
+same workload).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--replicas N]
 """
@@ -472,10 +473,11 @@ def main():
         f"{v['sublattice_attempts_per_s'] / 1e6:.1f} M/s" for k, v in single.items()), flush=True)
     serving = None
     if not args.no_serving:
-        serving = serving_trace(cpm)
-        print(f"[bench] serving: {serving['requests']} requests in {serving['wall_s']:.1f} s, "
-              f"p50 {serving['latency_p50_s']:.3f} s, p99 {serving['latency_p99_s']:.3f} s",
-              flush=True)
+        serving = [serving_trace(cpm, rate=r) for r in (80.0, 200.0)]
+        for sv in serving:
+            print(f"[bench] serving at {sv['offered_rate_per_s']:.0f} requests/s: "
+                  f"{sv['requests']} requests in {sv['wall_s']:.1f} s, "
+                  f"p50 {sv['latency_p50_s']:.3f} s, p99 {sv['latency_p99_s']:.3f} s", flush=True)
     if args.copy_calibration:
         copy_calibration()
     cpu = None if args.no_cpu else cpu_baseline(params)
@@ -516,6 +518,8 @@ def main():
             "peak_source": "LDS read microbenchmark on this GPU"},
         "batch_64x64": b64,
         "cpu_baseline": cpu,
+        # below capacity (latency = service time) and at the configured
+        # 200 requests/s (above capacity for this mix: latency includes queueing)
         "serving": serving,
         "single_replica": single,
     }
