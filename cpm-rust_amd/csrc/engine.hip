@@ -477,7 +477,7 @@ constexpr uint32_t kStageBytes = 64 * 16;
 // LDS the pipelined kernel adds to the single-wave one: the second wave's
 // stage and two block buffers of KB slots (16 bytes per lane).
 __host__ __device__ constexpr uint32_t pipe_extra_bytes(int D) {
-    return kStageBytes + 2u * (D == 2 ? 3u : 2u) * 64u * 16u;
+    return kStageBytes + 2u * (D == 2 ? 3u : 4u) * 64u * 16u;
 }
 
 // Packed sigma words, in LDS or (G) in HBM.  The HBM copy is read through
@@ -609,7 +609,9 @@ __global__ void __launch_bounds__(PW ? 128 : 64) __attribute__((amdgpu_waves_per
     // buffer position where slot 0's window starts, p the chain position in
     // it; the decode of the window in slot k reads the 128 draws from its
     // start, halves hr[k] and hr[k + 1] of the block.
-    constexpr int KB = D == 2 ? 3 : 2;
+    // (3D: four windows amortise the block's dependent global-memory round
+    // trips — null check, neighbourhood, 2^26-bit table — over more attempts)
+    constexpr int KB = D == 2 ? 3 : 4;
     static_assert(pipe_extra_bytes(D) == kStageBytes + 2u * KB * 64u * 16u, "block buffers");
     uint4* const stage = (uint4*)(smem + Ly.total + wave * kStageBytes);  // one pass's attempts
     uint32_t base = 0, p = 0;
