@@ -184,7 +184,7 @@ def copy_calibration(mib: int = 256, reps: int = 4):
 
 
 def serving_trace(cpm, n_requests: int = 1500, rate: float = 80.0, seed: int = 7,
-                  quantum: int = 1 << 17, max_active: int = 4096):
+                  quantum: int = 1 << 17, max_active: int = 4096, round_ms: float = 30.0):
     """The serving workload: a Poisson stream (mean `rate` requests/s) of
     mixed run requests — 70 % 2D periodic lattices of side U{64..256}, 30 %
     3D periodic lattices of side U{32..40} (the LDS-resident range) — with
@@ -229,7 +229,7 @@ def serving_trace(cpm, n_requests: int = 1500, rate: float = 80.0, seed: int = 7
     with _w.catch_warnings():
         _w.simplefilter("ignore")
         sims = [build(sp) for sp in specs]
-    with cpm.Server(quantum_attempts=quantum, max_active=max_active) as srv, \
+    with cpm.Server(quantum_attempts=quantum, max_active=max_active, round_ms=round_ms) as srv, \
             ThreadPoolExecutor(16) as pool:
         t0 = time.perf_counter()
         futs = []
