@@ -149,12 +149,12 @@ void Server::round() {
     // the readouts of the previous round read the host copies that this
     // round's syncs overwrite, and may still append to results
     wait_readouts();
+    std::vector<uint64_t> done_mcs(n, 0);  // whole MCS completed, published under the lock
     if (failure.empty()) {
         for (size_t i = 0; i < n; ++i) {
             Request* r = active[i];
-            const uint64_t done = r->m->mcs - mcs_before[i];  // whole MCS completed
-            r->in_step += done;
-            r->info.mcs_done += done;
+            done_mcs[i] = r->m->mcs - mcs_before[i];
+            r->in_step += done_mcs[i];
             if (r->in_step < r->sch[r->step].mcs) continue;
             const Step& s = r->sch[r->step];
             if (s.trace) r->trace.push_back(r->m->energy);
@@ -178,6 +178,7 @@ void Server::round() {
     }
     std::lock_guard<std::mutex> lk(mu);
     const double t_end = now();
+    for (size_t i = 0; i < n; ++i) active[i]->info.mcs_done += done_mcs[i];
     st.rounds += 1;
     st.attempts += cnt.attempts;
     st.busy_s += t_end - t_start;
