@@ -177,6 +177,31 @@ def test_run_samples_match_oracle_run_with():
     assert res.metadata.derived_phi == ref.derived_phi
 
 
+def test_readme_quickstart_full_length_matches_oracle():
+    """The CPU-runnable configuration at its full length: README quickstart
+    (30x30 periodic, seed 1, 5 cells), burn-in 5000 + readout 5000 MCS,
+    a sample every 100 MCS — every sample, the trace-derived status and
+    the final hashes bit-identical to the reference loop."""
+    sim = quickstart(cpm, seed=1)
+    sim.initialize(warn_on_default_init=False)
+    print("[case] quickstart 30x30, 5000 + 5000 MCS", flush=True)
+    res = sim.run(burn_in_mcs=5000, readout_mcs=5000, sampling_interval_mcs=100)
+    cfg = quickstart_oracle(O, seed=1, windows=(5000, 5000, 100))
+    cfg.warn_on_default_init = False
+    ref = O.run_with(O.Model(cfg))
+    assert len(res.samples) == len(ref.samples) == 50
+    for s, r in zip(res.samples, ref.samples):
+        assert s.mcs == r.mcs
+        assert s.conservative_energy == r.energy
+        assert np.array_equal(s.volume, r.volume)
+        assert np.array_equal(s.interface, r.interface)
+        assert np.array_equal(s.contact, r.contact)
+        assert np.array_equal(s.centroid, r.centroid)
+    assert res.status.kind == ref.status[0]
+    assert res.metadata.model_hash == ref.model_hash
+    assert res.metadata.derived_phi == ref.derived_phi
+
+
 def test_run_batch_matches_oracle_per_replica():
     """batch.rs:110-153: theta injected per replica, seed derive_seed_u64."""
     sim = cpm.CPM(grid=(20, 20), boundary="periodic", seed=1)
