@@ -75,6 +75,10 @@ CASES = [
                              ti=100, phi=0.5, boundary="fixed"), 8),
     ("sorting3d_26", dict(kind="sort", grid=(12, 12, 12), seed=23, n_types=2, tv=60, ti=120,
                           phi=0.4, energy="twenty_six"), 8),
+    # the 2D sorting configuration: 256x256 periodic, 4 types, V*=40, I*=30,
+    # lambda 5 / 1, phi = 0.8 (1311 cells), J medium row 16 and U[2,14]
+    ("sorting256_config", dict(kind="sort", grid=(256, 256), seed=1, n_types=4, tv=40, ti=30,
+                               phi=0.8), 5),
     # lattices whose haloed u16 copy exceeds LDS keep sigma in HBM
     ("hbm_sigma_2d", dict(kind="sort", grid=(300, 296), seed=31, n_types=3, tv=60, ti=40,
                           phi=0.6), 2),
