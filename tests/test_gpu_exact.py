@@ -263,6 +263,40 @@ def test_run_batch_matches_oracle_per_replica():
         assert res.metadata.model_hash == ref.model_hash
 
 
+def test_run_batch_configs_prior_box_matches_oracle():
+    """The batched workload's own shape at small count: 16 configurations
+    from the seeded prior box (sides 48-128, 2 types, phi = 0.7, seeds
+    derive_seed_u64(42, i)) through run_batch_configs — uneven grids in one
+    batch, the placement tuner's slices, readouts overlapped with the next
+    interval — each equal to the reference loop run alone."""
+    import bench
+
+    params = bench.draw_batch(16)
+    sims, cfgs = [], []
+    for p in params:
+        seed = cpm.derive_seed_u64(42, p["index"])
+        sims.append(bench.make_sim(cpm, p, seed))
+        cfgs.append(O.Config(grid=(p["side"], p["side"]), boundary="periodic", seed=seed,
+                             cell_types=[O.CellTypeSpec("a", p["tv"], p["ti"], p["lv"][0],
+                                                        p["li"][0]),
+                                         O.CellTypeSpec("b", p["tv"], p["ti"], p["lv"][1],
+                                                        p["li"][1])],
+                             cell_counts=p["counts"], adhesion=p["J"],
+                             warn_on_default_init=False, burn_in_mcs=0, readout_mcs=400,
+                             sampling_interval_mcs=200))
+    print("[case] run_batch_configs, 16 prior-box configurations, 400 MCS", flush=True)
+    results = cpm.run_batch_configs(sims, 0, 400, 200)
+    for res, cfg in zip(results, cfgs):
+        ref = O.run_with(O.Model(cfg))
+        assert len(res.samples) == len(ref.samples) == 2
+        for s, r in zip(res.samples, ref.samples):
+            assert s.mcs == r.mcs
+            assert s.conservative_energy == r.energy
+            assert np.array_equal(s.volume, r.volume)
+            assert np.array_equal(s.contact, r.contact)
+        assert res.metadata.model_hash == ref.model_hash
+
+
 def test_device_simple_point_tables_match_the_definition():
     import ctypes as C
     from cpm import _native as N
