@@ -10,9 +10,9 @@ one-replica-per-launch kernel on a 64x64 replica, 4096 64x64 replicas
 batched (the batched-vs-plain ratio on that shape), the run_batch workload
 end to end at its full 10k MCS (measured sims/hour), the dominant kernel's
 HBM and LDS roofline fractions from its algorithmic bytes (the LDS peak from
-a microbenchmark run first), single replicas, serving traces at 80 and 200
-requests/s, and the CPU oracle on the host cores (a bounded sample of the
-same workload).
+a microbenchmark run first), single replicas, 256 seeds of the 256x256
+sorting configuration, serving traces at 80 and 200 requests/s, and the CPU
+oracle on the host cores (a bounded sample of the same workload).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--replicas N]
 """
@@ -321,6 +321,26 @@ def organoid_64(cpm, seed: int = 1):
     return sim
 
 
+def many_seeds_256(cpm, n: int = 256):
+    """The 256x256 sorting configuration over n seeds (its KS workload):
+    copy attempts/s of the whole batch in exact replay and in sublattice
+    mode (a few MCS after 2 warm ones)."""
+    import warnings as _w
+
+    with _w.catch_warnings():
+        _w.simplefilter("ignore")
+        reps = [cpm.Replica(sorting_256(cpm, seed=s)) for s in range(n)]
+    out = {"replicas": n}
+    for mode, mcs in (("exact_replay", 10), ("sublattice", 2)):
+        cpm.Replica.run_mcs_many(reps, 2, mode=mode)
+        t0 = time.perf_counter()
+        c = cpm.Replica.run_mcs_many(reps, mcs, mode=mode)
+        out[mode + "_attempts_per_s"] = c["attempts"] / (time.perf_counter() - t0)
+    print(f"[bench] {n} seeds of 256x256 sorting: exact {out['exact_replay_attempts_per_s'] / 1e9:.2f}"
+          f" G/s, sublattice {out['sublattice_attempts_per_s'] / 1e9:.3f} G/s", flush=True)
+    return out
+
+
 def single_replica_rates(cpm):
     """Copy attempts/s of one replica of each single-lattice workload, in
     exact replay and in sublattice mode (short runs after one warm MCS)."""
@@ -387,6 +407,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-batch64", action="store_true",
                     help="skip the 64x64 batched-vs-plain comparison")
+    ap.add_argument("--no-many-seeds", action="store_true",
+                    help="skip the 256-seed 256x256 sorting batch")
     ap.add_argument("--no-run-batch", action="store_true",
                     help="skip the end-to-end run_batch (10k MCS per simulation)")
     ap.add_argument("--run-batch-mcs", type=int, default=10000)
@@ -468,6 +490,7 @@ def main():
                               min(os.cpu_count() or 1, 16))
 
     single = single_replica_rates(cpm)
+    seeds = None if args.no_many_seeds else many_seeds_256(cpm)
     print("[bench] single replicas: " + ", ".join(
         f"{k} exact {v['exact_replay_attempts_per_s'] / 1e6:.1f} M/s, sublattice "
         f"{v['sublattice_attempts_per_s'] / 1e6:.1f} M/s" for k, v in single.items()), flush=True)
@@ -522,6 +545,7 @@ def main():
         # 200 requests/s (above capacity for this mix: latency includes queueing)
         "serving": serving,
         "single_replica": single,
+        "many_seeds_256x256": seeds,
     }
     print(json.dumps(line), flush=True)
 
