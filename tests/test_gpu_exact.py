@@ -88,7 +88,23 @@ CASES = [
     ("lane_path_4352_cells", dict(kind="sort", grid=(160, 160), seed=33, n_types=2, tv=5,
                                   ti=10, phi=0.85), 2),
     ("long_run_many_rounds", dict(kind="quick", grid=(30, 30), seed=9), 2000),
+    # the edges of the packed 16-bit words: 4095 cells and 15 types stay on
+    # the wave path, 4096 cells go to the lane path
+    ("wave_path_4095_cells", dict(kind="sort", grid=(160, 160), seed=34, n_types=2, tv=5,
+                                  ti=10, phi=4095 * 5 / 25600), 2),
+    ("lane_path_4096_cells", dict(kind="sort", grid=(160, 160), seed=35, n_types=2, tv=5,
+                                  ti=10, phi=4096 * 5 / 25600), 2),
+    ("wave_path_15_types", dict(kind="sort", grid=(64, 64), seed=36, n_types=15, tv=20, ti=18,
+                                phi=0.7), 10),
 ]
+
+
+def test_sixteen_cell_types_are_refused():
+    """The device J table holds (T+1)^2 <= 256 entries: a configuration with
+    16 cell types is refused at configuration time with a ValueError."""
+    with pytest.raises(ValueError, match="at most 15 cell types"):
+        sim, _ = sorting_pair(cpm, O, 37, grid=(64, 64), n_types=16, tv=20, ti=18, phi=0.7)
+        cpm.Replica(sim)
 
 
 def build_case(args):
