@@ -19,9 +19,9 @@ void run_mcs(Engine* e, Model* const* models, size_t n, uint64_t n_mcs, int mode
              cpm_counters* counters);
 // Advance models by per-model MCS counts in one launch (todo[i] may be 0).
 // With budget_s > 0 (exact replay), each wave-path chain also stops once it
-// has run budget_s seconds, possibly mid-MCS (Model::attempt_in_mcs); todo
-// counts an MCS in progress as one.  The results do not depend on where
-// calls stop.
+// has run budget_s seconds, possibly mid-MCS (Model::attempt_in_mcs), and a
+// lane-path model (more than 4095 cells) runs at most one MCS; todo counts
+// an MCS in progress as one.  The results do not depend on where calls stop.
 void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* todo, int mode,
                   cpm_counters* counters, double budget_s = 0.0);
 // One model per launch, one launch per MCS.

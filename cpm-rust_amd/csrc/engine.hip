@@ -2566,8 +2566,10 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
         R.error = 0;
         for (int k = 0; k < 6; ++k) R.cnt[k] = 0;
         R.t_run = 0;
-        // 100 MHz ticks (s_memrealtime); the lane path runs whole MCS only
+        // 100 MHz ticks (s_memrealtime); the lane path runs whole MCS only,
+        // one per time-limited call
         R.budget = budget_s > 0 && m->dev->lds ? std::max<uint64_t>(1, (uint64_t)(budget_s * 1e8)) : 0;
+        if (budget_s > 0 && !m->dev->lds) R.todo = std::min<uint64_t>(R.todo, 1);
         if (m->attempt_in_mcs && !m->dev->lds)
             throw Error(CPM_ERR_DEVICE, "a lane-path model stopped mid-MCS");
         reps[i] = R;

@@ -70,6 +70,33 @@ def test_request_result_is_the_same_alone_and_in_a_mix(mode):
     assert st["rounds"] > len(others)
 
 
+def test_lane_path_request_in_time_limited_rounds():
+    """A request with more cells than the packed words hold (4352 cells on
+    160x160, the lane path: one MCS per time-limited round) served beside
+    wave-path requests gives the same result as CPM.run."""
+    big = cpm.CPM(grid=(160, 160), boundary="periodic", seed=33)
+    big.register_cell_type("a", target_volume=5, target_interface=10, lambda_volume=5.0,
+                           lambda_interface=1.0)
+    big.register_cell_type("b", target_volume=5, target_interface=10, lambda_volume=5.0,
+                           lambda_interface=1.0)
+    big.add_cells("a", 2176)
+    big.add_cells("b", 2176)
+    big.set_adhesion([[0.0, 16.0, 16.0], [16.0, 3.0, 11.0], [16.0, 11.0, 5.0]])
+    windows = dict(burn_in_mcs=0, readout_mcs=6, sampling_interval_mcs=3)
+    direct = big.run(**windows, include_lattice=True)
+    small = [sim_of(300 + i, side=48) for i in range(3)]
+    with cpm.Server(round_ms=2.0) as srv:
+        print("[case] lane-path request among wave-path requests", flush=True)
+        reqs = [srv.submit(s, **WINDOWS) for s in small]
+        mine = srv.submit(big, **windows, include_lattice=True)
+        served = mine.result(300)
+        for r in reqs:
+            assert r.result(300).status.kind is not None
+        st = srv.stats()
+    same_result(direct, served)
+    assert st["failed"] == 0
+
+
 def test_admission_respects_the_hbm_budget_and_reports_latency():
     sims = [sim_of(7 + i, side=48) for i in range(12)]
     budget = 4 << 20
