@@ -272,7 +272,7 @@ typedef struct {
     uint64_t quantum_attempts; /* per sublattice-mode request per round;    */
                                /* 0: 2^18                                   */
     double round_ms;           /* device time of an exact-replay chain per  */
-                               /* round; 0: 30 ms                           */
+                               /* round (at most 40 s); 0: 30 ms            */
 } cpm_server_options;
 enum { CPM_REQ_QUEUED = 0, CPM_REQ_ACTIVE = 1, CPM_REQ_DONE = 2, CPM_REQ_FAILED = 3 };
 typedef struct {
