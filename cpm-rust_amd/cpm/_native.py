@@ -113,6 +113,22 @@ class CpmSampleView(C.Structure):
     ]
 
 
+class CpmSamplesShape(C.Structure):
+    _fields_ = [
+        ("n_samples", C.c_uint64), ("n_cells", C.c_uint64), ("dim", C.c_uint32),
+        ("has_lattice", C.c_int32), ("lattice_sites", C.c_uint64),
+        ("n_fragmented_total", C.c_uint64), ("n_medium_total", C.c_uint64),
+    ]
+
+
+class CpmSamplesBuffers(C.Structure):
+    _fields_ = [(name, C.c_void_p) for name in (
+        "mcs", "conservative_energy", "volume", "interface", "centroid", "eigenvalues",
+        "radius_of_gyration_sq", "kappa_sq", "contact", "fragmented_offsets",
+        "fragmented_cells", "medium_offsets", "medium_component_volumes",
+        "medium_component_touches_boundary", "lattice")]
+
+
 _lib = None
 _lib_lock = threading.Lock()
 
@@ -167,6 +183,10 @@ def lib():
         L.cpm_result_get_info.restype = st
         L.cpm_result_get_sample.argtypes = [vp, C.c_uint64, C.POINTER(CpmSampleView)]
         L.cpm_result_get_sample.restype = st
+        L.cpm_result_get_samples_shape.argtypes = [vp, C.POINTER(CpmSamplesShape)]
+        L.cpm_result_get_samples_shape.restype = st
+        L.cpm_result_copy_samples.argtypes = [vp, C.POINTER(CpmSamplesBuffers)]
+        L.cpm_result_copy_samples.restype = st
         L.cpm_result_destroy.argtypes = [vp]
         L.cpm_theta_len.argtypes = [C.c_uint32]
         L.cpm_theta_len.restype = C.c_size_t
@@ -327,6 +347,29 @@ def _np(ptr, n, dtype, shape=None):
 class Sample:
     """crates/cpm-py/src/convert.rs:53-93"""
 
+    @classmethod
+    def _of_packed(cls, k, a, grid):
+        """Sample k of the field-by-field arrays `a` (each field one NumPy
+        array over all samples; the per-sample arrays are rows of it)."""
+        self = cls.__new__(cls)
+        self.mcs = int(a["mcs"][k])
+        self.conservative_energy = float(a["energy"][k])
+        self.ids = a["ids"].copy()
+        self.volume = a["volume"][k]
+        self.interface = a["interface"][k]
+        self.centroid = a["centroid"][k]
+        self.eigenvalues = a["eigenvalues"][k]
+        self.radius_of_gyration_sq = a["rg2"][k]
+        self.kappa_sq = a["kappa"][k]
+        self.contact = a["contact"][k]
+        f0, f1 = a["frag_off"][k], a["frag_off"][k + 1]
+        self.fragmented_cells = a["frag"][f0:f1].tolist()
+        m0, m1 = a["med_off"][k], a["med_off"][k + 1]
+        self.medium_component_volumes = a["med_vol"][m0:m1].tolist()
+        self.medium_component_touches_boundary = [bool(x) for x in a["med_touch"][m0:m1]]
+        self.lattice = a["lattice"][k] if a["lattice"] is not None else None
+        return self
+
     def __init__(self, v: CpmSampleView, dim: int, grid):
         n = int(v.n_cells)
         self.mcs = int(v.mcs)
@@ -360,12 +403,31 @@ class RunResult:
         self.metadata = RunMetadata(info)
         self.status = Status(info)
         dim = int(info.dim)
-        samples = []
-        for i in range(int(info.n_samples)):
-            v = CpmSampleView()
-            check(L.cpm_result_get_sample(handle, i, C.byref(v)))
-            samples.append(Sample(v, dim, grid))
-        self.samples = samples
+        sh = CpmSamplesShape()
+        check(L.cpm_result_get_samples_shape(handle, C.byref(sh)))
+        ns, n = int(sh.n_samples), int(sh.n_cells)
+        a = {
+            "mcs": np.empty(ns, np.uint64), "energy": np.empty(ns, np.float64),
+            "ids": np.arange(1, n + 1, dtype=np.uint32),
+            "volume": np.empty((ns, n), np.uint32), "interface": np.empty((ns, n), np.uint32),
+            "centroid": np.empty((ns, n, dim)), "eigenvalues": np.empty((ns, n, dim)),
+            "rg2": np.empty((ns, n)), "kappa": np.empty((ns, n)),
+            "contact": np.empty((ns, n + 1, n + 1), np.uint32),
+            "frag_off": np.empty(ns + 1, np.uint64), "med_off": np.empty(ns + 1, np.uint64),
+            "frag": np.empty(int(sh.n_fragmented_total), np.uint32),
+            "med_vol": np.empty(int(sh.n_medium_total), np.uint32),
+            "med_touch": np.empty(int(sh.n_medium_total), np.uint8),
+            "lattice": (np.empty((ns,) + tuple(grid), np.uint32) if sh.has_lattice else None),
+        }
+        def ptr(x):
+            return x.ctypes.data if x is not None else None
+        buf = CpmSamplesBuffers(
+            ptr(a["mcs"]), ptr(a["energy"]), ptr(a["volume"]), ptr(a["interface"]),
+            ptr(a["centroid"]), ptr(a["eigenvalues"]), ptr(a["rg2"]), ptr(a["kappa"]),
+            ptr(a["contact"]), ptr(a["frag_off"]), ptr(a["frag"]), ptr(a["med_off"]),
+            ptr(a["med_vol"]), ptr(a["med_touch"]), ptr(a["lattice"]))
+        check(L.cpm_result_copy_samples(handle, C.byref(buf)))
+        self.samples = [Sample._of_packed(k, a, grid) for k in range(ns)]
         self.cell_type_index = _np(info.cell_type_index, int(info.n_cells), np.uint32)
         self.cell_type_names = list(cell_type_names)
         self.used_default = bool(info.used_default)

@@ -347,6 +347,83 @@ cpm_status cpm_result_get_sample(const cpm_result* result, uint64_t index, cpm_s
     });
 }
 
+cpm_status cpm_result_get_samples_shape(const cpm_result* result, cpm_samples_shape* o) {
+    return guarded([&] {
+        need(result && o, "result/out");
+        const Result& r = result->r;
+        *o = cpm_samples_shape{};
+        o->n_samples = r.samples.size();
+        o->n_cells = r.samples.empty() ? r.n_cells : r.samples[0].ids.size();
+        o->dim = r.dim;
+        const bool lat = !r.samples.empty() && r.samples[0].has_lattice;
+        o->has_lattice = lat ? 1 : 0;
+        o->lattice_sites = lat ? r.samples[0].lattice.size() : 0;
+        for (const Sample& s : r.samples) {
+            o->n_fragmented_total += s.fragmented.size();
+            o->n_medium_total += s.medium_volumes.size();
+        }
+    });
+}
+
+cpm_status cpm_result_copy_samples(const cpm_result* result, const cpm_samples_buffers* o) {
+    return guarded([&] {
+        need(result && o, "result/out");
+        const Result& r = result->r;
+        const size_t ns = r.samples.size();
+        if (!ns) {
+            if (o->fragmented_offsets) o->fragmented_offsets[0] = 0;
+            if (o->medium_offsets) o->medium_offsets[0] = 0;
+            return;
+        }
+        const size_t n = r.samples[0].ids.size(), D = r.dim, n1 = n + 1;
+        const bool lat = r.samples[0].has_lattice;
+        const size_t sites = lat ? r.samples[0].lattice.size() : 0;
+        need(o->mcs && o->conservative_energy && o->volume && o->interface && o->centroid &&
+                 o->eigenvalues && o->radius_of_gyration_sq && o->kappa_sq && o->contact &&
+                 o->fragmented_offsets && o->medium_offsets && (!lat || o->lattice),
+             "sample buffers");
+        uint64_t fo = 0, mo = 0;
+        for (size_t k = 0; k < ns; ++k) {
+            const Sample& s = r.samples[k];
+            need(s.ids.size() == n && s.lattice.size() == sites, "samples of one shape");
+            o->mcs[k] = s.mcs;
+            o->conservative_energy[k] = s.energy;
+            o->fragmented_offsets[k] = fo;
+            o->medium_offsets[k] = mo;
+            fo += s.fragmented.size();
+            mo += s.medium_volumes.size();
+        }
+        o->fragmented_offsets[ns] = fo;
+        o->medium_offsets[ns] = mo;
+        need((fo == 0 || o->fragmented_cells) &&
+                 (mo == 0 || (o->medium_component_volumes && o->medium_component_touches_boundary)),
+             "list buffers");
+        // the per-sample arrays, on several host cores when there is much to
+        // copy (a sample holds a dense contact matrix)
+        auto copy_one = [&](size_t k) {
+            const Sample& s = r.samples[k];
+            std::copy(s.volume.begin(), s.volume.end(), o->volume + k * n);
+            std::copy(s.interface.begin(), s.interface.end(), o->interface + k * n);
+            std::copy(s.centroid.begin(), s.centroid.end(), o->centroid + k * n * D);
+            std::copy(s.eigen.begin(), s.eigen.end(), o->eigenvalues + k * n * D);
+            std::copy(s.rg2.begin(), s.rg2.end(), o->radius_of_gyration_sq + k * n);
+            std::copy(s.kappa.begin(), s.kappa.end(), o->kappa_sq + k * n);
+            std::copy(s.contact.begin(), s.contact.end(), o->contact + k * n1 * n1);
+            std::copy(s.fragmented.begin(), s.fragmented.end(),
+                      o->fragmented_cells + o->fragmented_offsets[k]);
+            std::copy(s.medium_volumes.begin(), s.medium_volumes.end(),
+                      o->medium_component_volumes + o->medium_offsets[k]);
+            std::copy(s.medium_touch.begin(), s.medium_touch.end(),
+                      o->medium_component_touches_boundary + o->medium_offsets[k]);
+            if (lat) std::copy(s.lattice.begin(), s.lattice.end(), o->lattice + k * sites);
+        };
+        if ((double)ns * (double)(n1 * n1 * 4 + sites * 4) > (double)(16u << 20))
+            parallel_for(ns, copy_one);
+        else
+            for (size_t k = 0; k < ns; ++k) copy_one(k);
+    });
+}
+
 void cpm_result_destroy(cpm_result* result) { delete result; }
 
 size_t cpm_theta_len(uint32_t n_types) { return theta_len(n_types); }

@@ -232,9 +232,47 @@ typedef struct {
     const uint8_t* medium_component_touches_boundary;
     const uint32_t* lattice;    /* NULL unless include_lattice */
 } cpm_sample_view;
+/* Every sample of a result in one call, field by field (the same data as
+   cpm_result_get_sample, for bindings that convert many samples at once):
+   cpm_result_get_samples_shape gives the sizes, the caller allocates one
+   buffer per field and cpm_result_copy_samples fills them.  Sample k's
+   values of a per-cell field start at k * n_cells (k * n_cells * dim for
+   centroid and eigenvalues, k * (n_cells+1)^2 for contact, k *
+   lattice_sites for lattice); the variable-length lists of sample k are
+   [offsets[k], offsets[k+1]) (offsets hold n_samples + 1 entries).  Sample
+   ids are 1..n_cells.  No counterpart in the reference (its PyO3 layer
+   converts in Rust, crates/cpm-py/src/convert.rs:53-115). */
+typedef struct {
+    uint64_t n_samples;
+    uint64_t n_cells;
+    uint32_t dim;
+    int32_t has_lattice;
+    uint64_t lattice_sites;
+    uint64_t n_fragmented_total;
+    uint64_t n_medium_total;
+} cpm_samples_shape;
+typedef struct {
+    uint64_t* mcs;                          /* n_samples */
+    double* conservative_energy;            /* n_samples */
+    uint32_t* volume;                       /* n_samples x n_cells */
+    uint32_t* interface;
+    double* centroid;                       /* n_samples x n_cells x dim */
+    double* eigenvalues;
+    double* radius_of_gyration_sq;          /* n_samples x n_cells */
+    double* kappa_sq;
+    uint32_t* contact;                      /* n_samples x (n_cells+1)^2 */
+    uint64_t* fragmented_offsets;           /* n_samples + 1 */
+    uint32_t* fragmented_cells;             /* n_fragmented_total */
+    uint64_t* medium_offsets;               /* n_samples + 1 */
+    uint32_t* medium_component_volumes;     /* n_medium_total */
+    uint8_t* medium_component_touches_boundary;
+    uint32_t* lattice;                      /* n_samples x lattice_sites, or NULL */
+} cpm_samples_buffers;
 cpm_status cpm_result_get_info(const cpm_result* result, cpm_result_info* out);
 cpm_status cpm_result_get_sample(const cpm_result* result, uint64_t index,
                                  cpm_sample_view* out);
+cpm_status cpm_result_get_samples_shape(const cpm_result* result, cpm_samples_shape* out);
+cpm_status cpm_result_copy_samples(const cpm_result* result, const cpm_samples_buffers* out);
 void cpm_result_destroy(cpm_result* result);
 
 /* theta (crates/cpm-core/src/theta.rs:43-178) */
