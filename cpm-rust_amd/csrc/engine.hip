@@ -34,6 +34,7 @@
 
 #include "core.hpp"
 #include "device.hpp"
+#include "driver.hpp"
 #include "host.hpp"
 #include "rng_jump.hpp"
 
@@ -2879,12 +2880,14 @@ static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint
         }
     }
     // the running energy is recomputed from scratch: a running FP64 sum of
-    // concurrently accepted moves would depend on their order
-    for (size_t i = 0; i < n; ++i) {
-        if (!todo[i]) continue;
+    // concurrently accepted moves would depend on their order (one host
+    // thread per replica: with many replicas this sum costs more than the
+    // launch)
+    parallel_for(n, [&](size_t i) {
+        if (!todo[i]) return;
         sync_to_host(models[i]);
         models[i]->energy = models[i]->global_energy();
-    }
+    });
 }
 
 // Coarse signature of an exact-replay batch for the placement autotuner:
