@@ -2386,7 +2386,15 @@ void sync_to_host(Model* m) {
     const bool contact_apart = contact_bytes > (1u << 20);
     const size_t end = contact_apart ? L.contact : L.contact + contact_bytes;
     std::vector<uint8_t> stage(end - L.sigma);
-    HIP_OK(hipMemcpy(stage.data(), m->dev->base + L.sigma, stage.size(), hipMemcpyDeviceToHost));
+    // the calling thread's own stream: host threads syncing different
+    // replicas then do not queue behind each other on the null stream (the
+    // device work that wrote this state has completed before any sync)
+    HIP_OK(hipMemcpyAsync(stage.data(), m->dev->base + L.sigma, stage.size(),
+                          hipMemcpyDeviceToHost, hipStreamPerThread));
+    if (contact_apart)
+        HIP_OK(hipMemcpyAsync(m->contact.data(), m->dev->base + L.contact, contact_bytes,
+                              hipMemcpyDeviceToHost, hipStreamPerThread));
+    HIP_OK(hipStreamSynchronize(hipStreamPerThread));
     auto put = [&](void* dst, size_t off, size_t bytes) {
         if (bytes) std::memcpy(dst, stage.data() + (off - L.sigma), bytes);
     };
@@ -2395,11 +2403,7 @@ void sync_to_host(Model* m) {
     put(m->interface.data(), L.interface, n * 4);
     put(m->sum_pos.data(), L.sum_pos, n * D * 8);
     put(m->sum_outer.data(), L.sum_outer, n * D * D * 8);
-    if (contact_apart)
-        HIP_OK(hipMemcpy(m->contact.data(), m->dev->base + L.contact, contact_bytes,
-                         hipMemcpyDeviceToHost));
-    else
-        put(m->contact.data(), L.contact, contact_bytes);
+    if (!contact_apart) put(m->contact.data(), L.contact, contact_bytes);
     m->device_dirty = false;
 }
 
