@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -2031,6 +2032,7 @@ struct Device {
     bool gsig = false;    // sigma does not fit in LDS: always in HBM
     uint32_t lds_bytes = 0;       // LDS with sigma in LDS (when it fits)
     uint32_t lds_bytes_hbm = 0;   // LDS with sigma in HBM
+    int device = 0;               // HIP device holding `base`
 };
 
 // LDS kept free of the dynamic allocation for the kernels' static LDS
@@ -2252,6 +2254,7 @@ static void upload(Engine* e, Model* m) {
         throw Error(CPM_ERR_OOM, std::string("device allocation failed: ") + hipGetErrorString(err));
     }
     m->dev = d;
+    d->device = e->device;
     d->lds = lds;
     d->gsig = lds && gsig;
     d->lds_bytes = lds_bytes;
@@ -2375,6 +2378,32 @@ void device_release(Model* m) {
     m->dev = nullptr;
 }
 
+// Streams for state copies to the host, shared by the host threads that
+// sync replicas in parallel: on the null stream their copies would run one
+// after another, and a stream of each thread's own would be created and
+// destroyed with each short-lived worker thread.  Created on first use per
+// device and kept until the process ends.
+constexpr int kCopyStreams = 16;
+static hipStream_t copy_stream_for(int device) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static hipStream_t pool[kMaxDevices][kCopyStreams] = {};
+    static std::atomic<uint32_t> next_slot{0};
+    thread_local const uint32_t slot = next_slot.fetch_add(1) % kCopyStreams;
+    if (device < 0 || device >= kMaxDevices) throw Error(CPM_ERR_DEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lk(mu);
+    hipStream_t& st = pool[device][slot];
+    if (!st) {
+        int cur = 0;
+        HIP_OK(hipGetDevice(&cur));
+        HIP_OK(hipSetDevice(device));
+        const hipError_t err = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        HIP_OK(hipSetDevice(cur));
+        HIP_OK(err);
+    }
+    return st;
+}
+
 void sync_to_host(Model* m) {
     if (!m->dev || !m->device_dirty) return;
     const Layout L = layout_of(*m, m->dev->lds);
@@ -2386,15 +2415,14 @@ void sync_to_host(Model* m) {
     const bool contact_apart = contact_bytes > (1u << 20);
     const size_t end = contact_apart ? L.contact : L.contact + contact_bytes;
     std::vector<uint8_t> stage(end - L.sigma);
-    // the calling thread's own stream: host threads syncing different
-    // replicas then do not queue behind each other on the null stream (the
-    // device work that wrote this state has completed before any sync)
+    // the device work that wrote this state has completed before any sync
+    hipStream_t st = copy_stream_for(m->dev->device);
     HIP_OK(hipMemcpyAsync(stage.data(), m->dev->base + L.sigma, stage.size(),
-                          hipMemcpyDeviceToHost, hipStreamPerThread));
+                          hipMemcpyDeviceToHost, st));
     if (contact_apart)
         HIP_OK(hipMemcpyAsync(m->contact.data(), m->dev->base + L.contact, contact_bytes,
-                              hipMemcpyDeviceToHost, hipStreamPerThread));
-    HIP_OK(hipStreamSynchronize(hipStreamPerThread));
+                              hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
     auto put = [&](void* dst, size_t off, size_t bytes) {
         if (bytes) std::memcpy(dst, stage.data() + (off - L.sigma), bytes);
     };
