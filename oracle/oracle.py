@@ -107,6 +107,9 @@ def lib():
     L.oc_recompute.argtypes = [C.c_void_p, u32p, i64p, i64p, u32p]
     L.oc_label.argtypes = [C.c_void_p, u32p, C.c_uint64, u32p, u8p, C.c_uint64, u64p]
     L.oc_label.restype = C.c_uint64
+    L.oc_metropolis_log.argtypes = [C.c_void_p, f64p, C.c_uint64]
+    L.oc_metropolis_log.restype = C.c_uint64
+    L.oc_exp_neg.argtypes = [f64p, C.c_uint64, f64p]
     _lib = L
     return L
 
@@ -141,6 +144,15 @@ def blake3_hex(data: bytes) -> str:
     out = (C.c_uint8 * 32)()
     lib().oc_blake3(data, len(data), out)
     return bytes(out).hex()
+
+
+def exp_neg(x: np.ndarray) -> np.ndarray:
+    """exp(-x) with the platform libm, element by element (the reference's
+    f64::exp is glibc's on its CI)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty_like(x)
+    lib().oc_exp_neg(_ptr(x, C.c_double), x.size, _ptr(out, C.c_double))
+    return out
 
 
 def derive_seed_u64(master: int, index: int) -> int:
@@ -466,6 +478,17 @@ class Model:
     def model_hash(self) -> str:
         return model_hash(self.cfg, self.lattice().ravel(), self.type_index())
 
+    def record_metropolis(self, cap: int) -> None:
+        """Start recording (dH, u) of every uphill Metropolis test (up to cap)."""
+        self._mlog = np.zeros(2 * cap, dtype=np.float64)
+        lib().oc_metropolis_log(self.h, _ptr(self._mlog, C.c_double), cap)
+
+    def metropolis_draws(self) -> np.ndarray:
+        """The recorded (dH, u) pairs, shape (k, 2); stops recording."""
+        seen = int(lib().oc_metropolis_log(self.h, None, 0))
+        k = min(seen, len(self._mlog) // 2)
+        return self._mlog[: 2 * k].reshape(k, 2).copy()
+
 
 # ---------------------------------------------------------------- readouts
 
@@ -614,30 +637,40 @@ def run_with(m: Model, include_lattice: bool = False) -> RunResult:
         burned += chunk
         if burned >= start:
             trace.append(m.energy)
-    status = ("NotEquilibrated", None, None) if not_equilibrated(trace) else ("Ok", None, None)
     n_samples = max(cfg.readout_mcs // max(interval, 1), 1)
     samples = []
     for _ in range(n_samples):
         m.run_mcs(interval)
         samples.append(take_sample(m, include_lattice))
+    status = status_of(samples, trace, m.n_cells)
+    vol = m.volume()
+    return RunResult(samples, status, trace, m.model_hash(), convention_hash(cfg),
+                     float(int(vol.sum())) / float(m.n_sites))
+
+
+def status_of(samples: list, trace: list, n_cells: int) -> tuple:
+    """The status rules of run_with (output.rs:470-509, 531-559), in their
+    order: NotEquilibrated, Fragmented (first sample with a fragmented
+    cell), CellLost (a cell at volume <= 1 over the trailing half of the
+    samples), Degenerate (the last sample's contact graph joins every
+    cell); (kind, cell id, mcs)."""
+    status = ("NotEquilibrated", None, None) if not_equilibrated(trace) else ("Ok", None, None)
     if status[0] == "Ok":
         for s in samples:
             if s.fragmented:
                 status = ("Fragmented", s.fragmented[0], s.mcs)
                 break
-    if status[0] == "Ok":
+    if status[0] == "Ok" and samples:
         window = int(math.ceil(len(samples) * 0.5))
         window = min(max(window, 1), len(samples))
         trailing = samples[len(samples) - window:]
-        for i in range(m.n_cells):
+        for i in range(n_cells):
             if all(int(s.volume[i]) <= 1 for s in trailing):
                 status = ("CellLost", i + 1, trailing[0].mcs)
                 break
-    if status[0] == "Ok" and samples and _fully_connected(samples[-1].contact, m.n_cells):
+    if status[0] == "Ok" and samples and _fully_connected(samples[-1].contact, n_cells):
         status = ("Degenerate", None, None)
-    vol = m.volume()
-    return RunResult(samples, status, trace, m.model_hash(), convention_hash(cfg),
-                     float(int(vol.sum())) / float(m.n_sites))
+    return status
 
 
 def _fully_connected(contact: np.ndarray, n_cells: int) -> bool:
