@@ -11,9 +11,11 @@ package without the built library raises.
 
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 import threading
+import weakref
 
 import numpy as np
 
@@ -88,6 +90,13 @@ class CpmServerStats(C.Structure):
 class CpmModelInfo(C.Structure):
     _fields_ = [("n_sites", C.c_uint64), ("n_cells", C.c_uint64), ("mcs", C.c_uint64),
                 ("conservative_energy", C.c_double), ("rng_state", C.c_uint64 * 4)]
+
+
+class CpmModelState(C.Structure):
+    _fields_ = [("lattice", u32p), ("n_cells", C.c_uint64), ("type_index", u32p),
+                ("volume", u32p), ("interface", u32p), ("sum_position", i64p),
+                ("sum_outer", i64p), ("contact", u32p), ("has_energy", C.c_int32),
+                ("conservative_energy", C.c_double), ("rng_state", u64p), ("mcs", C.c_uint64)]
 
 
 class CpmResultInfo(C.Structure):
@@ -169,6 +178,12 @@ def lib():
         L.cpm_model_hashes.restype = st
         L.cpm_model_global_energy.argtypes = [vp, f64p]
         L.cpm_model_global_energy.restype = st
+        L.cpm_model_set_state.argtypes = [vp, C.POINTER(CpmModelState)]
+        L.cpm_model_set_state.restype = st
+        L.cpm_model_sample.argtypes = [vp, C.c_int32, C.POINTER(vp)]
+        L.cpm_model_sample.restype = st
+        L.cpm_not_equilibrated.argtypes = [f64p, C.c_size_t, C.POINTER(C.c_int32)]
+        L.cpm_not_equilibrated.restype = st
         L.cpm_run.argtypes = [vp, vp, C.c_int32, C.c_int32, C.POINTER(vp)]
         L.cpm_run.restype = st
         L.cpm_run_batch.argtypes = [vp, C.POINTER(CpmConfig), f64p, C.c_size_t, C.c_size_t,
@@ -198,6 +213,15 @@ def lib():
         L.cpm_derive_seed_u64.restype = C.c_uint64
         L.cpm_engine_sp_tables.argtypes = [vp, u8p, u64p]
         L.cpm_engine_sp_tables.restype = st
+        L.cpm_engine_set_sigma_placement.argtypes = [vp, C.c_int32, C.c_double]
+        L.cpm_engine_set_sigma_placement.restype = st
+        L.cpm_metropolis_check.argtypes = [vp, f64p, f64p, C.c_size_t, f64p, f64p, u8p]
+        L.cpm_metropolis_check.restype = st
+        L.cpm_engine_set_pipelining.argtypes = [vp, C.c_int32]
+        L.cpm_engine_set_pipelining.restype = st
+        L.cpm_engine_kernel_launches.argtypes = [vp, C.POINTER(C.c_int32), u64p, C.c_size_t,
+                                                 C.POINTER(C.c_size_t)]
+        L.cpm_engine_kernel_launches.restype = st
         L.cpm_serve_create.argtypes = [vp, C.POINTER(CpmServerOptions), C.POINTER(vp)]
         L.cpm_serve_create.restype = st
         L.cpm_serve_destroy.argtypes = [vp]
@@ -228,17 +252,75 @@ def check(status: int) -> None:
 
 _engine = None
 _engine_lock = threading.Lock()
+# servers still open, closed before the engine at interpreter exit
+_servers = weakref.WeakSet()
 
 
 def engine() -> C.c_void_p:
-    """The process-wide engine on HIP device 0 (created on first use)."""
+    """The process-wide engine on HIP device 0 (created on first use and
+    destroyed at interpreter exit, after any server still open)."""
     global _engine
     with _engine_lock:
         if _engine is None:
             h = C.c_void_p()
             check(lib().cpm_engine_create(0, C.byref(h)))
             _engine = h
+            atexit.register(_destroy_engine)
         return _engine
+
+
+def set_sigma_placement(order: str = "auto", share: float = 0.0) -> None:
+    """Fixes the exact-replay sigma placement of the process-wide engine
+    ("smallest" or "longest" first, filling `share` of the chip's LDS) or
+    returns it to the autotuner ("auto")."""
+    code = {"auto": 0, "smallest": 1, "longest": 2}[order]
+    check(lib().cpm_engine_set_sigma_placement(engine(), code, float(share)))
+
+
+def metropolis_check(delta_h, u=None, device: bool = True):
+    """(exp_libm, exp_cr, accept) of cpm_metropolis_check: the device's (or,
+    with device=False, the host's) libm exp(-dH), the correctly rounded
+    exp(-dH), and the exact-replay decision for u (None without u)."""
+    dh = np.ascontiguousarray(np.asarray(delta_h, dtype=np.float64))
+    pl = np.empty_like(dh)
+    pc = np.empty_like(dh)
+    uu = None if u is None else np.ascontiguousarray(np.asarray(u, dtype=np.float64))
+    acc = None if u is None else np.empty(dh.size, dtype=np.uint8)
+    check(lib().cpm_metropolis_check(
+        engine() if device else None, dh.ctypes.data_as(f64p),
+        None if uu is None else uu.ctypes.data_as(f64p), dh.size, pl.ctypes.data_as(f64p),
+        pc.ctypes.data_as(f64p), None if acc is None else acc.ctypes.data_as(u8p)))
+    return pl, pc, (None if acc is None else acc.astype(bool))
+
+
+def set_pipelining(on: bool) -> None:
+    """Allows (default) or forbids the two-wave pipelined exact-replay kernel."""
+    check(lib().cpm_engine_set_pipelining(engine(), 1 if on else 0))
+
+
+def kernel_launches() -> dict:
+    """{instantiation key: launches} of the exact-replay wave kernels
+    (key layout in cpm_engine_kernel_launches)."""
+    n = C.c_size_t(0)
+    check(lib().cpm_engine_kernel_launches(engine(), None, None, 0, C.byref(n)))
+    cap = max(int(n.value), 1)
+    keys = (C.c_int32 * cap)()
+    counts = (C.c_uint64 * cap)()
+    check(lib().cpm_engine_kernel_launches(engine(), keys, counts, cap, C.byref(n)))
+    return {int(keys[i]): int(counts[i]) for i in range(int(n.value))}
+
+
+def _destroy_engine() -> None:
+    """Releases the engine's streams, events and device buffers while the
+    HIP runtime is still up (models and results hold their own memory and
+    may be freed later)."""
+    global _engine
+    for srv in list(_servers):
+        srv.close()
+    with _engine_lock:
+        if _engine is not None:
+            lib().cpm_engine_destroy(_engine)
+            _engine = None
 
 
 def _arr(values, dtype):

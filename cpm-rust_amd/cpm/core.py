@@ -27,7 +27,7 @@ class Replica:
     """A validated, initialised model (CPM::new + initialize)."""
 
     def __init__(self, sim, burn_in_mcs: int = 0, readout_mcs: int = 1,
-                 sampling_interval_mcs: int = 1):
+                 sampling_interval_mcs: int = 1, initialize: bool = True):
         spec = sim._windowed(burn_in_mcs, readout_mcs, sampling_interval_mcs)
         self.spec = spec
         self.dim = len(spec["grid"])
@@ -35,6 +35,9 @@ class Replica:
         self._holder = N.ConfigHolder(spec)
         self.h = C.c_void_p()
         N.check(L.cpm_model_create(self._holder.ref(), C.byref(self.h)))
+        self.used_default = False
+        if not initialize:  # the state comes from set_state
+            return
         used = C.c_int32(0)
         try:
             N.check(L.cpm_model_initialize(self.h, C.byref(used)))
@@ -115,6 +118,56 @@ class Replica:
         out = C.c_double()
         N.check(N.lib().cpm_model_global_energy(self.h, C.byref(out)))
         return float(out.value)
+
+    def set_state(self, lattice, type_index, volume=None, interface=None, sum_position=None,
+                  sum_outer=None, contact=None, energy=None, rng=None, mcs: int = 0) -> None:
+        """Replaces the whole state by the given one (the fields of CPM<D> and
+        State<D>, crates/cpm-core/src/model.rs:15-38, state.rs:168-203);
+        omitted per-cell fields are derived from the lattice as
+        initialization does, an omitted energy by global_energy, an
+        omitted rng keeps the current stream."""
+        keep = []
+
+        def arr(x, dtype, ptype):
+            if x is None:
+                return None
+            a = np.ascontiguousarray(np.asarray(x, dtype=dtype).ravel())
+            keep.append(a)
+            return a.ctypes.data_as(ptype)
+
+        typ = np.asarray(type_index, dtype=np.uint32).ravel()
+        st = N.CpmModelState()
+        st.lattice = arr(lattice, np.uint32, N.u32p)
+        st.n_cells = len(typ)
+        st.type_index = arr(typ, np.uint32, N.u32p)
+        st.volume = arr(volume, np.uint32, N.u32p)
+        st.interface = arr(interface, np.uint32, N.u32p)
+        st.sum_position = arr(sum_position, np.int64, N.i64p)
+        st.sum_outer = arr(sum_outer, np.int64, N.i64p)
+        st.contact = arr(contact, np.uint32, N.u32p)
+        st.has_energy = 0 if energy is None else 1
+        st.conservative_energy = 0.0 if energy is None else float(energy)
+        st.rng_state = arr(rng, np.uint64, N.u64p)
+        st.mcs = int(mcs)
+        if np.asarray(lattice).size != int(np.prod(self.spec["grid"])):
+            raise ValueError("lattice must hold one id per site")
+        N.check(N.lib().cpm_model_set_state(self.h, C.byref(st)))
+
+    def sample(self, include_lattice: bool = False):
+        """take_sample of the current state (crates/cpm-core/src/
+        output.rs:289-337) as a RunResult holding that one sample, its
+        status from run_with's rules applied to it."""
+        h = C.c_void_p()
+        N.check(N.lib().cpm_model_sample(self.h, 1 if include_lattice else 0, C.byref(h)))
+        return N.RunResult(h, self.spec["grid"], [t["name"] for t in self.spec["cell_types"]])
+
+
+def not_equilibrated(trace) -> bool:
+    """crates/cpm-core/src/output.rs:388-420 through the engine library."""
+    a = np.ascontiguousarray(np.asarray(trace, dtype=np.float64))
+    out = C.c_int32(0)
+    N.check(N.lib().cpm_not_equilibrated(a.ctypes.data_as(N.f64p), len(a), C.byref(out)))
+    return bool(out.value)
 
 
 def run_batch_configs(sims, burn_in_mcs: int, readout_mcs: int, sampling_interval_mcs: int,

@@ -68,6 +68,7 @@ class Server:
         self.h = C.c_void_p()
         N.check(N.lib().cpm_serve_create(N.engine(), C.byref(opt), C.byref(self.h)))
         self._lock = threading.Lock()
+        N._servers.add(self)
 
     def close(self) -> None:
         if getattr(self, "h", None):

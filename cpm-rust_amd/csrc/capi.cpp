@@ -3,6 +3,7 @@
 // output.rs:434-529) and batch::run_batch (crates/cpm-core/src/
 // batch.rs:110-153).
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <cstring>
 #include <memory>
@@ -15,6 +16,7 @@
 #include "device.hpp"
 #include "driver.hpp"
 #include "host.hpp"
+#include "metropolis.hpp"
 #include "serve.hpp"
 
 using namespace cpm;
@@ -76,6 +78,11 @@ void run_lockstep(Engine* e, std::vector<Model*>& ms, bool include_lattice, int 
     std::future<void> readouts;
     for (size_t k = 0; k < steps; ++k) {
         for (size_t i = 0; i < n; ++i) todo[i] = k < sch[i].size() ? sch[i][k].mcs : 0;
+        // a sublattice-mode call writes the host copies of its replicas when
+        // it ends (the energy is recomputed from them), so the readouts of
+        // step k must be done before it starts; an exact-replay call leaves
+        // the host copies alone until the next sync
+        if (mode == CPM_MODE_SUBLATTICE && readouts.valid()) readouts.get();
         run_mcs_each(e, ms.data(), n, todo.data(), mode, counters);
         if (readouts.valid()) readouts.get();
         std::vector<size_t> sampled;
@@ -222,6 +229,55 @@ cpm_status cpm_model_global_energy(cpm_model* model, double* out) {
         need(model && out, "model/out");
         sync_to_host(model->m);
         *out = model->m->global_energy();
+    });
+}
+
+cpm_status cpm_model_set_state(cpm_model* model, const cpm_model_state* state) {
+    return guarded([&] {
+        need(model && state, "model/state");
+        // validated before anything changes; the device copy (if any) holds
+        // the previous state and is dropped only once the new one is in
+        Model* m = model->m;
+        Model next(m->cfg);
+        next.set_state(*state);
+        device_release(m);
+        m->n_cells = next.n_cells;
+        m->sigma.swap(next.sigma);
+        m->type_index.swap(next.type_index);
+        m->volume.swap(next.volume);
+        m->interface.swap(next.interface);
+        m->sum_pos.swap(next.sum_pos);
+        m->sum_outer.swap(next.sum_outer);
+        m->contact.swap(next.contact);
+        m->energy = next.energy;
+        m->rng = state->rng_state ? next.rng : m->rng;
+        m->mcs = next.mcs;
+        m->attempt_in_mcs = 0;
+        m->has_sublattice_key = false;
+        m->attempt_rate = 0.0;
+        m->initialized = true;
+        m->used_default = false;
+        m->device_dirty = false;
+    });
+}
+
+cpm_status cpm_model_sample(cpm_model* model, int32_t include_lattice, cpm_result** out) {
+    return guarded([&] {
+        need(model && out, "model/out");
+        Model* m = model->m;
+        need(m->initialized, "model must be initialized before cpm_model_sample");
+        sync_to_host(m);
+        Result r;
+        r.samples.push_back(take_sample(*m, include_lattice != 0));
+        finish_result(*m, {}, r);
+        *out = new cpm_result{std::move(r)};
+    });
+}
+
+cpm_status cpm_not_equilibrated(const double* trace, size_t n, int32_t* out) {
+    return guarded([&] {
+        need(out && (trace || n == 0), "trace/out");
+        *out = not_equilibrated(std::vector<double>(trace, trace + n)) ? 1 : 0;
     });
 }
 
@@ -458,6 +514,53 @@ cpm_status cpm_engine_sp_tables(cpm_engine* engine, uint8_t* table2d, uint64_t* 
     return guarded([&] {
         need(engine, "engine");
         engine_sp_tables(engine->e, table2d, table3d_words);
+    });
+}
+
+cpm_status cpm_engine_set_sigma_placement(cpm_engine* engine, int32_t order, double share) {
+    return guarded([&] {
+        need(engine, "engine");
+        engine_set_placement(engine->e, order, share);
+    });
+}
+
+cpm_status cpm_metropolis_check(cpm_engine* engine, const double* delta_h, const double* u,
+                                size_t n, double* exp_libm, double* exp_cr, uint8_t* accept) {
+    return guarded([&] {
+        need((delta_h && exp_libm && exp_cr && (!u || accept)) || n == 0,
+             "delta_h/exp_libm/exp_cr/accept");
+        if (engine) {
+            engine_metropolis_check(engine->e, delta_h, u, n, exp_libm, exp_cr, accept);
+            return;
+        }
+        for (size_t i = 0; i < n; ++i) {
+            exp_libm[i] = std::exp(-delta_h[i]);
+            exp_cr[i] = cpm::exp_cr(-delta_h[i]);
+            if (u) {
+                bool tie = false;
+                accept[i] = metropolis_accept(u[i], delta_h[i], &tie) ? 1 : 0;
+            }
+        }
+    });
+}
+
+cpm_status cpm_engine_set_pipelining(cpm_engine* engine, int32_t on) {
+    return guarded([&] {
+        need(engine, "engine");
+        engine_set_pipelining(engine->e, on != 0);
+    });
+}
+
+cpm_status cpm_engine_kernel_launches(cpm_engine* engine, int32_t* keys, uint64_t* launches,
+                                      size_t cap, size_t* n) {
+    return guarded([&] {
+        need(engine && n && (cap == 0 || (keys && launches)), "engine/keys/launches/n");
+        auto v = engine_launch_counts(engine->e);
+        *n = v.size();
+        for (size_t i = 0; i < v.size() && i < cap; ++i) {
+            keys[i] = v[i].first;
+            launches[i] = v[i].second;
+        }
     });
 }
 

@@ -3,6 +3,8 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <utility>
+#include <vector>
 
 #include "../../include/cpm_engine.h"
 
@@ -35,6 +37,18 @@ void sync_scalars(Model* m);
 void device_release(Model* m);
 
 void engine_sp_tables(Engine* e, uint8_t* t2, uint64_t* t3);
+// Fix the sigma placement of exact-replay calls (order 1: smallest
+// lattices first, 2: longest chains first, filling `share` of the chip's
+// LDS) or hand it back to the autotuner (order 0).
+void engine_set_placement(Engine* e, int order, double share);
+// exp(-dh[i]) with the device libm and correctly rounded, and (when u is
+// given) the exact-replay kernels' Metropolis decision for u[i].
+void engine_metropolis_check(Engine* e, const double* dh, const double* u, size_t n,
+                             double* p_libm, double* p_cr, uint8_t* acc);
+// Allow (default) or forbid the pipelined two-wave exact-replay kernel.
+void engine_set_pipelining(Engine* e, bool on);
+// (instantiation key, launches) of the exact-replay wave kernels so far.
+std::vector<std::pair<int, uint64_t>> engine_launch_counts(Engine* e);
 
 // Device bytes a model occupies once uploaded.
 size_t device_footprint(Engine* e, const Model& m);

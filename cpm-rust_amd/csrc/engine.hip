@@ -37,6 +37,7 @@
 #include "device.hpp"
 #include "driver.hpp"
 #include "host.hpp"
+#include "metropolis.hpp"
 #include "rng_jump.hpp"
 
 namespace cpm {
@@ -190,9 +191,9 @@ __device__ void replica_chain(Rep& R, const uint64_t* __restrict__ sp3d) {
             bool accept = dh <= 0.0;
             if (!accept) {
                 const double u = gen_f64(rng);
-                const double p = exp(-dh);
-                accept = u < p;
-                if (fabs(u - p) <= p * 0x1p-50) cnt[5]++;
+                bool tie = false;
+                accept = metropolis_accept(u, dh, &tie);
+                if (tie) cnt[5]++;
             }
             if (!accept) continue;
             cnt[4]++;
@@ -472,7 +473,7 @@ constexpr uint32_t kMu = 1u << 10;  // dH > 0 after the checks: u is drawn
 constexpr uint32_t kMa = 1u << 11;  // accepted
 constexpr uint32_t kMp = 1u << 12;  // topology checked (t not medium)
 constexpr uint32_t kMg = 1u << 13;  // passed the checks: energy evaluated
-constexpr uint32_t kMt = 1u << 14;  // u within 4 ulp of exp(-dH)
+constexpr uint32_t kMt = 1u << 14;  // u near exp(-dH): decided by the correctly rounded exp
 // LDS the wave path adds after the replica's tables: one evaluation pass's
 // 64 staged attempts (target, source, u draw).
 constexpr uint32_t kStageBytes = 64 * 16;
@@ -755,9 +756,7 @@ __global__ void __launch_bounds__(PW ? 128 : 64) __attribute__((amdgpu_waves_per
         bool acc = go, tie = false;                                                          \
         if (needu) {                                                                         \
             const double u = (double)(eu_ >> 11) * (1.0 / 9007199254740992.0);                \
-            const double pr = exp(-ev_dh);                                                   \
-            acc = u < pr;                                                                    \
-            tie = fabs(u - pr) <= pr * 0x1p-50;                                              \
+            acc = metropolis_accept(u, ev_dh, &tie);                                         \
         }                                                                                    \
         ev_f = (nn ? kMn : 0u) | (needu ? kMu : 0u) | (acc ? kMa : 0u) | (top ? kMp : 0u) |  \
                (go ? kMg : 0u) | (tie ? kMt : 0u);                                           \
@@ -2033,6 +2032,7 @@ struct Device {
     uint32_t lds_bytes = 0;       // LDS with sigma in LDS (when it fits)
     uint32_t lds_bytes_hbm = 0;   // LDS with sigma in HBM
     int device = 0;               // HIP device holding `base`
+    Engine* eng = nullptr;        // the engine whose launches write `base`
 };
 
 // LDS kept free of the dynamic allocation for the kernels' static LDS
@@ -2080,6 +2080,21 @@ struct Engine {
     size_t grid_idx_cap = 0;
     unsigned long long* grid_counters = nullptr;  // grid sublattice: round counters
     size_t grid_counters_cap = 0;
+    // Streams for state copies to the host, shared by the host threads that
+    // sync replicas in parallel: on the null stream their copies would run
+    // one after another, and a stream of each thread's own would be created
+    // and destroyed with each short-lived worker thread.  Every copy first
+    // waits for `work_done`, recorded on `stream` after each call's device
+    // work, so a copy never reads state a kernel is still writing.
+    static constexpr int kCopyStreams = 16;
+    hipStream_t copy_stream[kCopyStreams] = {};
+    hipEvent_t work_done = nullptr;
+    // a sigma placement fixed by the caller (fill 0: the autotuner decides)
+    Placement forced{0.0, false};
+    // false: no replica runs on the pipelined (two-wave) kernel
+    bool pipelining = true;
+    // launches of the exact-replay wave kernels per instantiation key
+    std::map<int, uint64_t> launches_by_key;
     std::mutex mu;
 };
 
@@ -2097,6 +2112,10 @@ Engine* engine_create(int device) {
         HIP_OK(hipStreamCreateWithFlags(&e->cls_stream[i], hipStreamNonBlocking));
         HIP_OK(hipEventCreateWithFlags(&e->cls_done[i], hipEventDisableTiming));
     }
+    for (int i = 0; i < Engine::kCopyStreams; ++i)
+        HIP_OK(hipStreamCreateWithFlags(&e->copy_stream[i], hipStreamNonBlocking));
+    HIP_OK(hipEventCreateWithFlags(&e->work_done, hipEventDisableTiming));
+    HIP_OK(hipEventRecord(e->work_done, e->stream));
     uint8_t t2[256];
     sp_table2d(t2);
     uint32_t w[8] = {0};
@@ -2170,6 +2189,9 @@ void engine_destroy(Engine* e) {
         if (e->cls_done[i]) (void)hipEventDestroy(e->cls_done[i]);
     }
     for (hipEvent_t ev : e->launch_ev) (void)hipEventDestroy(ev);
+    for (int i = 0; i < Engine::kCopyStreams; ++i)
+        if (e->copy_stream[i]) (void)hipStreamDestroy(e->copy_stream[i]);
+    if (e->work_done) (void)hipEventDestroy(e->work_done);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
@@ -2255,6 +2277,7 @@ static void upload(Engine* e, Model* m) {
     }
     m->dev = d;
     d->device = e->device;
+    d->eng = e;
     d->lds = lds;
     d->gsig = lds && gsig;
     d->lds_bytes = lds_bytes;
@@ -2378,30 +2401,12 @@ void device_release(Model* m) {
     m->dev = nullptr;
 }
 
-// Streams for state copies to the host, shared by the host threads that
-// sync replicas in parallel: on the null stream their copies would run one
-// after another, and a stream of each thread's own would be created and
-// destroyed with each short-lived worker thread.  Created on first use per
-// device and kept until the process ends.
-constexpr int kCopyStreams = 16;
-static hipStream_t copy_stream_for(int device) {
-    constexpr int kMaxDevices = 64;
-    static std::mutex mu;
-    static hipStream_t pool[kMaxDevices][kCopyStreams] = {};
+// The copy stream of the calling host thread: threads take the engine's
+// copy streams round robin.
+static hipStream_t copy_stream_for(Engine* e) {
     static std::atomic<uint32_t> next_slot{0};
-    thread_local const uint32_t slot = next_slot.fetch_add(1) % kCopyStreams;
-    if (device < 0 || device >= kMaxDevices) throw Error(CPM_ERR_DEVICE, "device index out of range");
-    std::lock_guard<std::mutex> lk(mu);
-    hipStream_t& st = pool[device][slot];
-    if (!st) {
-        int cur = 0;
-        HIP_OK(hipGetDevice(&cur));
-        HIP_OK(hipSetDevice(device));
-        const hipError_t err = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
-        HIP_OK(hipSetDevice(cur));
-        HIP_OK(err);
-    }
-    return st;
+    thread_local const uint32_t slot = next_slot.fetch_add(1) % Engine::kCopyStreams;
+    return e->copy_stream[slot];
 }
 
 void sync_to_host(Model* m) {
@@ -2415,8 +2420,8 @@ void sync_to_host(Model* m) {
     const bool contact_apart = contact_bytes > (1u << 20);
     const size_t end = contact_apart ? L.contact : L.contact + contact_bytes;
     std::vector<uint8_t> stage(end - L.sigma);
-    // the device work that wrote this state has completed before any sync
-    hipStream_t st = copy_stream_for(m->dev->device);
+    hipStream_t st = copy_stream_for(m->dev->eng);
+    HIP_OK(hipStreamWaitEvent(st, m->dev->eng->work_done, 0));
     HIP_OK(hipMemcpyAsync(stage.data(), m->dev->base + L.sigma, stage.size(),
                           hipMemcpyDeviceToHost, st));
     if (contact_apart)
@@ -2520,7 +2525,7 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
         // them gains from a SIMD of its own
         const double share = budget_s > 0 ? 0.0 : kPipeShare;
         for (uint32_t i : cand) {
-            if (work(i) < share * max_work || waves + 1 > simds) break;
+            if (!e->pipelining || work(i) < share * max_work || waves + 1 > simds) break;
             piped[i] = 1;
             ++waves;
         }
@@ -2725,6 +2730,7 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
             HIP_OK(hipEventRecord(e->launch_ev[2 * li], st));
             launch_lds_engine(l.D, l.lds, l.count, st, e->reps_dev, e->idx_dev + l.first, e->sp3d);
             HIP_OK(hipGetLastError());
+            e->launches_by_key[l.D] += 1;
             HIP_OK(hipEventRecord(e->launch_ev[2 * li + 1], st));
         }
         for (int k = 0; k < ns; ++k) {
@@ -2733,6 +2739,7 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
         }
     }
     HIP_OK(hipEventRecord(e->ev1, e->stream));
+    HIP_OK(hipEventRecord(e->work_done, e->stream));
     HIP_OK(hipMemcpyAsync(reps.data(), e->reps_dev, n * sizeof(Rep), hipMemcpyDeviceToHost,
                           e->stream));
     HIP_OK(hipStreamSynchronize(e->stream));
@@ -2898,6 +2905,7 @@ static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint
             }
         }
         HIP_OK(hipEventRecord(e->ev1, e->stream));
+        HIP_OK(hipEventRecord(e->work_done, e->stream));
         HIP_OK(hipMemcpyAsync(reps.data(), e->reps_dev, n * sizeof(Rep), hipMemcpyDeviceToHost,
                               e->stream));
         HIP_OK(hipStreamSynchronize(e->stream));
@@ -2981,13 +2989,14 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
     std::lock_guard<std::mutex> lock(e->mu);
     HIP_OK(hipSetDevice(e->device));
     for (size_t i = 0; i < n; ++i) upload(e, models[i]);
+    const Placement fixed = e->forced.fill > 0.0 ? e->forced : fixed_placement();
     if (budget_s > 0) {  // a time-limited call: no tuning slices
-        const Placement pl = fixed_placement().fill > 0.0 ? fixed_placement() : kDefaultPlacement;
-        run_exact(e, models, n, todo, pl, counters, budget_s);
+        run_exact(e, models, n, todo, fixed.fill > 0.0 ? fixed : kDefaultPlacement, counters,
+                  budget_s);
         return;
     }
-    if (fixed_placement().fill > 0.0) {
-        run_exact(e, models, n, todo, fixed_placement(), counters);
+    if (fixed.fill > 0.0) {
+        run_exact(e, models, n, todo, fixed, counters);
         return;
     }
     const uint64_t sig = batch_signature(e, models, n, todo);
@@ -3036,6 +3045,69 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
     bool more = false;
     for (size_t i = 0; i < n; ++i) more |= left[i] != 0;
     if (more) run_exact(e, models, n, left.data(), kPlacements[best], counters);
+}
+
+void engine_set_placement(Engine* e, int order, double share) {
+    std::lock_guard<std::mutex> lock(e->mu);
+    if (order == 0) {
+        e->forced = Placement{0.0, false};
+        return;
+    }
+    if ((order != 1 && order != 2) || !(share > 0.0))
+        throw Error(CPM_ERR_ARG, "sigma placement: order must be 0, 1 or 2 and share > 0");
+    e->forced = Placement{share, order == 2};
+}
+
+// exp(-dh) with the device libm, the correctly rounded exp(-dh), and the
+// Metropolis decision for u, each exactly as the exact-replay kernels
+// compute them (metropolis.hpp).
+__global__ void k_metropolis_check(const double* dh, const double* u, uint64_t n, double* p_libm,
+                                   double* p_cr, uint8_t* acc) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = dh[i];
+    p_libm[i] = exp(-x);
+    p_cr[i] = exp_cr(-x);
+    if (u) {
+        bool tie = false;
+        acc[i] = metropolis_accept(u[i], x, &tie) ? 1 : 0;
+    }
+}
+
+void engine_metropolis_check(Engine* e, const double* dh, const double* u, size_t n,
+                             double* p_libm, double* p_cr, uint8_t* acc) {
+    std::lock_guard<std::mutex> lock(e->mu);
+    HIP_OK(hipSetDevice(e->device));
+    if (!n) return;
+    const size_t bytes = n * sizeof(double);
+    char* buf = nullptr;
+    const size_t total = (u ? 4 : 3) * bytes + n;
+    HIP_OK(hipMalloc(&buf, total));
+    double* d_dh = (double*)buf;
+    double* d_pl = (double*)(buf + bytes);
+    double* d_pc = (double*)(buf + 2 * bytes);
+    double* d_u = u ? (double*)(buf + 3 * bytes) : nullptr;
+    uint8_t* d_acc = u ? (uint8_t*)(buf + 4 * bytes) : nullptr;
+    HIP_OK(hipMemcpyAsync(d_dh, dh, bytes, hipMemcpyHostToDevice, e->stream));
+    if (u) HIP_OK(hipMemcpyAsync(d_u, u, bytes, hipMemcpyHostToDevice, e->stream));
+    hipLaunchKernelGGL(k_metropolis_check, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0,
+                       e->stream, d_dh, d_u, (uint64_t)n, d_pl, d_pc, d_acc);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(p_libm, d_pl, bytes, hipMemcpyDeviceToHost, e->stream));
+    HIP_OK(hipMemcpyAsync(p_cr, d_pc, bytes, hipMemcpyDeviceToHost, e->stream));
+    if (u) HIP_OK(hipMemcpyAsync(acc, d_acc, n, hipMemcpyDeviceToHost, e->stream));
+    HIP_OK(hipStreamSynchronize(e->stream));
+    HIP_OK(hipFree(buf));
+}
+
+void engine_set_pipelining(Engine* e, bool on) {
+    std::lock_guard<std::mutex> lock(e->mu);
+    e->pipelining = on;
+}
+
+std::vector<std::pair<int, uint64_t>> engine_launch_counts(Engine* e) {
+    std::lock_guard<std::mutex> lock(e->mu);
+    return {e->launches_by_key.begin(), e->launches_by_key.end()};
 }
 
 void run_mcs(Engine* e, Model* const* models, size_t n, uint64_t n_mcs, int mode,

@@ -548,6 +548,63 @@ void Model::initialize() {
     initialized = true;
 }
 
+// The fields of CPM<D> and State<D> (model.rs:15-38, state.rs:168-203)
+// taken from the caller; derived fields that are not given are computed
+// from the lattice as initialization does (initialization.rs:396-436,
+// energy.rs:561-565).  Only what device memory safety needs is checked.
+void Model::set_state(const cpm_model_state& st) {
+    auto bad = [](const std::string& m) { throw Error(CPM_ERR_ARG, "invalid state: " + m); };
+    if (!st.lattice || !st.type_index) bad("lattice and type_index are required");
+    if (st.n_cells < 1 || st.n_cells > n_sites || st.n_cells >= 0xffffffffull)
+        bad("n_cells must be in 1..=N_sites (" + std::to_string(n_sites) + "), got " +
+            std::to_string(st.n_cells));
+    const size_t n = st.n_cells;
+    for (size_t i = 0; i < n; ++i)
+        if (st.type_index[i] >= cfg.types.size())
+            bad("cell " + std::to_string(i + 1) + " has type_index " +
+                std::to_string(st.type_index[i]) + ", but only " +
+                std::to_string(cfg.types.size()) + " types are declared");
+    for (uint64_t k = 0; k < n_sites; ++k)
+        if (st.lattice[k] > n)
+            bad("lattice references cell id " + std::to_string(st.lattice[k]) + ", but only " +
+                std::to_string(n) + " cells are given");
+    if (st.rng_state && !(st.rng_state[0] | st.rng_state[1] | st.rng_state[2] | st.rng_state[3]))
+        bad("the all-zero Xoshiro256++ state is a fixed point");
+    n_cells = n;
+    std::fill(sigma.begin(), sigma.end(), 0u);
+    size_t k = 0;
+    each_interior(*this, [&](const uint64_t* c) { sigma[flat(c)] = st.lattice[k++]; });
+    refresh_halo();
+    type_index.assign(st.type_index, st.type_index + n);
+    if (st.volume) {
+        volume.assign(st.volume, st.volume + n);
+    } else {
+        volume.assign(n, 0);
+        for (uint64_t q = 0; q < n_sites; ++q)
+            if (st.lattice[q]) volume[st.lattice[q] - 1] += 1;
+    }
+    std::vector<uint32_t> itf, ct;
+    std::vector<int64_t> sp, so;
+    recompute(itf, sp, so, ct);
+    interface = st.interface ? std::vector<uint32_t>(st.interface, st.interface + n) : itf;
+    sum_pos = st.sum_position ? std::vector<int64_t>(st.sum_position, st.sum_position + n * D)
+                              : sp;
+    sum_outer = st.sum_outer
+                    ? std::vector<int64_t>(st.sum_outer, st.sum_outer + n * D * D)
+                    : so;
+    contact = st.contact ? std::vector<uint32_t>(st.contact, st.contact + (n + 1) * (n + 1))
+                         : ct;
+    energy = st.has_energy ? st.conservative_energy : global_energy();
+    if (st.rng_state)
+        for (int q = 0; q < 4; ++q) rng.s[q] = st.rng_state[q];
+    mcs = st.mcs;
+    attempt_in_mcs = 0;
+    has_sublattice_key = false;
+    attempt_rate = 0.0;
+    initialized = true;
+    used_default = false;
+}
+
 Model::~Model() { device_release(this); }
 
 // ----------------------------------------------------------------------

@@ -97,6 +97,9 @@ struct Model {
     uint64_t wrapped(uint64_t flat) const;
     void refresh_halo();
     void initialize();  // initialization::initialize, throws Error(INIT)
+    // Replaces the state by a caller's (cpm_model_set_state); throws
+    // Error(ARG) and leaves the model unchanged on invalid input.
+    void set_state(const cpm_model_state& st);
     void recompute(std::vector<uint32_t>& itf, std::vector<int64_t>& sp,
                    std::vector<int64_t>& so, std::vector<uint32_t>& ct) const;
     double global_energy() const;

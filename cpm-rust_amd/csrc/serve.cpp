@@ -126,6 +126,14 @@ void Server::round() {
     }
     cpm_counters cnt{};
     std::string failure;
+    // a sublattice-mode call writes its replicas' host copies when it ends
+    // (its energy is recomputed there), which the previous round's readouts
+    // may still be reading: they finish first
+    for (size_t i = 0; i < n; ++i)
+        if (active[i]->mode == CPM_MODE_SUBLATTICE) {
+            wait_readouts();
+            break;
+        }
     try {
         for (int mode : {CPM_MODE_EXACT_REPLAY, CPM_MODE_SUBLATTICE}) {
             std::vector<Model*> ms;

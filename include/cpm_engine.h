@@ -169,6 +169,63 @@ cpm_status cpm_model_hashes(cpm_model* model, char convention_hash[65], char mod
 /* global_energy (crates/cpm-core/src/energy.rs:561-565) from scratch */
 cpm_status cpm_model_global_energy(cpm_model* model, double* out);
 
+/*
+ * State import: replaces the whole simulation state of a created model by
+ * the caller's, the fields of CPM<D> (rng, mcs; crates/cpm-core/src/
+ * model.rs:15-38) and State<D> (lattice, cells, volume, interface,
+ * position sums, contact, conservative_energy; state.rs:168-203).  This is
+ * what a cpm-core shim needs to hand an existing `&mut CPM<D>` to the
+ * device inside `run_mcs` (monte_carlo.rs:327) and continue its trajectory
+ * exactly: after the import the model is the caller's model, at the
+ * caller's RNG position and MCS count.
+ *
+ * lattice (n_sites, row-major, unpadded) and type_index (n_cells) are
+ * required; any other per-cell pointer may be NULL, and is then derived
+ * from the lattice as initialization does (volume by counting;
+ * interface, sum_position, sum_outer, contact by
+ * recompute_interface_and_positions, initialization.rs:396-436).
+ * sum_position / sum_outer are the unwrapped sums the reference carries
+ * (a recompute from a periodic lattice gives the wrapped coordinates).
+ * has_energy = 0 recomputes conservative_energy with global_energy
+ * (energy.rs:561-565).  rng_state NULL keeps the model's current stream.
+ * The values are taken as given — no connectivity or consistency check,
+ * as in the reference, whose fields are public — except what device
+ * memory safety needs: every lattice id <= n_cells, every type_index <
+ * n_types, 1 <= n_cells < 2^32 - 1, and a non-zero RNG state (Xoshiro's
+ * fixed point); violations return CPM_ERR_ARG and leave the model as it
+ * was.  Any device copy of the previous state is dropped.
+ */
+typedef struct {
+    const uint32_t* lattice;      /* n_sites, row-major, unpadded     */
+    uint64_t n_cells;             /* cell ids are 1..n_cells          */
+    const uint32_t* type_index;   /* n_cells                          */
+    const uint32_t* volume;       /* n_cells, or NULL                 */
+    const uint32_t* interface;    /* n_cells, or NULL                 */
+    const int64_t* sum_position;  /* n_cells * dim, or NULL           */
+    const int64_t* sum_outer;     /* n_cells * dim * dim, or NULL     */
+    const uint32_t* contact;      /* (n_cells + 1)^2, or NULL         */
+    int32_t has_energy;
+    double conservative_energy;
+    const uint64_t* rng_state;    /* Xoshiro256++ s[0..4], or NULL    */
+    uint64_t mcs;
+} cpm_model_state;
+cpm_status cpm_model_set_state(cpm_model* model, const cpm_model_state* state);
+
+/* take_sample (crates/cpm-core/src/output.rs:289-337, with
+   label_components, labelling.rs:70-154) of the model's current state, as
+   a result holding that one sample (include_lattice as in cpm_run).  Its
+   status applies run_with's rules (output.rs:470-509) to this one sample
+   with an empty burn-in trace: Fragmented if a cell is in pieces, CellLost
+   if a cell has volume <= 1, Degenerate if the contact graph joins every
+   cell, else Ok. */
+cpm_status cpm_model_sample(cpm_model* model, int32_t include_lattice, cpm_result** out);
+
+/* not_equilibrated (crates/cpm-core/src/output.rs:388-420, private in the
+   reference, where run_with applies it to the burn-in energy trace):
+   *out = 1 if the least-squares drift over the trace exceeds twice the
+   residual scatter. */
+cpm_status cpm_not_equilibrated(const double* trace, size_t n, int32_t* out);
+
 /* output::run_with (crates/cpm-core/src/output.rs:434-529): burn-in,
    equilibration check, sampled readout window, statuses, metadata.  The
    MC steps run on the device in `mode`. */
@@ -287,6 +344,45 @@ uint64_t cpm_derive_seed_u64(uint64_t master_seed, uint64_t index);
 /* Copies the engine's simple-point tables to the host (2D: 256 entries as
    bytes; 3D: 2^20 little-endian 64-bit words), for verification. */
 cpm_status cpm_engine_sp_tables(cpm_engine* engine, uint8_t* table2d, uint64_t* table3d_words);
+
+/* The Metropolis test of exact replay (crates/cpm-core/src/
+   monte_carlo.rs:147-158: accept iff u < exp(-delta_h)), for verification
+   against the reference's libm.  For each i: exp_libm[i] = exp(-delta_h[i])
+   from the device libm, exp_cr[i] = exp(-delta_h[i]) correctly rounded
+   (double-double evaluation; results >= 2^-1022), and, when u is not
+   NULL, accept[i] = the kernels' decision for u[i]: the libm value decides
+   unless u[i] lies within 2^-40 (relative) of it, and there the correctly
+   rounded one does.  engine NULL computes the same on the host (host
+   libm).  Any output may be NULL only together with u for accept. */
+cpm_status cpm_metropolis_check(cpm_engine* engine, const double* delta_h, const double* u,
+                                size_t n, double* exp_libm, double* exp_cr, uint8_t* accept);
+
+/* Exact-replay sigma placement (no counterpart in the reference, whose
+   lattices live in host memory).  A batch keeps each lattice's sigma in
+   LDS or in HBM; by default an autotuner picks per batch signature.
+   order 1 fixes "smallest lattices first", 2 "longest chains first", each
+   filling at most `share` of the chip's LDS (a tiny share puts every
+   sigma in HBM); order 0 returns the choice to the autotuner.  Results do
+   not depend on the placement. */
+enum { CPM_PLACEMENT_AUTO = 0, CPM_PLACEMENT_SMALLEST = 1, CPM_PLACEMENT_LONGEST = 2 };
+cpm_status cpm_engine_set_sigma_placement(cpm_engine* engine, int32_t order, double share);
+
+/* Exact-replay pipelining (no counterpart in the reference): by default
+   the replicas with the longest chains run on a two-wave kernel (one wave
+   decodes and evaluates ahead while the other walks the chain) while the
+   batch leaves SIMDs free; on = 0 runs every replica on one wave.
+   Results do not depend on it. */
+cpm_status cpm_engine_set_pipelining(cpm_engine* engine, int32_t on);
+
+/* How often each exact-replay wave-kernel variant has been launched by
+   this engine: writes up to cap (key, launches) pairs and sets *n to the
+   number of variants launched so far.  Key = variant * 2000 + base, where
+   variant is 0 (plain), 1 (pipelined: two waves per replica), 2
+   (time-limited) or 3 (pipelined and time-limited), and base is D * 100 +
+   the energy-stencil size (204, 208, 306, 318, 326), plus 1000 when sigma
+   is read from HBM instead of LDS. */
+cpm_status cpm_engine_kernel_launches(cpm_engine* engine, int32_t* keys, uint64_t* launches,
+                                      size_t cap, size_t* n);
 
 /*
  * Serving.  No counterpart in the reference, which runs one configuration

@@ -436,6 +436,9 @@ struct ocpm {
     uint64_t rng[4];
     uint64_t mcs;
     uint64_t counters[5];
+    /* optional record of the Metropolis draws: (dH, u) per uphill test */
+    double* mlog;
+    uint64_t mlog_cap, mlog_n;
 };
 
 static ptrdiff_t off_delta(const ocpm* m, const int32_t* o) {
@@ -748,24 +751,35 @@ static int topo_ok(ocpm* m, size_t t, uint32_t L) {
 }
 
 /* monte_carlo.rs:37-145, 147-158 */
-static void attempt(ocpm* m) {
+static inline void attempt(ocpm* m, uint64_t* restrict rng, uint64_t* restrict cnt) {
     size_t c[3];
-    for (int d = 0; d < m->D; ++d) c[d] = (size_t)oc_rng_gen_range(m->rng, m->dims[d]);
+    for (int d = 0; d < m->D; ++d) c[d] = (size_t)oc_rng_gen_range(rng, m->dims[d]);
     size_t t = flat_of(m, c);
-    size_t k = (size_t)oc_rng_gen_range(m->rng, (uint64_t)m->n_copy);
+    size_t k = (size_t)oc_rng_gen_range(rng, (uint64_t)m->n_copy);
     size_t s = (size_t)((ptrdiff_t)t + m->copy_off[k]);
-    m->counters[0]++;
+    cnt[0]++;
     uint32_t L = m->cells[t], G = m->cells[s];
     if (L == G) return;
-    m->counters[1]++;
+    cnt[1]++;
     if (!topo_ok(m, t, L)) return;
     if (L != 0 && m->volume[L - 1] <= 1) return;
-    m->counters[3]++;
+    cnt[3]++;
     double dh = total_delta(m, t, L, G);
     double dh_acc = dh + 0.0; /* act term inactive */
-    int accept = dh_acc <= 0.0 || oc_rng_gen_f64(m->rng) < exp(-dh_acc);
+    int accept = dh_acc <= 0.0;
+    if (!accept) {
+        const double u = oc_rng_gen_f64(rng);
+        accept = u < exp(-dh_acc);
+        if (m->mlog) {
+            if (m->mlog_n < m->mlog_cap) {
+                m->mlog[2 * m->mlog_n] = dh_acc;
+                m->mlog[2 * m->mlog_n + 1] = u;
+            }
+            m->mlog_n++;
+        }
+    }
     if (!accept) return;
-    m->counters[4]++;
+    cnt[4]++;
     double dl, dg;
     interface_deltas(m, t, L, G, &dl, &dg);
     commit(m, t, L, G, dl, dg);
@@ -773,10 +787,18 @@ static void attempt(ocpm* m) {
 }
 
 void oc_run_mcs(ocpm* m, uint64_t n) {
+    /* the stream state and the A/N/E/C counters live in locals for the
+       loop (the topology counter stays in the model: topo_ok updates it) */
+    uint64_t rng[4], cnt[5];
+    memcpy(rng, m->rng, sizeof rng);
+    memcpy(cnt, m->counters, sizeof cnt);
     for (uint64_t i = 0; i < n; ++i) {
-        for (size_t a = 0; a < m->n_sites; ++a) attempt(m);
+        for (size_t a = 0; a < m->n_sites; ++a) attempt(m, rng, cnt);
         m->mcs += 1;
     }
+    memcpy(m->rng, rng, sizeof rng);
+    cnt[2] = m->counters[2];
+    memcpy(m->counters, cnt, sizeof cnt);
 }
 
 /* Visit interior coordinates row-major, last axis fastest. */
@@ -985,7 +1007,7 @@ int oc_init_scatter(ocpm* m, const uint64_t* cell_counts) {
     for (size_t i = 0; i < n; ++i) {
         int placed = 0;
         for (int a = 0; a < 20000 && !placed; ++a) {
-            size_t cand[3];
+            size_t cand[3] = {0, 0, 0};
             for (int d = 0; d < m->D; ++d) cand[d] = (size_t)oc_rng_gen_range(m->rng, m->dims[d]);
             int far = 1;
             for (size_t j = 0; j < i && far; ++j) {
@@ -1168,3 +1190,19 @@ void oc_contact(const ocpm* m, uint32_t* out) {
     memcpy(out, m->contact, sizeof(uint32_t) * (m->n_cells + 1) * (m->n_cells + 1));
 }
 void oc_counters(const ocpm* m, uint64_t out[5]) { memcpy(out, m->counters, sizeof(uint64_t) * 5); }
+
+/* Records the (dH, u) pair of every uphill Metropolis test from now on into
+   buf (2 doubles each, up to cap pairs); returns the number of tests seen
+   since the last call (buf NULL: stop recording). */
+uint64_t oc_metropolis_log(ocpm* m, double* buf, uint64_t cap) {
+    const uint64_t seen = m->mlog_n;
+    m->mlog = buf;
+    m->mlog_cap = cap;
+    m->mlog_n = 0;
+    return seen;
+}
+
+/* exp(-x[i]) with this platform's libm (the reference's f64::exp). */
+void oc_exp_neg(const double* x, uint64_t n, double* out) {
+    for (uint64_t i = 0; i < n; ++i) out[i] = exp(-x[i]);
+}

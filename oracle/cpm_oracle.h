@@ -118,6 +118,12 @@ uint64_t oc_label(const ocpm* m, uint32_t* fragmented, uint64_t max_frag,
                   uint32_t* medium_volume, uint8_t* medium_touch,
                   uint64_t max_components, uint64_t* n_medium);
 
+/* Records (dH, u) of every uphill Metropolis test (monte_carlo.rs:158)
+   into buf, up to cap pairs; returns the tests seen since the last call. */
+uint64_t oc_metropolis_log(ocpm* m, double* buf, uint64_t cap);
+/* exp(-x[i]) with the platform libm (glibc here, as in the reference's CI). */
+void oc_exp_neg(const double* x, uint64_t n, double* out);
+
 #ifdef __cplusplus
 }
 #endif
