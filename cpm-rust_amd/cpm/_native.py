@@ -168,6 +168,9 @@ def lib():
         L.cpm_run_mcs.argtypes = [vp, C.POINTER(vp), C.c_size_t, C.c_uint64, C.c_int32,
                                   C.POINTER(CpmCounters)]
         L.cpm_run_mcs.restype = st
+        L.cpm_run_mcs_noalloc.argtypes = [vp, C.POINTER(vp), C.c_size_t, C.c_uint64,
+                                          C.POINTER(CpmCounters)]
+        L.cpm_run_mcs_noalloc.restype = st
         L.cpm_run_mcs_plain.argtypes = [vp, vp, C.c_uint64, C.POINTER(CpmCounters)]
         L.cpm_run_mcs_plain.restype = st
         L.cpm_model_get_info.argtypes = [vp, C.POINTER(CpmModelInfo)]

@@ -172,6 +172,30 @@ cpm_status cpm_run_mcs(cpm_engine* engine, cpm_model* const* models, size_t n, u
     });
 }
 
+cpm_status cpm_run_mcs_noalloc(cpm_engine* engine, cpm_model* const* models, size_t n,
+                               uint64_t n_mcs, cpm_counters* counters) {
+    return guarded([&] {
+        need(engine && (models || n == 0), "engine/models");
+        if (n > CPM_NOALLOC_MAX_REPLICAS)
+            throw Error(CPM_ERR_ARG, "cpm_run_mcs_noalloc: " + std::to_string(n) +
+                                         " models exceed the maximum of " +
+                                         std::to_string(CPM_NOALLOC_MAX_REPLICAS) + " per call");
+        Model* stack_models[64];
+        std::vector<Model*> heap_models;
+        Model** ms = stack_models;
+        if (n > 64) {
+            heap_models.resize(n);
+            ms = heap_models.data();
+        }
+        for (size_t i = 0; i < n; ++i) {
+            need(models[i], "models[i]");
+            ms[i] = models[i]->m;
+        }
+        if (counters) *counters = cpm_counters{};
+        if (n) run_mcs_noalloc(engine->e, ms, n, n_mcs, counters);
+    });
+}
+
 cpm_status cpm_run_mcs_plain(cpm_engine* engine, cpm_model* model, uint64_t n_mcs,
                              cpm_counters* counters) {
     return guarded([&] {

@@ -26,6 +26,12 @@ void run_mcs(Engine* e, Model* const* models, size_t n, uint64_t n_mcs, int mode
 // an MCS in progress as one.  The results do not depend on where calls stop.
 void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* todo, int mode,
                   cpm_counters* counters, double budget_s = 0.0);
+// run_mcs in exact replay without allocating device memory or events:
+// every model must already be resident on this engine's device; the
+// call's descriptors live in the library's fixed scratch array (at most
+// CPM_NOALLOC_MAX_REPLICAS models per call).
+void run_mcs_noalloc(Engine* e, Model* const* models, size_t n, uint64_t n_mcs,
+                     cpm_counters* counters);
 // One model per launch, one launch per MCS.
 void run_mcs_plain(Engine* e, Model* m, uint64_t n_mcs, cpm_counters* counters);
 

@@ -92,6 +92,16 @@ struct Rep {
 
 __constant__ uint32_t c_sp2d[8];  // 256-bit 2D simple-point table
 
+// Scratch of the entry point that allocates nothing (cpm_run_mcs_noalloc):
+// one call's replica descriptors, then its launch order.  Each device holds
+// its own copy, whose address the engine takes once at creation; every
+// call on a device overwrites the whole used range before its kernels read
+// it, so nothing needs clearing, and overlapping calls on one device race.
+constexpr size_t kNoallocMaxReplicas = CPM_NOALLOC_MAX_REPLICAS;
+constexpr size_t kNoallocRepBytes = (kNoallocMaxReplicas * sizeof(Rep) + 255) / 256 * 256;
+constexpr size_t kNoallocScratchBytes = kNoallocRepBytes + kNoallocMaxReplicas * sizeof(uint32_t);
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
 // ===========================================================================
 // Global path: one lane per replica (lattices too large for LDS)
 // ===========================================================================
@@ -2093,6 +2103,8 @@ struct Engine {
     Placement forced{0.0, false};
     // false: no replica runs on the pipelined (two-wave) kernel
     bool pipelining = true;
+    // this device's copy of `scratch` (hipGetSymbolAddress at creation)
+    unsigned char* noalloc_scratch = nullptr;
     // launches of the exact-replay wave kernels per instantiation key
     std::map<int, uint64_t> launches_by_key;
     std::mutex mu;
@@ -2116,6 +2128,10 @@ Engine* engine_create(int device) {
         HIP_OK(hipStreamCreateWithFlags(&e->copy_stream[i], hipStreamNonBlocking));
     HIP_OK(hipEventCreateWithFlags(&e->work_done, hipEventDisableTiming));
     HIP_OK(hipEventRecord(e->work_done, e->stream));
+    // the device address of this device's `scratch` (the array's name is a
+    // host-side stand-in, not a device address)
+    HIP_OK(hipGetSymbolAddress((void**)&e->noalloc_scratch, HIP_SYMBOL(scratch)));
+    if (!e->noalloc_scratch) throw Error(CPM_ERR_DEVICE, "no device address for the scratch array");
     uint8_t t2[256];
     sp_table2d(t2);
     uint32_t w[8] = {0};
@@ -2482,16 +2498,41 @@ static void launch_lds_engine(int key, uint32_t lds, uint32_t count, hipStream_t
 
 // One exact-replay call with a given sigma placement fill (caller holds the
 // engine lock).
+// noalloc: allocate no device memory and create no events — every model
+// must be resident already, the descriptors and launch order go to
+// `scratch`, and launches are not timed one by one.
 static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t* todo,
-                      Placement placement, cpm_counters* counters, double budget_s = 0.0) {
+                      Placement placement, cpm_counters* counters, double budget_s = 0.0,
+                      bool noalloc = false) {
     bool any3d = false;
     for (size_t i = 0; i < n; ++i) {
+        if (noalloc && (!models[i]->dev || models[i]->dev->eng != e))
+            throw Error(CPM_ERR_ARG,
+                        "cpm_run_mcs_noalloc: model " + std::to_string(i) +
+                            " is not resident on this engine's device (advance it once with "
+                            "cpm_run_mcs)");
         upload(e, models[i]);
         any3d |= models[i]->D == 3;
     }
+    if (any3d && noalloc && !e->sp3d)
+        throw Error(CPM_ERR_ARG, "cpm_run_mcs_noalloc: the 3D simple-point table is not built yet "
+                                 "(advance a 3D model once with cpm_run_mcs)");
     if (any3d) ensure_sp3d(e);
-    ensure_buf(e->reps_dev, e->reps_cap, n);
-    ensure_buf(e->idx_dev, e->idx_cap, n);
+    Rep* reps_dev;
+    uint32_t* idx_dev;
+    if (noalloc) {
+        if (n > kNoallocMaxReplicas)
+            throw Error(CPM_ERR_ARG, "cpm_run_mcs_noalloc: " + std::to_string(n) +
+                                         " models exceed the maximum of " +
+                                         std::to_string(kNoallocMaxReplicas) + " per call");
+        reps_dev = (Rep*)e->noalloc_scratch;
+        idx_dev = (uint32_t*)(e->noalloc_scratch + kNoallocRepBytes);
+    } else {
+        ensure_buf(e->reps_dev, e->reps_cap, n);
+        ensure_buf(e->idx_dev, e->idx_cap, n);
+        reps_dev = e->reps_dev;
+        idx_dev = e->idx_dev;
+    }
     std::vector<Rep> reps(n);
     // Sigma placement for this batch: a lattice too big for LDS keeps sigma
     // in HBM; the others keep it in LDS in the placement's order (smallest
@@ -2699,17 +2740,17 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
     }
     const uint32_t global_first = (uint32_t)order.size();
     order.insert(order.end(), global_idx.begin(), global_idx.end());
-    HIP_OK(hipMemcpyAsync(e->reps_dev, reps.data(), n * sizeof(Rep), hipMemcpyHostToDevice,
+    HIP_OK(hipMemcpyAsync(reps_dev, reps.data(), n * sizeof(Rep), hipMemcpyHostToDevice,
                           e->stream));
     if (!order.empty())
-        HIP_OK(hipMemcpyAsync(e->idx_dev, order.data(), order.size() * 4, hipMemcpyHostToDevice,
+        HIP_OK(hipMemcpyAsync(idx_dev, order.data(), order.size() * 4, hipMemcpyHostToDevice,
                               e->stream));
     HIP_OK(hipEventRecord(e->ev0, e->stream));
     HIP_OK(hipEventRecord(e->cls_done[0], e->stream));  // descriptors uploaded
     if (!global_idx.empty()) {
         const uint32_t ng = (uint32_t)global_idx.size();
         hipLaunchKernelGGL(k_exact_replay, dim3((ng + 63) / 64), dim3(64), 0, e->stream,
-                           e->reps_dev, e->idx_dev + global_first, ng, e->sp3d);
+                           reps_dev, idx_dev + global_first, ng, e->sp3d);
         HIP_OK(hipGetLastError());
     }
     double replay_ms = 0.0;
@@ -2717,7 +2758,7 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
     if (!launches.empty()) {
         // one launch per group: each wave runs its replica to its MCS
         // target, refilling its own stream buffer as it goes
-        while (e->launch_ev.size() < 2 * launches.size()) {
+        while (!noalloc && e->launch_ev.size() < 2 * launches.size()) {
             hipEvent_t ev;
             HIP_OK(hipEventCreate(&ev));
             e->launch_ev.push_back(ev);
@@ -2727,11 +2768,11 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
         for (size_t li = 0; li < launches.size(); ++li) {
             const Launch& l = launches[li];
             hipStream_t st = e->cls_stream[li % Engine::kClassStreams];
-            HIP_OK(hipEventRecord(e->launch_ev[2 * li], st));
-            launch_lds_engine(l.D, l.lds, l.count, st, e->reps_dev, e->idx_dev + l.first, e->sp3d);
+            if (!noalloc) HIP_OK(hipEventRecord(e->launch_ev[2 * li], st));
+            launch_lds_engine(l.D, l.lds, l.count, st, reps_dev, idx_dev + l.first, e->sp3d);
             HIP_OK(hipGetLastError());
             e->launches_by_key[l.D] += 1;
-            HIP_OK(hipEventRecord(e->launch_ev[2 * li + 1], st));
+            if (!noalloc) HIP_OK(hipEventRecord(e->launch_ev[2 * li + 1], st));
         }
         for (int k = 0; k < ns; ++k) {
             HIP_OK(hipEventRecord(e->cls_done[k], e->cls_stream[k]));
@@ -2740,12 +2781,12 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
     }
     HIP_OK(hipEventRecord(e->ev1, e->stream));
     HIP_OK(hipEventRecord(e->work_done, e->stream));
-    HIP_OK(hipMemcpyAsync(reps.data(), e->reps_dev, n * sizeof(Rep), hipMemcpyDeviceToHost,
+    HIP_OK(hipMemcpyAsync(reps.data(), reps_dev, n * sizeof(Rep), hipMemcpyDeviceToHost,
                           e->stream));
     HIP_OK(hipStreamSynchronize(e->stream));
     float ms = 0.f;
     HIP_OK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
-    for (size_t li = 0; li < launches.size(); ++li) {
+    for (size_t li = 0; li < (noalloc ? 0 : launches.size()); ++li) {
         float lms = 0.f;
         HIP_OK(hipEventElapsedTime(&lms, e->launch_ev[2 * li], e->launch_ev[2 * li + 1]));
         replay_ms += lms;
@@ -3108,6 +3149,25 @@ void engine_set_pipelining(Engine* e, bool on) {
 std::vector<std::pair<int, uint64_t>> engine_launch_counts(Engine* e) {
     std::lock_guard<std::mutex> lock(e->mu);
     return {e->launches_by_key.begin(), e->launches_by_key.end()};
+}
+
+void run_mcs_noalloc(Engine* e, Model* const* models, size_t n, uint64_t n_mcs,
+                     cpm_counters* counters) {
+    for (size_t i = 0; i < n; ++i)
+        if (models[i]->attempt_in_mcs)
+            throw Error(CPM_ERR_ARG, "a model stopped mid-MCS continues in exact replay only");
+    std::lock_guard<std::mutex> lock(e->mu);
+    HIP_OK(hipSetDevice(e->device));
+    std::vector<uint64_t> todo(n, n_mcs);
+    const Placement fixed = e->forced.fill > 0.0 ? e->forced : fixed_placement();
+    Placement pl = kDefaultPlacement;
+    if (fixed.fill > 0.0) {
+        pl = fixed;
+    } else {
+        auto it = e->tuned_placement.find(batch_signature(e, models, n, todo.data()));
+        if (it != e->tuned_placement.end()) pl = kPlacements[it->second];
+    }
+    run_exact(e, models, n, todo.data(), pl, counters, 0.0, true);
 }
 
 void run_mcs(Engine* e, Model* const* models, size_t n, uint64_t n_mcs, int mode,

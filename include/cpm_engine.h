@@ -148,6 +148,24 @@ cpm_status cpm_run_mcs(cpm_engine* engine, cpm_model* const* models, size_t n,
 cpm_status cpm_run_mcs_plain(cpm_engine* engine, cpm_model* model, uint64_t n_mcs,
                              cpm_counters* counters);
 
+/* No counterpart in the reference: cpm_run_mcs in CPM_MODE_EXACT_REPLAY
+   for callers that cannot allocate device memory (code under HIP graph
+   capture, tight serving loops, harnesses that forbid hipMalloc).  It
+   allocates nothing on the device and creates no HIP events: every model
+   must already be resident on this engine's device (advanced by an earlier
+   cpm_run_mcs or cpm_run on it; 3D models also need the engine's 3D table,
+   built by that call), and the call's replica descriptors and launch order
+   live in a fixed device array of the library, at most
+   CPM_NOALLOC_MAX_REPLICAS models per call (larger n returns CPM_ERR_ARG;
+   the array takes about 2.2 MiB of each device the library runs on, until
+   the process ends).  The results are bit-identical to cpm_run_mcs.  Every
+   call on one device shares that array: calls that overlap on one device
+   race and must not be made.  counters->replay_launch_ms stays 0 (no
+   per-launch events). */
+#define CPM_NOALLOC_MAX_REPLICAS 4096
+cpm_status cpm_run_mcs_noalloc(cpm_engine* engine, cpm_model* const* models, size_t n,
+                               uint64_t n_mcs, cpm_counters* counters);
+
 /* State access (the fields of State<D>, crates/cpm-core/src/state.rs:168-203,
    plus the RNG state and mcs of CPM<D>, model.rs:15-38).  Sizes: lattice
    n_sites (row-major, unpadded), volume/interface/type_index n_cells,
