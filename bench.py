@@ -256,11 +256,15 @@ def serving_trace(cpm, n_requests: int = 1500, rate: float = 80.0, seed: int = 7
                    "U[40,160] MCS per request"}
 
 
-def sorting_256(cpm, seed: int = 1):
+def sorting_256_parts(seed: int = 1):
     """The single-replica sorting workload: 256x256 periodic, 4 cell types
     scattered at random, phi = 0.8 via cells_for_density with V* = 40,
     I* = 30, lambda_V = 5, lambda_I = 1, J medium row 16 and type pairs
-    U[2,14] drawn from the seed."""
+    U[2,14] drawn from the seed.  Returns (cell counts per type, J)."""
+    import warnings as _w
+
+    from cpm.parameters import cells_for_density
+
     rng = np.random.default_rng(seed)
     T = 4
     J = np.zeros((T + 1, T + 1))
@@ -268,22 +272,30 @@ def sorting_256(cpm, seed: int = 1):
     for a in range(1, T + 1):
         for b in range(a, T + 1):
             J[a, b] = J[b, a] = float(rng.uniform(2, 14))
+    with _w.catch_warnings():
+        _w.simplefilter("ignore")  # the low-resolution warning of V* = 40
+        n = cells_for_density((256, 256), 40, 0.8)
+    return [n // T + (1 if t < n % T else 0) for t in range(T)], J.tolist()
+
+
+def sorting_256(cpm, seed: int = 1):
+    counts, J = sorting_256_parts(seed)
     sim = cpm.CPM(grid=(256, 256), boundary="periodic", seed=seed)
-    n = cpm.cells_for_density((256, 256), 40, 0.8)
-    for t in range(T):
+    for t, c in enumerate(counts):
         sim.register_cell_type(f"t{t}", 40, 30, 5.0, 1.0)
-        sim.add_cells(f"t{t}", n // T + (1 if t < n % T else 0))
-    sim.set_adhesion(J.tolist())
+        sim.add_cells(f"t{t}", c)
+    sim.set_adhesion(J)
     sim.initialize(warn_on_default_init=False)
     return sim
 
 
-def organoid_64(cpm, seed: int = 1):
+def organoid_64_parts(seed: int = 1):
     """The 3D organoid workload: 64^3 fixed boundary, 3 cell types in nested
     spherical shells (radii 10 / 18 / 24 sites) cut into cells of about
     V* = 1048 sites (Voronoi within each shell, each cell reduced to its
     largest face-connected piece), I* = 2760, lambda_V = 10, lambda_I = 2,
-    J medium row 16 and type pairs U[2,16] drawn from the seed."""
+    J medium row 16 and type pairs U[2,16] drawn from the seed.  Returns
+    (lattice, cell type of each cell, J)."""
     from scipy import ndimage
 
     rng = np.random.default_rng(seed)
@@ -313,10 +325,15 @@ def organoid_64(cpm, seed: int = 1):
     for a in range(1, T + 1):
         for b in range(a, T + 1):
             J[a, b] = J[b, a] = float(rng.uniform(2, 16))
-    sim = cpm.CPM(grid=(L, L, L), boundary="fixed", seed=seed)
-    for t in range(T):
+    return lat, type_of, J.tolist()
+
+
+def organoid_64(cpm, seed: int = 1):
+    lat, type_of, J = organoid_64_parts(seed)
+    sim = cpm.CPM(grid=lat.shape, boundary="fixed", seed=seed)
+    for t in range(3):
         sim.register_cell_type(f"t{t}", 1048, 2760, 10.0, 2.0)
-    sim.set_adhesion(J.tolist())
+    sim.set_adhesion(J)
     sim.initialize(lattice=lat.ravel().tolist(), cell_type_of=type_of)
     return sim
 
@@ -359,15 +376,32 @@ def single_replica_rates(cpm):
     return out
 
 
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+# the host threads a one-GPU share of the machine may keep busy
+HOST_THREADS = 16
+
+
 def cpu_baseline(params, seconds: float = 12.0):
     """The oracle (C restatement of the reference loop) on the host cores:
     replicas of the same workload advanced by one sampling interval each,
-    one replica per thread (ctypes releases the GIL)."""
+    one replica per thread (ctypes releases the GIL), as the reference's
+    run_batch spreads replicas over a thread pool.  Uses HOST_THREADS
+    threads (a one-GPU share of the host), fewer if the host has fewer."""
     from concurrent.futures import ThreadPoolExecutor
 
     from oracle import oracle as O
 
-    cores = min(os.cpu_count() or 1, 16)
+    cores = min(os.cpu_count() or 1, HOST_THREADS)
     models = []
     for p in params[: cores * 2]:
         cfg = O.Config(grid=(p["side"], p["side"]), boundary="periodic", seed=p["seed"],
@@ -391,8 +425,61 @@ def cpu_baseline(params, seconds: float = 12.0):
     dt = time.perf_counter() - t0
     attempts = sum(int(m.counters()[0]) for m in models) - before
     return {"value": attempts / dt, "unit": "copy attempts/s", "cores": cores, "kind": "port",
+            "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
             "sample": f"{len(models)} replicas of the batched workload x {mcs} MCS "
-                      f"({attempts} attempts in {dt:.1f} s)"}
+                      f"({attempts} attempts in {dt:.1f} s on {cores} threads)",
+            "single_core": cpu_single_core(O)}
+
+
+def _time_oracle(m, seconds: float) -> dict:
+    """run_mcs on one core for about `seconds` (after one warm MCS)."""
+    m.run_mcs(1)
+    t0 = time.perf_counter()
+    m.run_mcs(1)
+    one = max(time.perf_counter() - t0, 1e-6)
+    mcs = max(1, int(seconds / one))
+    a0 = int(m.counters()[0])
+    t0 = time.perf_counter()
+    m.run_mcs(mcs)
+    dt = time.perf_counter() - t0
+    att = int(m.counters()[0]) - a0
+    return {"mcs_per_s": mcs / dt, "attempts_per_s": att / dt, "mcs_timed": mcs}
+
+
+def cpu_single_core(O, seconds: float = 3.0) -> dict:
+    """One-core oracle rates of the single-replica workloads (the reference's
+    run is single-threaded): the README quickstart (30x30, 5 cells), the
+    256x256 four-type sorting lattice and the 64^3 organoid, plus the
+    reference's published 30x30 / 10-cell fixture
+    (notebooks/performance_demo.ipynb:106: 68,579.0 MCS/s, hardware not
+    stated) as a check on the oracle's speed."""
+    out = {}
+    qs = O.Config(grid=(30, 30), boundary="periodic", seed=1,
+                  cell_types=[O.CellTypeSpec("epithelial", 50, 75, 10.0, 2.0)], cell_counts=[5],
+                  adhesion=[[0.0, 5.0], [5.0, 2.0]], warn_on_default_init=False)
+    out["quickstart_30x30"] = _time_oracle(O.Model(qs), seconds)
+    counts, J = sorting_256_parts(1)
+    srt = O.Config(grid=(256, 256), boundary="periodic", seed=1,
+                   cell_types=[O.CellTypeSpec(f"t{t}", 40, 30, 5.0, 1.0) for t in range(4)],
+                   cell_counts=counts, adhesion=J, warn_on_default_init=False)
+    out["sorting_256x256_4types"] = _time_oracle(O.Model(srt), seconds)
+    lat, type_of, J = organoid_64_parts(1)
+    org = O.Config(grid=lat.shape, boundary="fixed", seed=1,
+                   cell_types=[O.CellTypeSpec(f"t{t}", 1048, 2760, 10.0, 2.0) for t in range(3)],
+                   cell_counts=[0, 0, 0], adhesion=J, init_lattice=lat.ravel().tolist(),
+                   init_cell_type_of=type_of)
+    out["organoid_64cubed_3types"] = _time_oracle(O.Model(org), seconds)
+    fx = O.Config(grid=(30, 30), boundary="periodic", seed=1,
+                  cell_types=[O.CellTypeSpec("a", 30, 40, 1.0, 0.2)], cell_counts=[10],
+                  adhesion=[[0.0, 3.0], [3.0, 1.0]], warn_on_default_init=False)
+    m = O.Model(fx)
+    t0 = time.perf_counter()
+    m.run_mcs(20000)
+    out["published_fixture_30x30_10cells"] = {
+        "oracle_mcs_per_s": 20000 / (time.perf_counter() - t0),
+        "reference_published_mcs_per_s": 68579.0,
+        "source": "notebooks/performance_demo.ipynb:106 (hardware not stated)"}
+    return out
 
 
 def main():
@@ -437,7 +524,7 @@ def main():
     from concurrent.futures import ThreadPoolExecutor
 
     sims = [make_sim(cpm, p, p["seed"]) for p in params]
-    with ThreadPoolExecutor(min(os.cpu_count() or 1, 16)) as ex:  # init is host C++ code
+    with ThreadPoolExecutor(min(os.cpu_count() or 1, HOST_THREADS)) as ex:  # host C++ init
         reps = list(ex.map(lambda s: cpm.Replica(s, 0, 10000, INTERVAL), sims))
     t_init = time.perf_counter() - t_init0
     print(f"[bench] {len(reps)} replicas initialised in {t_init:.1f} s", flush=True)
@@ -481,13 +568,13 @@ def main():
     sims_per_hour = n_reps * 3600.0 / (t_init + 20 * step_s)
 
     b64 = None if args.no_batch64 else batch_64(cpm, args.replicas, args.interval, plain_rate,
-                                                 min(os.cpu_count() or 1, 16))
+                                                 min(os.cpu_count() or 1, HOST_THREADS))
 
     full = None
     if not args.no_run_batch:
         del reps  # the step replicas' device memory
         full = run_batch_full(cpm, args.replicas, args.run_batch_mcs, args.interval,
-                              min(os.cpu_count() or 1, 16))
+                              min(os.cpu_count() or 1, HOST_THREADS))
 
     single = single_replica_rates(cpm)
     seeds = None if args.no_many_seeds else many_seeds_256(cpm)
@@ -505,7 +592,8 @@ def main():
         copy_calibration()
     cpu = None if args.no_cpu else cpu_baseline(params)
     line = {
-        "metric": "copy attempts/s on 1 GPU (batched exact-replay run_batch)",
+        "metric": "copy attempts/s on 1 GPU (batched exact replay: run_mcs steps over the "
+                  "4096-replica run_batch workload)",
         "value": value,
         "unit": "copy attempts/s",
         "n_gpus": 1,
@@ -523,7 +611,8 @@ def main():
                    "parallelism": "1 GPU, replicas batched in one launch"},
         "plain_attempts_per_s": plain_rate,
         "batched_vs_plain": value / plain_rate,
-        "sims_per_hour_projected": sims_per_hour,
+        "sims_per_hour": None if full is None else full["sims_per_hour"],
+        "sims_per_hour_projected_from_steps": sims_per_hour,
         "run_batch": full,
         "counters": {k: totals[k] for k in totals},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
