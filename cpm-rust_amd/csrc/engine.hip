@@ -20,7 +20,6 @@
 // reference's operation order (crates/cpm-core/src/energy.rs:92-103,
 // 167-202, 242-264, 521-523); sums of +-1.0 are formed from popcounts,
 // which is exact.
-#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -74,7 +73,6 @@ struct Rep {
     const uint32_t* targets;  // V*[T], I*[T]
     uint64_t* draws;          // kFill buffered draws + kSegs segment states (wave path)
     uint16_t* sig16;          // packed sigma in HBM (wave path with sigma outside LDS)
-    unsigned long long* scratch;  // grid sublattice: per-cell priorities + 2 counters
     uint64_t rng[4];          // consumer state at the start / end of a call
     double energy;
     uint64_t mcs;
@@ -1366,11 +1364,19 @@ __device__ __forceinline__ uint64_t splitmix_at(uint64_t key, uint64_t ctr) {
 }
 
 constexpr uint32_t kSlThreads = 256;
-// Sublattice lattices with at least this many sites run grid-wide
-// (k_sublattice_grid), smaller ones one workgroup per replica.  The choice
-// depends on the replica alone, so its trajectory does not depend on what
-// else is in the batch.
-constexpr uint64_t kSlGridSites = 16384;
+// Lattices with at least kSlBigSites sites (or whose sigma does not fit in
+// LDS) run on workgroups of kSlThreadsBig threads, one per replica, smaller
+// ones on workgroups of kSlThreads; a batch is one site per thread.  The
+// choice depends on the replica alone, so its trajectory does not depend on
+// what else is in the batch.
+constexpr uint32_t kSlThreadsBig = 512;
+constexpr uint32_t kSlSitesPerThread = 16;  // sites per thread per batch, big workgroups
+constexpr uint64_t kSlBigSites = 16384;
+// the sublattice kernels: (key, D, sigma in HBM, threads)
+#define CPM_SL_KERNELS(X)                                                                    \
+    X(200, 2, false, kSlThreads) X(300, 3, false, kSlThreads)                                \
+    X(210, 2, false, kSlThreadsBig) X(211, 2, true, kSlThreadsBig)                           \
+    X(310, 3, false, kSlThreadsBig) X(311, 3, true, kSlThreadsBig)
 constexpr uint32_t kSlSub = 4;  // sub-sweeps per MCS (a power of two)
 
 // Dynamic LDS of the sublattice kernel: the exact-replay layout plus one
@@ -1435,10 +1441,12 @@ __global__ void __launch_bounds__(NT) k_sublattice(Rep* __restrict__ reps,
     }
     const uint32_t n_copy = R.n_copy, n_energy = R.n_energy, n_topo = R.n_topo;
     const bool periodic = R.boundary == CPM_BOUNDARY_PERIODIC;
-    const uint64_t ncell1 = (uint64_t)n_cells + 1;
     const uint64_t key = R.fill_state[0];
     unsigned long long c_att = 0, c_nn = 0, c_top = 0, c_en = 0, c_acc = 0;
     constexpr uint32_t NC = 1u << D;
+    // sites per thread in a batch: several on the big workgroups, which
+    // amortise each round's barriers and commit latency over more attempts
+    constexpr uint32_t KS = NT == kSlThreadsBig ? kSlSitesPerThread : 1u;
 
     for (uint64_t m = 0; m < R.todo; ++m) {
         const uint64_t mcs = R.mcs + m;
@@ -1468,180 +1476,224 @@ __global__ void __launch_bounds__(NT) k_sublattice(Rep* __restrict__ reps,
                 if (nsite == 0) continue;
                 // batches start at a random rotation of the colour's sites
                 const uint32_t rot = (uint32_t)(splitmix_at(key ^ 0x5bd1e995ull, phase) % nsite);
-                for (uint32_t b0 = 0; b0 < nsite; b0 += NT) {
-                    const uint32_t jb = b0 + tid;
-                    uint32_t j = 0, c[D], t = 0, s = 0, Lw = 0, Gw = 0;
-                    unsigned long long pr = 0;
-                    uint64_t ctr = 0;
-                    bool pending = false;
-                    if (jb < nsite) {
-                        j = jb + rot;
-                        if (j >= nsite) j -= nsite;
-                        uint32_t rem = j;
+                // a colour site's coordinates and padded flat index
+                auto site_of = [&](uint32_t j, uint32_t* c, uint32_t& t) {
+                    uint32_t rem = j;
+                    t = 0;
 #pragma unroll
-                        for (int d = D - 1; d >= 0; --d) {
-                            c[d] = 2 * (rem % half[d]) + ((col >> d) & 1u);
-                            rem /= half[d];
-                            t += (c[d] + 1) * stride[d];
+                    for (int d = D - 1; d >= 0; --d) {
+                        c[d] = 2 * (rem % half[d]) + ((col >> d) & 1u);
+                        rem /= half[d];
+                        t += (c[d] + 1) * stride[d];
+                    }
+                };
+                // a winning attempt: volume check, dH, Metropolis, commit
+                // (dynamics.rs:23-224); no other winner of its round shares
+                // a cell with it or a site of its neighbourhood
+                auto commit_win = [&](uint32_t j, uint32_t Lw, uint32_t Gw) {
+                    const uint32_t L = Lw & kIdMask, G = Gw & kIdMask;
+                    uint32_t c[D], t;
+                    site_of(j, c, t);
+                    const uint64_t ctr = (phase * (uint64_t)R.n_sites + j) * 4;
+                    const uint32_t vL = L ? vol[L - 1] : 0u;
+                    if (L == 0 || vL > 1) {
+                        c_en++;
+                        const uint32_t lidx = Lw >> 12, gidx = Gw >> 12;
+                        const uint32_t tl = lidx ? lidx - 1 : 0u, tg = gidx ? gidx - 1 : 0u;
+                        int nl = 0, ng = 0;
+                        double dadh = 0.0;  // energy.rs:242-264, stencil order
+                        for (uint32_t q = 0; q < n_energy; ++q) {
+                            const uint32_t w = sig.ld((int32_t)t + R.energy_off[q]);
+                            const uint32_t nb = w & kIdMask, ni = w >> 12;
+                            nl += nb == L;
+                            ng += nb == G;
+                            const double before = nb != L ? J[lidx * T1 + ni] : 0.0;
+                            const double after = nb != G ? J[gidx * T1 + ni] : 0.0;
+                            dadh += 1.0 * (after - before);
                         }
-                        ctr = (phase * (uint64_t)R.n_sites + j) * 4;
-                        const uint64_t h0 = splitmix_at(key, ctr);
-                        if ((h0 & (kSlSub - 1)) == 0) {  // this site is attempted
-                            const uint32_t k = (uint32_t)(((h0 >> 32) * (uint64_t)n_copy) >> 32);
-                            s = (uint32_t)((int32_t)t + R.copy_off[k]);
-                            c_att++;
-                            Lw = sig.ld(t);
-                            Gw = sig.ld(s);
-                            if (Lw != Gw) {
-                                c_nn++;
-                                const uint32_t L = Lw & kIdMask;
-                                bool simple = true;
-                                if (L != 0) {
-                                    uint32_t pat = 0;
-                                    for (uint32_t i = 0; i < n_topo; ++i)
-                                        pat |= (uint32_t)((sig.ld((int32_t)t + R.topo_off[i]) & kIdMask) == L) << i;
-                                    c_top++;
-                                    simple = D == 2 ? ((c_sp2d[pat >> 5] >> (pat & 31)) & 1u)
-                                                    : ((sp3d[pat >> 6] >> (pat & 63)) & 1ull);
+                        const int dl_i = 2 * nl - (int)n_energy, dg_i = (int)n_energy - 2 * ng;
+                        double dvol = 0.0, ditf = 0.0;
+                        if (L) {
+                            const double v = (double)vL;
+                            const double x1 = (v - 1.0) - (double)tv[tl], x0 = v - (double)tv[tl];
+                            dvol += lv[tl] * (x1 * x1) - lv[tl] * (x0 * x0);
+                            const double i0 = (double)itf[L - 1];
+                            const double y1 = (i0 + (double)dl_i) - (double)ti[tl], y0 = i0 - (double)ti[tl];
+                            ditf += li[tl] * (y1 * y1) - li[tl] * (y0 * y0);
+                        }
+                        if (G) {
+                            const double v = (double)vol[G - 1];
+                            const double x1 = (v + 1.0) - (double)tv[tg], x0 = v - (double)tv[tg];
+                            dvol += lv[tg] * (x1 * x1) - lv[tg] * (x0 * x0);
+                            const double i0 = (double)itf[G - 1];
+                            const double y1 = (i0 + (double)dg_i) - (double)ti[tg], y0 = i0 - (double)ti[tg];
+                            ditf += li[tg] * (y1 * y1) - li[tg] * (y0 * y0);
+                        }
+                        const double dh = dvol + ditf + dadh;
+                        bool accept = dh <= 0.0;
+                        if (!accept) {
+                            const uint64_t hu = splitmix_at(key, ctr + 2);
+                            const double u = (double)(hu >> 11) * (1.0 / 9007199254740992.0);
+                            accept = u < exp(-dh);
+                        }
+                        if (accept) {
+                            c_acc++;
+                            // (the contact counts are recounted from the lattice
+                            // after the call: k_contact_count)
+                            sig.st(t, Gw);
+                            // this round owns L's and G's V, I and position sums
+                            if (L) {
+                                vol[L - 1] = vL - 1;
+                                itf[L - 1] = (uint32_t)((int32_t)itf[L - 1] + dl_i);
+                                // no other winner of this round touches cell L
+                                // or G: the sums are updated without waiting
+                                unsigned long long* sp =
+                                    (unsigned long long*)(R.sum_pos + (uint64_t)(L - 1) * D);
+                                unsigned long long* so =
+                                    (unsigned long long*)(R.sum_outer + (uint64_t)(L - 1) * D * D);
+                                for (int d = 0; d < D; ++d)
+                                    atomicAdd(&sp[d], (unsigned long long)(-(int64_t)c[d]));
+                                for (int d = 0; d < D; ++d)
+                                    for (int e2 = 0; e2 < D; ++e2)
+                                        atomicAdd(&so[d * D + e2],
+                                                  (unsigned long long)(-((int64_t)c[d] * (int64_t)c[e2])));
+                            }
+                            if (G) {
+                                const uint32_t vg_new = vol[G - 1] + 1;
+                                vol[G - 1] = vg_new;
+                                itf[G - 1] = (uint32_t)((int32_t)itf[G - 1] + dg_i);
+                                int64_t* sp = R.sum_pos + (uint64_t)(G - 1) * D;
+                                unsigned long long* so =
+                                    (unsigned long long*)(R.sum_outer + (uint64_t)(G - 1) * D * D);
+                                const uint32_t vbg = max(vg_new - 1, 1u);
+                                int64_t img[D];
+                                for (int d = 0; d < D; ++d) {
+                                    const double cen = (double)sp[d] / (double)vbg;
+                                    int64_t best = (int64_t)c[d];
+                                    if (periodic) {
+                                        double bd = fabs((double)best - cen);
+                                        const int64_t lo = (int64_t)c[d] - (int64_t)dims[d];
+                                        const int64_t hi = (int64_t)c[d] + (int64_t)dims[d];
+                                        const double dlo = fabs((double)lo - cen);
+                                        if (dlo < bd) {
+                                            best = lo;
+                                            bd = dlo;
+                                        }
+                                        if (fabs((double)hi - cen) < bd) best = hi;
+                                    }
+                                    img[d] = best;
                                 }
-                                if (simple) {
-                                    pending = true;
-                                    // unique within the phase: site index in the low 24 bits
-                                    pr = 0x8000000000000000ull |
-                                         (splitmix_at(key, ctr + 1) & 0x7fffffffff000000ull) | j;
+                                for (int d = 0; d < D; ++d)
+                                    atomicAdd((unsigned long long*)&sp[d], (unsigned long long)img[d]);
+                                for (int d = 0; d < D; ++d)
+                                    for (int e2 = 0; e2 < D; ++e2)
+                                        atomicAdd(&so[d * D + e2], (unsigned long long)(img[d] * img[e2]));
+                            }
+                            if (periodic) {  // t's halo mirrors
+                                uint32_t opt[D][3], nopt[D], total = 1;
+                                for (int d = 0; d < D; ++d) {
+                                    nopt[d] = 1;
+                                    opt[d][0] = c[d] + 1;
+                                    if (c[d] == 0) opt[d][nopt[d]++] = dims[d] + 1;
+                                    if (c[d] == dims[d] - 1) opt[d][nopt[d]++] = 0;
+                                    total *= nopt[d];
+                                }
+                                for (uint32_t q = 1; q < total; ++q) {
+                                    uint32_t r = q, f = 0;
+                                    for (int d = D - 1; d >= 0; --d) {
+                                        f += opt[d][r % nopt[d]] * stride[d];
+                                        r /= nopt[d];
+                                    }
+                                    sig.st(f, Gw);
                                 }
                             }
                         }
                     }
+                };
+                // batches of KS sites per thread (NT * KS sites)
+                for (uint32_t b0 = 0; b0 < nsite; b0 += NT * KS) {
+                    uint32_t pj[KS], pw[KS];  // site index; packed words L | G << 16
+                    unsigned long long ppr[KS];
+                    uint32_t pmask = 0;  // bit q: slot q holds a pending attempt
+#pragma unroll
+                    for (uint32_t q = 0; q < KS; ++q) {
+                        pj[q] = 0;
+                        pw[q] = 0;
+                        ppr[q] = 0;
+                        const uint32_t jb = b0 + q * NT + tid;
+                        if (jb >= nsite) continue;
+                        uint32_t j = jb + rot;
+                        if (j >= nsite) j -= nsite;
+                        uint32_t c[D], t;
+                        site_of(j, c, t);
+                        const uint64_t ctr = (phase * (uint64_t)R.n_sites + j) * 4;
+                        const uint64_t h0 = splitmix_at(key, ctr);
+                        if ((h0 & (kSlSub - 1)) != 0) continue;  // this site is not attempted
+                        const uint32_t k = (uint32_t)(((h0 >> 32) * (uint64_t)n_copy) >> 32);
+                        const uint32_t s = (uint32_t)((int32_t)t + R.copy_off[k]);
+                        c_att++;
+                        const uint32_t Lw = sig.ld(t), Gw = sig.ld(s);
+                        if (Lw == Gw) continue;
+                        c_nn++;
+                        const uint32_t L = Lw & kIdMask;
+                        bool simple = true;
+                        if (L != 0) {
+                            uint32_t pat = 0;
+                            for (uint32_t i = 0; i < n_topo; ++i)
+                                pat |= (uint32_t)((sig.ld((int32_t)t + R.topo_off[i]) & kIdMask) == L) << i;
+                            c_top++;
+                            simple = D == 2 ? ((c_sp2d[pat >> 5] >> (pat & 31)) & 1u)
+                                            : ((sp3d[pat >> 6] >> (pat & 63)) & 1ull);
+                        }
+                        if (!simple) continue;
+                        pmask |= 1u << q;
+                        pj[q] = j;
+                        pw[q] = Lw | Gw << 16;
+                        // unique within the phase: site index in the low 24 bits
+                        ppr[q] = 0x8000000000000000ull |
+                                 (splitmix_at(key, ctr + 1) & 0x7fffffffff000000ull) | j;
+                    }
                     // commit rounds: the highest priority on both cells goes first
                     for (;;) {
-                        if (!__syncthreads_or(pending)) break;
-                        const uint32_t L = Lw & kIdMask, G = Gw & kIdMask;
-                        if (pending) {
-                            if (L) atomicMax(&prio[L - 1], pr);
-                            if (G) atomicMax(&prio[G - 1], pr);
-                        }
-                        __syncthreads();
-                        const bool win = pending && (!L || prio[L - 1] == pr) &&
-                                         (!G || prio[G - 1] == pr);
-                        __syncthreads();
-                        if (pending) {
-                            if (L) prio[L - 1] = 0;
-                            if (G) prio[G - 1] = 0;
-                        }
-                        if (win) {
-                            pending = false;
-                            const uint32_t vL = L ? vol[L - 1] : 0u;
-                            if (L == 0 || vL > 1) {
-                                c_en++;
-                                const uint32_t lidx = Lw >> 12, gidx = Gw >> 12;
-                                const uint32_t tl = lidx ? lidx - 1 : 0u, tg = gidx ? gidx - 1 : 0u;
-                                int nl = 0, ng = 0;
-                                double dadh = 0.0;  // energy.rs:242-264, stencil order
-                                for (uint32_t q = 0; q < n_energy; ++q) {
-                                    const uint32_t w = sig.ld((int32_t)t + R.energy_off[q]);
-                                    const uint32_t nb = w & kIdMask, ni = w >> 12;
-                                    nl += nb == L;
-                                    ng += nb == G;
-                                    const double before = nb != L ? J[lidx * T1 + ni] : 0.0;
-                                    const double after = nb != G ? J[gidx * T1 + ni] : 0.0;
-                                    dadh += 1.0 * (after - before);
-                                }
-                                const int dl_i = 2 * nl - (int)n_energy, dg_i = (int)n_energy - 2 * ng;
-                                double dvol = 0.0, ditf = 0.0;
-                                if (L) {
-                                    const double v = (double)vL;
-                                    const double x1 = (v - 1.0) - (double)tv[tl], x0 = v - (double)tv[tl];
-                                    dvol += lv[tl] * (x1 * x1) - lv[tl] * (x0 * x0);
-                                    const double i0 = (double)itf[L - 1];
-                                    const double y1 = (i0 + (double)dl_i) - (double)ti[tl], y0 = i0 - (double)ti[tl];
-                                    ditf += li[tl] * (y1 * y1) - li[tl] * (y0 * y0);
-                                }
-                                if (G) {
-                                    const double v = (double)vol[G - 1];
-                                    const double x1 = (v + 1.0) - (double)tv[tg], x0 = v - (double)tv[tg];
-                                    dvol += lv[tg] * (x1 * x1) - lv[tg] * (x0 * x0);
-                                    const double i0 = (double)itf[G - 1];
-                                    const double y1 = (i0 + (double)dg_i) - (double)ti[tg], y0 = i0 - (double)ti[tg];
-                                    ditf += li[tg] * (y1 * y1) - li[tg] * (y0 * y0);
-                                }
-                                const double dh = dvol + ditf + dadh;
-                                bool accept = dh <= 0.0;
-                                if (!accept) {
-                                    const uint64_t hu = splitmix_at(key, ctr + 2);
-                                    const double u = (double)(hu >> 11) * (1.0 / 9007199254740992.0);
-                                    accept = u < exp(-dh);
-                                }
-                                if (accept) {
-                                    c_acc++;
-                                    // dynamics.rs:73-98: contacts on the pre-move lattice
-                                    for (uint32_t q = 0; q < n_energy; ++q) {
-                                        const uint32_t nb = sig.ld((int32_t)t + R.energy_off[q]) & kIdMask;
-                                        if (nb != L)
-                                            atomicSub(&R.contact[(uint64_t)min(L, nb) * ncell1 + max(L, nb)], 1u);
-                                        if (nb != G)
-                                            atomicAdd(&R.contact[(uint64_t)min(G, nb) * ncell1 + max(G, nb)], 1u);
-                                    }
-                                    sig.st(t, Gw);
-                                    // this round owns L's and G's V, I and position sums
-                                    if (L) {
-                                        vol[L - 1] = vL - 1;
-                                        itf[L - 1] = (uint32_t)((int32_t)itf[L - 1] + dl_i);
-                                        int64_t* sp = R.sum_pos + (uint64_t)(L - 1) * D;
-                                        int64_t* so = R.sum_outer + (uint64_t)(L - 1) * D * D;
-                                        for (int d = 0; d < D; ++d) sp[d] -= (int64_t)c[d];
-                                        for (int d = 0; d < D; ++d)
-                                            for (int e2 = 0; e2 < D; ++e2)
-                                                so[d * D + e2] -= (int64_t)c[d] * (int64_t)c[e2];
-                                    }
-                                    if (G) {
-                                        const uint32_t vg_new = vol[G - 1] + 1;
-                                        vol[G - 1] = vg_new;
-                                        itf[G - 1] = (uint32_t)((int32_t)itf[G - 1] + dg_i);
-                                        int64_t* sp = R.sum_pos + (uint64_t)(G - 1) * D;
-                                        int64_t* so = R.sum_outer + (uint64_t)(G - 1) * D * D;
-                                        const uint32_t vbg = max(vg_new - 1, 1u);
-                                        int64_t img[D];
-                                        for (int d = 0; d < D; ++d) {
-                                            const double cen = (double)sp[d] / (double)vbg;
-                                            int64_t best = (int64_t)c[d];
-                                            if (periodic) {
-                                                double bd = fabs((double)best - cen);
-                                                const int64_t lo = (int64_t)c[d] - (int64_t)dims[d];
-                                                const int64_t hi = (int64_t)c[d] + (int64_t)dims[d];
-                                                const double dlo = fabs((double)lo - cen);
-                                                if (dlo < bd) {
-                                                    best = lo;
-                                                    bd = dlo;
-                                                }
-                                                if (fabs((double)hi - cen) < bd) best = hi;
-                                            }
-                                            img[d] = best;
-                                        }
-                                        for (int d = 0; d < D; ++d) sp[d] += img[d];
-                                        for (int d = 0; d < D; ++d)
-                                            for (int e2 = 0; e2 < D; ++e2) so[d * D + e2] += img[d] * img[e2];
-                                    }
-                                    if (periodic) {  // t's halo mirrors
-                                        uint32_t opt[D][3], nopt[D], total = 1;
-                                        for (int d = 0; d < D; ++d) {
-                                            nopt[d] = 1;
-                                            opt[d][0] = c[d] + 1;
-                                            if (c[d] == 0) opt[d][nopt[d]++] = dims[d] + 1;
-                                            if (c[d] == dims[d] - 1) opt[d][nopt[d]++] = 0;
-                                            total *= nopt[d];
-                                        }
-                                        for (uint32_t q = 1; q < total; ++q) {
-                                            uint32_t r = q, f = 0;
-                                            for (int d = D - 1; d >= 0; --d) {
-                                                f += opt[d][r % nopt[d]] * stride[d];
-                                                r /= nopt[d];
-                                            }
-                                            sig.st(f, Gw);
-                                        }
-                                    }
-                                }
+                        if (!__syncthreads_or(pmask != 0)) break;
+#pragma unroll
+                        for (uint32_t q = 0; q < KS; ++q)
+                            if ((pmask >> q) & 1u) {
+                                const uint32_t L = pw[q] & kIdMask, G = (pw[q] >> 16) & kIdMask;
+                                if (L) atomicMax(&prio[L - 1], ppr[q]);
+                                if (G) atomicMax(&prio[G - 1], ppr[q]);
                             }
+                        __syncthreads();
+                        uint32_t wmask = 0;
+#pragma unroll
+                        for (uint32_t q = 0; q < KS; ++q)
+                            if ((pmask >> q) & 1u) {
+                                const uint32_t L = pw[q] & kIdMask, G = (pw[q] >> 16) & kIdMask;
+                                if ((!L || prio[L - 1] == ppr[q]) && (!G || prio[G - 1] == ppr[q]))
+                                    wmask |= 1u << q;
+                            }
+                        __syncthreads();
+#pragma unroll
+                        for (uint32_t q = 0; q < KS; ++q)
+                            if ((pmask >> q) & 1u) {
+                                const uint32_t L = pw[q] & kIdMask, G = (pw[q] >> 16) & kIdMask;
+                                if (L) prio[L - 1] = 0;
+                                if (G) prio[G - 1] = 0;
+                            }
+                        // one commit per thread and round: a thread's other
+                        // winners bid again and win again next round (nothing
+                        // that shares a cell with them can commit first)
+                        if (wmask) {
+                            const uint32_t qw = (uint32_t)__builtin_ctz(wmask);
+                            uint32_t jw = 0, ww = 0;
+#pragma unroll
+                            for (uint32_t q = 0; q < KS; ++q)
+                                if (q == qw) {
+                                    jw = pj[q];
+                                    ww = pw[q];
+                                }
+                            pmask &= ~(1u << qw);
+                            commit_win(jw, ww & 0xffffu, ww >> 16);
                         }
                         if constexpr (SG) __threadfence();  // sigma writes, HBM
                     }
@@ -1670,317 +1722,70 @@ __global__ void __launch_bounds__(NT) k_sublattice(Rep* __restrict__ reps,
 }
 
 // ===========================================================================
-// Sublattice mode for lattices whose sigma does not fit in LDS: a
-// cooperative launch over several replicas, each owning a contiguous range
-// of `blocks_per_rep` workgroups; all of them work on colour phases in step,
-// with packed sigma words (HBM copy), V, I and the per-cell priority words
-// in HBM.  Same schedule, priority rounds and counter-based randomness as
-// k_sublattice, with grid-wide barriers where that kernel uses workgroup
-// barriers; a replica's batches are as many sites as its workgroups have
-// threads.  Every loop that contains a barrier has the same trip count on
-// every workgroup (max_todo MCS, max_batches batches per phase, rounds
-// until no replica has a pending attempt).  A round costs one barrier: the
-// priority words and the pending counters rotate through three slots, so a
-// slot is cleared two rounds after its bids, when no workgroup can still be
-// reading it.
+// Contact counts recounted from the lattice (sublattice mode keeps no
+// running counts): the pairs of sites that differ, each unordered pair of
+// interior sites once (a halo neighbour counts as the interior site it
+// mirrors), exactly as initialization's recompute counts them
+// (initialization.rs:396-436).  One replica per blockIdx.y; integer counts,
+// so the result does not depend on the order of the atomics.
 // ===========================================================================
-__device__ __forceinline__ void grid_barrier(cooperative_groups::grid_group& grid) {
-    __threadfence();
-    grid.sync();
-    __threadfence();
+__global__ void __launch_bounds__(256) k_contact_clear(Rep* __restrict__ reps,
+                                                       const uint32_t* __restrict__ idx) {
+    Rep& R = reps[idx[blockIdx.y]];
+    const uint64_t n1 = (uint64_t)R.n_cells + 1, total = n1 * n1;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        R.contact[i] = 0;
 }
 
-__device__ __forceinline__ unsigned long long load_agent(const unsigned long long* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-template <int D>
-__global__ void __launch_bounds__(kSlThreads) k_sublattice_grid(
-    Rep* __restrict__ reps, const uint32_t* __restrict__ idx, uint32_t blocks_per_rep,
-    uint64_t max_todo, uint32_t max_batches, unsigned long long* __restrict__ pending_count,
-    const uint64_t* __restrict__ sp3d) {
-    namespace cg = cooperative_groups;
-    cg::grid_group grid = cg::this_grid();
-    __shared__ unsigned long long s_cnt[6];
-    Rep& R = reps[idx[blockIdx.x / blocks_per_rep]];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t gtid = (blockIdx.x % blocks_per_rep) * kSlThreads + tid;
-    const uint32_t gthreads = blocks_per_rep * kSlThreads;
-    const bool leader = blockIdx.x == 0 && tid == 0;
-    const uint32_t T = R.n_types, T1 = T + 1, n_cells = R.n_cells;
-    const uint32_t n_padded = R.n_padded;
-    __attribute__((address_space(1))) uint16_t* sig =
-        (__attribute__((address_space(1))) uint16_t*)R.sig16;
-    __attribute__((address_space(1))) uint32_t* vol =
-        (__attribute__((address_space(1))) uint32_t*)R.volume;
-    __attribute__((address_space(1))) uint32_t* itf =
-        (__attribute__((address_space(1))) uint32_t*)R.interface;
-    unsigned long long* prio = (unsigned long long*)R.scratch;  // [3][n_cells]
-    const double* __restrict__ J = R.params + 2 * T;
-    const double* __restrict__ lv = R.params;
-    const double* __restrict__ li = R.params + T;
-    const uint32_t* __restrict__ tv = R.targets;
-    const uint32_t* __restrict__ ti = R.targets + T;
-    const uint32_t* __restrict__ gtyp = R.type_index;
-    for (uint32_t i = gtid; i < n_padded; i += gthreads) {
-        const uint32_t id = R.sigma[i];
-        sig[i] = (uint16_t)(id ? id | ((gtyp[id - 1] + 1) << 12) : 0u);
-    }
-    for (uint32_t i = gtid; i < 3 * n_cells; i += gthreads) prio[i] = 0;
-    if (leader) pending_count[0] = pending_count[1] = pending_count[2] = 0;
-    if (tid < 6) s_cnt[tid] = 0;
-    grid_barrier(grid);
-
-    uint32_t dims[D], stride[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        dims[d] = R.dims[d];
-        stride[d] = R.strides[d];
-    }
-    const uint32_t n_copy = R.n_copy, n_energy = R.n_energy, n_topo = R.n_topo;
+__global__ void __launch_bounds__(256) k_contact_count(Rep* __restrict__ reps,
+                                                       const uint32_t* __restrict__ idx) {
+    Rep& R = reps[idx[blockIdx.y]];
+    const uint32_t D = R.D;
+    const uint64_t n1 = (uint64_t)R.n_cells + 1;
     const bool periodic = R.boundary == CPM_BOUNDARY_PERIODIC;
-    const uint64_t ncell1 = (uint64_t)n_cells + 1;
-    const uint64_t key = R.fill_state[0];
-    unsigned long long c_att = 0, c_nn = 0, c_top = 0, c_en = 0, c_acc = 0;
-    constexpr uint32_t NC = 1u << D;
-    uint32_t round_no = 0;
-    // cells this thread bid for in each of the last three rounds (0: none),
-    // indexed by round % 3, cleared from their slot two rounds later
-    uint32_t owe_l[3] = {0, 0, 0}, owe_g[3] = {0, 0, 0};
-
-    const uint64_t todo = R.todo;
-    for (uint64_t m = 0; m < max_todo; ++m) {
-        const uint64_t mcs = R.mcs + m;
-        for (uint32_t sub = 0; sub < kSlSub; ++sub) {
-            const uint64_t phase0 = (mcs * kSlSub + sub) * NC;
-            uint32_t order[NC];
-            for (uint32_t c = 0; c < NC; ++c) order[c] = c;
-            uint64_t hperm = splitmix_at(key, ~phase0);
-            for (uint32_t c = NC - 1; c > 0; --c) {
-                const uint32_t j = (uint32_t)((hperm & 0xffff) % (c + 1));
-                hperm >>= 16;
-                const uint32_t tmp = order[c];
-                order[c] = order[j];
-                order[j] = tmp;
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < R.n_sites;
+         q += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t c[3] = {0, 0, 0};
+        uint64_t rem = q, fi = 0;
+        for (int d = (int)D - 1; d >= 0; --d) {
+            c[d] = (uint32_t)(rem % R.dims[d]);
+            rem /= R.dims[d];
+            fi += (uint64_t)(c[d] + 1) * R.strides[d];
+        }
+        const uint32_t ci = R.sigma[fi];
+        for (uint32_t k = 0; k < R.n_energy; ++k) {
+            // the offset's components (each -1, 0 or 1: the flat offset
+            // rounded per axis), the neighbour's padded coordinates and the
+            // interior site it is or mirrors
+            int32_t o[3] = {0, 0, 0};
+            int64_t off = R.energy_off[k];
+            uint64_t fj = (uint64_t)((int64_t)fi + off), key = 0;
+            bool interior = true;
+            for (int d = 0; d < (int)D; ++d) {
+                const int64_t st = R.strides[d];
+                int64_t od = off >= 0 ? (off + st / 2) / st : -((-off + st / 2) / st);
+                o[d] = (int32_t)od;
+                off -= od * st;
             }
-            for (uint32_t ci = 0; ci < NC; ++ci) {
-                const uint32_t col = order[ci];
-                const uint64_t phase = phase0 + col;
-                uint32_t half[D], nsite = 1;
-#pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    const uint32_t cd = (col >> d) & 1u;
-                    half[d] = dims[d] > cd ? (dims[d] - cd + 1) / 2 : 0;
-                    nsite *= half[d];
+            for (int d = 0; d < (int)D; ++d) {
+                const int64_t p = (int64_t)c[d] + 1 + o[d];  // padded coordinate
+                uint64_t w = (uint64_t)p;
+                if (p == 0 || p == (int64_t)R.dims[d] + 1) {
+                    interior = false;
+                    w = p == 0 ? R.dims[d] : 1;  // the mirrored interior coordinate
                 }
-                const uint32_t rot =
-                    nsite ? (uint32_t)(splitmix_at(key ^ 0x5bd1e995ull, phase) % nsite) : 0u;
-                for (uint32_t bi = 0; bi < max_batches; ++bi) {
-                    const uint32_t jb = bi * gthreads + gtid;
-                    uint32_t j = 0, c[D], t = 0, Lw = 0, Gw = 0;
-                    unsigned long long pr = 0;
-                    uint64_t ctr = 0;
-                    bool pending = false;
-                    if (m < todo && jb < nsite) {
-                        j = jb + rot;
-                        if (j >= nsite) j -= nsite;
-                        uint32_t rem = j;
-#pragma unroll
-                        for (int d = D - 1; d >= 0; --d) {
-                            c[d] = 2 * (rem % half[d]) + ((col >> d) & 1u);
-                            rem /= half[d];
-                            t += (c[d] + 1) * stride[d];
-                        }
-                        ctr = (phase * (uint64_t)R.n_sites + j) * 4;
-                        const uint64_t h0 = splitmix_at(key, ctr);
-                        if ((h0 & (kSlSub - 1)) == 0) {
-                            const uint32_t k = (uint32_t)(((h0 >> 32) * (uint64_t)n_copy) >> 32);
-                            const uint32_t s = (uint32_t)((int32_t)t + R.copy_off[k]);
-                            c_att++;
-                            Lw = sig[t];
-                            Gw = sig[s];
-                            if (Lw != Gw) {
-                                c_nn++;
-                                const uint32_t L = Lw & kIdMask;
-                                bool simple = true;
-                                if (L != 0) {
-                                    uint32_t pat = 0;
-                                    for (uint32_t i = 0; i < n_topo; ++i)
-                                        pat |= (uint32_t)((sig[(int32_t)t + R.topo_off[i]] & kIdMask) == L) << i;
-                                    c_top++;
-                                    simple = D == 2 ? ((c_sp2d[pat >> 5] >> (pat & 31)) & 1u)
-                                                    : ((sp3d[pat >> 6] >> (pat & 63)) & 1ull);
-                                }
-                                if (simple) {
-                                    pending = true;
-                                    pr = 0x8000000000000000ull |
-                                         (splitmix_at(key, ctr + 1) & 0x7fffffffff000000ull) | j;
-                                }
-                            }
-                        }
-                    }
-                    for (;;) {
-                        const uint32_t L = Lw & kIdMask, G = Gw & kIdMask;
-                        const uint32_t slot = round_no % 3, next = (round_no + 1) % 3;
-                        unsigned long long* P = prio + (size_t)slot * n_cells;
-                        {   // clear this thread's bids of two rounds ago (slot `next`)
-                            unsigned long long* Q = prio + (size_t)next * n_cells;
-                            if (owe_l[next]) Q[owe_l[next] - 1] = 0;
-                            if (owe_g[next]) Q[owe_g[next] - 1] = 0;
-                            owe_l[next] = owe_g[next] = 0;
-                        }
-                        if (pending) {
-                            if (L) atomicMax(&P[L - 1], pr);
-                            if (G) atomicMax(&P[G - 1], pr);
-                            atomicAdd(&pending_count[slot], 1ull);
-                            owe_l[slot] = L;
-                            owe_g[slot] = G;
-                        }
-                        if (leader) pending_count[next] = 0;
-                        grid_barrier(grid);
-                        const unsigned long long left = load_agent(&pending_count[slot]);
-                        ++round_no;
-                        if (left == 0) {
-                            // clear the previous round's bids before their
-                            // slot comes round again (a barrier follows)
-                            const uint32_t prev = (slot + 2) % 3;
-                            unsigned long long* Q = prio + (size_t)prev * n_cells;
-                            if (owe_l[prev]) Q[owe_l[prev] - 1] = 0;
-                            if (owe_g[prev]) Q[owe_g[prev] - 1] = 0;
-                            owe_l[prev] = owe_g[prev] = 0;
-                            break;
-                        }
-                        const bool win = pending && (!L || load_agent(&P[L - 1]) == pr) &&
-                                         (!G || load_agent(&P[G - 1]) == pr);
-                        if (win) {
-                            pending = false;
-                            const uint32_t vL = L ? vol[L - 1] : 0u;
-                            if (L == 0 || vL > 1) {
-                                c_en++;
-                                const uint32_t lidx = Lw >> 12, gidx = Gw >> 12;
-                                const uint32_t tl = lidx ? lidx - 1 : 0u, tg = gidx ? gidx - 1 : 0u;
-                                int nl = 0, ng = 0;
-                                double dadh = 0.0;  // energy.rs:242-264, stencil order
-                                for (uint32_t q = 0; q < n_energy; ++q) {
-                                    const uint32_t w = sig[(int32_t)t + R.energy_off[q]];
-                                    const uint32_t nb = w & kIdMask, ni = w >> 12;
-                                    nl += nb == L;
-                                    ng += nb == G;
-                                    const double before = nb != L ? J[lidx * T1 + ni] : 0.0;
-                                    const double after = nb != G ? J[gidx * T1 + ni] : 0.0;
-                                    dadh += 1.0 * (after - before);
-                                }
-                                const int dl_i = 2 * nl - (int)n_energy, dg_i = (int)n_energy - 2 * ng;
-                                double dvol = 0.0, ditf = 0.0;
-                                if (L) {
-                                    const double v = (double)vL;
-                                    const double x1 = (v - 1.0) - (double)tv[tl], x0 = v - (double)tv[tl];
-                                    dvol += lv[tl] * (x1 * x1) - lv[tl] * (x0 * x0);
-                                    const double i0 = (double)itf[L - 1];
-                                    const double y1 = (i0 + (double)dl_i) - (double)ti[tl], y0 = i0 - (double)ti[tl];
-                                    ditf += li[tl] * (y1 * y1) - li[tl] * (y0 * y0);
-                                }
-                                if (G) {
-                                    const double v = (double)vol[G - 1];
-                                    const double x1 = (v + 1.0) - (double)tv[tg], x0 = v - (double)tv[tg];
-                                    dvol += lv[tg] * (x1 * x1) - lv[tg] * (x0 * x0);
-                                    const double i0 = (double)itf[G - 1];
-                                    const double y1 = (i0 + (double)dg_i) - (double)ti[tg], y0 = i0 - (double)ti[tg];
-                                    ditf += li[tg] * (y1 * y1) - li[tg] * (y0 * y0);
-                                }
-                                const double dh = dvol + ditf + dadh;
-                                bool accept = dh <= 0.0;
-                                if (!accept) {
-                                    const uint64_t hu = splitmix_at(key, ctr + 2);
-                                    const double u = (double)(hu >> 11) * (1.0 / 9007199254740992.0);
-                                    accept = u < exp(-dh);
-                                }
-                                if (accept) {
-                                    c_acc++;
-                                    for (uint32_t q = 0; q < n_energy; ++q) {
-                                        const uint32_t nb = sig[(int32_t)t + R.energy_off[q]] & kIdMask;
-                                        if (nb != L)
-                                            atomicSub(&R.contact[(uint64_t)min(L, nb) * ncell1 + max(L, nb)], 1u);
-                                        if (nb != G)
-                                            atomicAdd(&R.contact[(uint64_t)min(G, nb) * ncell1 + max(G, nb)], 1u);
-                                    }
-                                    sig[t] = (uint16_t)Gw;
-                                    if (L) {
-                                        vol[L - 1] = vL - 1;
-                                        itf[L - 1] = (uint32_t)((int32_t)itf[L - 1] + dl_i);
-                                        int64_t* sp = R.sum_pos + (uint64_t)(L - 1) * D;
-                                        int64_t* so = R.sum_outer + (uint64_t)(L - 1) * D * D;
-                                        for (int d = 0; d < D; ++d) sp[d] -= (int64_t)c[d];
-                                        for (int d = 0; d < D; ++d)
-                                            for (int e2 = 0; e2 < D; ++e2)
-                                                so[d * D + e2] -= (int64_t)c[d] * (int64_t)c[e2];
-                                    }
-                                    if (G) {
-                                        const uint32_t vg_new = vol[G - 1] + 1;
-                                        vol[G - 1] = vg_new;
-                                        itf[G - 1] = (uint32_t)((int32_t)itf[G - 1] + dg_i);
-                                        int64_t* sp = R.sum_pos + (uint64_t)(G - 1) * D;
-                                        int64_t* so = R.sum_outer + (uint64_t)(G - 1) * D * D;
-                                        const uint32_t vbg = max(vg_new - 1, 1u);
-                                        int64_t img[D];
-                                        for (int d = 0; d < D; ++d) {
-                                            const double cen = (double)sp[d] / (double)vbg;
-                                            int64_t best = (int64_t)c[d];
-                                            if (periodic) {
-                                                double bd = fabs((double)best - cen);
-                                                const int64_t lo = (int64_t)c[d] - (int64_t)dims[d];
-                                                const int64_t hi = (int64_t)c[d] + (int64_t)dims[d];
-                                                const double dlo = fabs((double)lo - cen);
-                                                if (dlo < bd) {
-                                                    best = lo;
-                                                    bd = dlo;
-                                                }
-                                                if (fabs((double)hi - cen) < bd) best = hi;
-                                            }
-                                            img[d] = best;
-                                        }
-                                        for (int d = 0; d < D; ++d) sp[d] += img[d];
-                                        for (int d = 0; d < D; ++d)
-                                            for (int e2 = 0; e2 < D; ++e2) so[d * D + e2] += img[d] * img[e2];
-                                    }
-                                    if (periodic) {
-                                        uint32_t opt[D][3], nopt[D], total = 1;
-                                        for (int d = 0; d < D; ++d) {
-                                            nopt[d] = 1;
-                                            opt[d][0] = c[d] + 1;
-                                            if (c[d] == 0) opt[d][nopt[d]++] = dims[d] + 1;
-                                            if (c[d] == dims[d] - 1) opt[d][nopt[d]++] = 0;
-                                            total *= nopt[d];
-                                        }
-                                        for (uint32_t q = 1; q < total; ++q) {
-                                            uint32_t r = q, f = 0;
-                                            for (int d = D - 1; d >= 0; --d) {
-                                                f += opt[d][r % nopt[d]] * stride[d];
-                                                r /= nopt[d];
-                                            }
-                                            sig[f] = (uint16_t)Gw;
-                                        }
-                                    }
-                                }
-                            }
-                        }
-                    }
-                }
+                key += w * R.strides[d];
+            }
+            if (interior) key = fj;
+            else if (!periodic) key = ~0ull;  // Fixed: a distinct, permanently medium site
+            if (key != ~0ull && key <= fi) continue;
+            const uint32_t cj = R.sigma[fj];
+            if (cj != ci) {
+                const uint32_t lo = min(ci, cj), hi = max(ci, cj);
+                atomicAdd(&R.contact[(uint64_t)lo * n1 + hi], 1u);
             }
         }
-    }
-    atomicAdd(&s_cnt[0], c_att);
-    atomicAdd(&s_cnt[1], c_nn);
-    atomicAdd(&s_cnt[2], c_top);
-    atomicAdd(&s_cnt[3], c_en);
-    atomicAdd(&s_cnt[4], c_acc);
-    __syncthreads();
-    if (tid < 5) atomicAdd((unsigned long long*)&R.cnt[tid], s_cnt[tid]);
-    for (uint32_t i = gtid; i < n_padded; i += gthreads) R.sigma[i] = sig[i] & kIdMask;
-    if (gtid == 0) {
-        R.mcs += todo;
-        R.todo = 0;
-        R.done = 1;
     }
 }
 
@@ -2086,10 +1891,6 @@ struct Engine {
     std::vector<hipEvent_t> launch_ev;  // start/stop pairs around replay launches
     // autotuned sigma placement: batch signature -> best candidate
     std::map<uint64_t, int> tuned_placement;
-    uint32_t* grid_idx = nullptr;       // grid sublattice: replica per workgroup range
-    size_t grid_idx_cap = 0;
-    unsigned long long* grid_counters = nullptr;  // grid sublattice: round counters
-    size_t grid_counters_cap = 0;
     // Streams for state copies to the host, shared by the host threads that
     // sync replicas in parallel: on the null stream their copies would run
     // one after another, and a stream of each thread's own would be created
@@ -2154,8 +1955,9 @@ Engine* engine_create(int device) {
     HIP_OK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device));
     e->n_cu = (uint32_t)n_cu;
 #define CPM_FN(k, d, ne, gs, pw, tb) (const void*)k_lds_engine<d, ne, gs, pw, tb>,
-    for (auto fn : {CPM_WAVE_KERNELS(CPM_FN)(const void*)k_sublattice<2, false, kSlThreads>,
-                    (const void*)k_sublattice<3, false, kSlThreads>})
+#define CPM_SLFN(k, d, sg, nt) (const void*)k_sublattice<d, sg, nt>,
+    for (auto fn : {CPM_WAVE_KERNELS(CPM_FN) CPM_SL_KERNELS(CPM_SLFN)})
+#undef CPM_SLFN
 #undef CPM_FN
         HIP_OK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    lds - kLdsStaticReserve));
@@ -2196,8 +1998,6 @@ void engine_destroy(Engine* e) {
     if (e->sp3d) (void)hipFree(e->sp3d);
     if (e->reps_dev) (void)hipFree(e->reps_dev);
     if (e->idx_dev) (void)hipFree(e->idx_dev);
-    if (e->grid_idx) (void)hipFree(e->grid_idx);
-    if (e->grid_counters) (void)hipFree(e->grid_counters);
     if (e->ev0) (void)hipEventDestroy(e->ev0);
     if (e->ev1) (void)hipEventDestroy(e->ev1);
     for (int i = 0; i < Engine::kClassStreams; ++i) {
@@ -2229,7 +2029,7 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct Layout {
     size_t sigma, type_index, volume, interface, sum_pos, sum_outer, contact, params, targets,
-        draws, sig16, scratch, total;
+        draws, sig16, total;
 };
 
 static Layout layout_of(const Model& m, bool with_draws) {
@@ -2252,7 +2052,6 @@ static Layout layout_of(const Model& m, bool with_draws) {
     L.targets = take(2 * T * 4);
     L.draws = with_draws ? take((size_t)kFill * 8 + (size_t)kSegs * 32) : 0;
     L.sig16 = with_draws ? take(m.n_padded * 2) : 0;
-    L.scratch = take(m.n_cells * 24 + 16);  // grid sublattice: 3 priority slots
     L.total = off;
     return L;
 }
@@ -2346,7 +2145,6 @@ static void upload(Engine* e, Model* m) {
     R.targets = (const uint32_t*)(d->base + L.targets);
     R.draws = lds ? (uint64_t*)(d->base + L.draws) : nullptr;
     R.sig16 = lds ? (uint16_t*)(d->base + L.sig16) : nullptr;
-    R.scratch = (unsigned long long*)(d->base + L.scratch);
     const size_t T = R.n_types;
     std::vector<double> params(2 * T + (T + 1) * (T + 1));
     std::vector<uint32_t> targets(2 * T);
@@ -2454,6 +2252,24 @@ void sync_to_host(Model* m) {
     put(m->sum_outer.data(), L.sum_outer, n * D * D * 8);
     if (!contact_apart) put(m->contact.data(), L.contact, contact_bytes);
     m->device_dirty = false;
+}
+
+// The lattice, type indices, volumes and interfaces only (adjacent at the
+// start of the allocation): what the energy needs.  The rest of the host
+// copy stays stale (device_dirty unchanged) until a full sync.
+static void sync_lattice_tables(Model* m) {
+    if (!m->dev || !m->device_dirty) return;
+    const Layout L = layout_of(*m, m->dev->lds);
+    const size_t n = m->n_cells;
+    std::vector<uint8_t> stage(L.interface + n * 4 - L.sigma);
+    hipStream_t st = copy_stream_for(m->dev->eng);
+    HIP_OK(hipStreamWaitEvent(st, m->dev->eng->work_done, 0));
+    HIP_OK(hipMemcpyAsync(stage.data(), m->dev->base + L.sigma, stage.size(),
+                          hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    std::memcpy(m->sigma.data(), stage.data(), m->n_padded * 4);
+    std::memcpy(m->volume.data(), stage.data() + (L.volume - L.sigma), n * 4);
+    std::memcpy(m->interface.data(), stage.data() + (L.interface - L.sigma), n * 4);
 }
 
 void sync_scalars(Model*) {}  // scalars are written back after every call
@@ -2840,8 +2656,9 @@ static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint
         ensure_buf(e->reps_dev, e->reps_cap, n);
         ensure_buf(e->idx_dev, e->idx_cap, n);
         std::vector<Rep> reps(n);
+        // (kernel key, LDS class) -> replicas; key = D * 100 + 10 for the big
+        // workgroups, + 1 for sigma in HBM
         std::map<std::pair<int, uint32_t>, std::vector<uint32_t>> classes;
-        std::vector<uint32_t> grid_path;  // lattices of kSlGridSites sites and more
         for (size_t i = 0; i < n; ++i) {
             Model* m = models[i];
             Rep R = m->dev->rep;
@@ -2859,10 +2676,12 @@ static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint
                 R.fill_state[0] = m->sublattice_key;
                 bool sg = false;
                 const uint32_t bytes = sublattice_lds(e, *m, &sg);
-                if (m->n_sites < kSlGridSites && !sg && bytes + kLdsStaticReserve <= e->max_lds)
-                    classes[{m->D * 100, lds_class(e, bytes)}].push_back((uint32_t)i);
-                else
-                    grid_path.push_back((uint32_t)i);
+                if (bytes + kLdsStaticReserve > e->max_lds)
+                    throw Error(CPM_ERR_ARG,
+                                "sublattice mode: the per-cell tables do not fit in LDS");
+                const bool big = sg || m->n_sites >= kSlBigSites;
+                const int key = m->D * 100 + (big ? 10 : 0) + (sg ? 1 : 0);
+                classes[{key, lds_class(e, bytes)}].push_back((uint32_t)i);
             }
             reps[i] = R;
         }
@@ -2884,66 +2703,22 @@ static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint
 #define CPM_SL(k, d, sg, nt)                                                                 \
     case k: hipLaunchKernelGGL((k_sublattice<d, sg, nt>), g, dim3(nt), lds, e->stream, rp, ix, \
                                e->sp3d); break;
-                CPM_SL(200, 2, false, kSlThreads) CPM_SL(300, 3, false, kSlThreads)
+                CPM_SL_KERNELS(CPM_SL)
 #undef CPM_SL
                 default: throw Error(CPM_ERR_DEVICE, "no sublattice kernel for this key");
             }
             HIP_OK(hipGetLastError());
             first += (uint32_t)v.size();
         }
-        // large replicas: cooperative launches per dimension, each replica
-        // owning an equal range of co-resident workgroups
-        if (!grid_path.empty()) {
-            ensure_buf(e->grid_counters, e->grid_counters_cap, 3);
-            for (int D : {2, 3}) {
-                std::vector<uint32_t> sel;
-                for (uint32_t i : grid_path)
-                    if (models[i]->D == D) sel.push_back(i);
-                if (sel.empty()) continue;
-                const void* fn = D == 2 ? (const void*)k_sublattice_grid<2>
-                                        : (const void*)k_sublattice_grid<3>;
-                int per_cu = 0;
-                HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kSlThreads, 0));
-                const uint64_t resident = (uint64_t)std::max(per_cu, 1) * e->n_cu;
-                // every replica covers a whole colour phase in one batch
-                // (its trajectory then does not depend on the launch it is
-                // in), so a launch takes as many replicas as fit resident
-                uint64_t max_sites = 1;
-                for (uint32_t i : sel) {
-                    uint64_t sites = 1;
-                    for (int d = 0; d < D; ++d) sites *= (models[i]->dims[d] + 1) / 2;
-                    max_sites = std::max(max_sites, sites);
-                }
-                const uint32_t bpr = (uint32_t)std::max<uint64_t>(
-                    1, std::min<uint64_t>((max_sites + kSlThreads - 1) / kSlThreads, resident));
-                const size_t per_launch = std::max<size_t>(1, resident / bpr);
-                for (size_t at = 0; at < sel.size();) {
-                    const size_t nr = std::min<size_t>(sel.size() - at, per_launch);
-                    uint64_t max_todo = 0;
-                    for (size_t k = at; k < at + nr; ++k)
-                        max_todo = std::max<uint64_t>(max_todo, todo[sel[k]]);
-                    uint32_t max_batches =
-                        (uint32_t)((max_sites + (uint64_t)bpr * kSlThreads - 1) /
-                                   ((uint64_t)bpr * kSlThreads));
-                    // one index entry per replica; workgroups map to it by division
-                    std::vector<uint32_t> ids(sel.begin() + at, sel.begin() + at + nr);
-                    ensure_buf(e->grid_idx, e->grid_idx_cap, ids.size());
-                    HIP_OK(hipMemcpyAsync(e->grid_idx, ids.data(), ids.size() * 4,
-                                          hipMemcpyHostToDevice, e->stream));
-                    Rep* reps_arg = e->reps_dev;
-                    const uint32_t* idx_arg = e->grid_idx;
-                    unsigned long long* cnt_arg = e->grid_counters;
-                    const uint64_t* sp3d = e->sp3d;
-                    void* args[] = {(void*)&reps_arg, (void*)&idx_arg, (void*)&bpr,
-                                    (void*)&max_todo, (void*)&max_batches, (void*)&cnt_arg,
-                                    (void*)&sp3d};
-                    HIP_OK(hipLaunchCooperativeKernel(fn, dim3((uint32_t)(nr * bpr)),
-                                                      dim3(kSlThreads), args, 0, e->stream));
-                    // the next launch reuses the index buffer
-                    HIP_OK(hipStreamSynchronize(e->stream));
-                    at += nr;
-                }
-            }
+        // contact counts of every replica that ran (at most 65535 per launch)
+        for (size_t at = 0; at < order.size(); at += 65535) {
+            const dim3 g(64, (uint32_t)std::min<size_t>(order.size() - at, 65535));
+            hipLaunchKernelGGL(k_contact_clear, g, dim3(256), 0, e->stream, e->reps_dev,
+                               e->idx_dev + at);
+            HIP_OK(hipGetLastError());
+            hipLaunchKernelGGL(k_contact_count, g, dim3(256), 0, e->stream, e->reps_dev,
+                               e->idx_dev + at);
+            HIP_OK(hipGetLastError());
         }
         HIP_OK(hipEventRecord(e->ev1, e->stream));
         HIP_OK(hipEventRecord(e->work_done, e->stream));
@@ -2966,7 +2741,7 @@ static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint
     // launch)
     parallel_for(n, [&](size_t i) {
         if (!todo[i]) return;
-        sync_to_host(models[i]);
+        sync_lattice_tables(models[i]);
         models[i]->energy = models[i]->global_energy();
     });
 }
