@@ -72,17 +72,12 @@ void run_lockstep(Engine* e, std::vector<Model*>& ms, bool include_lattice, int 
     std::vector<std::vector<double>> traces(n);
     out.assign(n, Result());
     std::vector<uint64_t> todo(n);
-    // the readouts of step k run on a second host thread while the device
-    // runs step k + 1; they read the host copies synced after step k (with
-    // that step's MCS and energy), so the next sync waits for them
+    // the readouts of step k are made on the device right after step k
+    // (before step k + 1 changes the state); a second host thread appends
+    // them to the results while the device runs step k + 1
     std::future<void> readouts;
     for (size_t k = 0; k < steps; ++k) {
         for (size_t i = 0; i < n; ++i) todo[i] = k < sch[i].size() ? sch[i][k].mcs : 0;
-        // a sublattice-mode call writes the host copies of its replicas when
-        // it ends (the energy is recomputed from them), so the readouts of
-        // step k must be done before it starts; an exact-replay call leaves
-        // the host copies alone until the next sync
-        if (mode == CPM_MODE_SUBLATTICE && readouts.valid()) readouts.get();
         run_mcs_each(e, ms.data(), n, todo.data(), mode, counters);
         if (readouts.valid()) readouts.get();
         std::vector<size_t> sampled;
@@ -97,16 +92,17 @@ void run_lockstep(Engine* e, std::vector<Model*>& ms, bool include_lattice, int 
                 at_energy.push_back(ms[i]->energy);
             }
         }
-        parallel_for(sampled.size(), [&](size_t j) { sync_to_host(ms[sampled[j]]); });
-        readouts = std::async(std::launch::async, [&out, &ms, include_lattice,
-                                                   sampled = std::move(sampled),
-                                                   at_mcs = std::move(at_mcs),
-                                                   at_energy = std::move(at_energy)] {
-            parallel_for(sampled.size(), [&](size_t j) {
-                const size_t i = sampled[j];
-                out[i].samples.push_back(
-                    take_sample(*ms[i], include_lattice, at_mcs[j], at_energy[j]));
-            });
+        // readouts on the device (labelling, shape); the samples are
+        // appended on a second host thread while the device runs step k + 1
+        std::vector<Model*> sm(sampled.size());
+        for (size_t j = 0; j < sampled.size(); ++j) sm[j] = ms[sampled[j]];
+        std::vector<Sample> smp(sampled.size());
+        device_samples(e, sm.data(), sm.size(), include_lattice, at_mcs.data(), at_energy.data(),
+                       smp.data());
+        readouts = std::async(std::launch::async, [&out, sampled = std::move(sampled),
+                                                   smp = std::move(smp)]() mutable {
+            for (size_t j = 0; j < sampled.size(); ++j)
+                out[sampled[j]].samples.push_back(std::move(smp[j]));
         });
     }
     if (readouts.valid()) readouts.get();
@@ -290,9 +286,16 @@ cpm_status cpm_model_sample(cpm_model* model, int32_t include_lattice, cpm_resul
         need(model && out, "model/out");
         Model* m = model->m;
         need(m->initialized, "model must be initialized before cpm_model_sample");
-        sync_to_host(m);
         Result r;
-        r.samples.push_back(take_sample(*m, include_lattice != 0));
+        r.samples.emplace_back();
+        if (Engine* e = device_engine(m)) {  // resident: readouts on the device
+            const uint64_t mcs = m->mcs;
+            const double energy = m->energy;
+            device_samples(e, &m, 1, include_lattice != 0, &mcs, &energy, &r.samples[0]);
+        } else {
+            r.samples[0] = take_sample(*m, include_lattice != 0);
+        }
+        sync_to_host(m);
         finish_result(*m, {}, r);
         *out = new cpm_result{std::move(r)};
     });

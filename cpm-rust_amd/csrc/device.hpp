@@ -11,6 +11,7 @@
 namespace cpm {
 
 struct Model;
+struct Sample;
 struct Engine;
 
 Engine* engine_create(int device);
@@ -35,6 +36,12 @@ void run_mcs_noalloc(Engine* e, Model* const* models, size_t n, uint64_t n_mcs,
 // One model per launch, one launch per MCS.
 void run_mcs_plain(Engine* e, Model* m, uint64_t n_mcs, cpm_counters* counters);
 
+// take_sample (output.rs:289-337) of each model's device-resident state,
+// readouts computed on the device (at_mcs / at_energy name the state).
+void device_samples(Engine* e, Model* const* models, size_t n, bool include_lattice,
+                    const uint64_t* at_mcs, const double* at_energy, Sample* out);
+// The engine a model is resident on (nullptr: host only).
+Engine* device_engine(const Model* m);
 // Bring the host copy of the state up to date (no-op when not stale).
 void sync_to_host(Model* m);
 // Scalars only: energy, rng, mcs.

@@ -73,6 +73,9 @@ struct Rep {
     const uint32_t* targets;  // V*[T], I*[T]
     uint64_t* draws;          // kFill buffered draws + kSegs segment states (wave path)
     uint16_t* sig16;          // packed sigma in HBM (wave path with sigma outside LDS)
+    uint32_t* labels;         // readouts: union-find parent per padded site
+    uint32_t* lcount;         // readouts: per-root counts (medium volume | touch << 31)
+    unsigned char* readout;   // readouts: the sample's device part (ReadoutLayout)
     uint64_t rng[4];          // consumer state at the start / end of a call
     double energy;
     uint64_t mcs;
@@ -1722,6 +1725,242 @@ __global__ void __launch_bounds__(NT) k_sublattice(Rep* __restrict__ reps,
 }
 
 // ===========================================================================
+// Readouts on the device (output.rs:147-337, labelling.rs:70-154): the
+// connected components of the lattice and the per-cell shape, written into
+// the replica's readout area and copied to the host once per sample.
+//
+// Components: a lock-free union-find over the interior sites (each link
+// points from the larger root to the smaller, so a component's root is its
+// first site in row-major order — the order in which the reference's scan
+// meets components), cells joined under von Neumann connectivity, medium
+// under the full shell, across the periodic wrap where there is one.  A cell
+// with more than one root is fragmented; each medium root gets its
+// component's volume and whether it touches a fixed boundary.  Shape: the
+// centroid and gyration tensor of every cell from its position sums, and
+// in 2D the eigenvalues, R_g^2 and kappa^2 with the reference's closed
+// form (+, -, *, /, sqrt only, so the same bits as the host's); the 3D
+// closed form needs acos and cos and is finished on the host with the same
+// libm as the reference.
+// ===========================================================================
+struct ReadoutLayout {
+    uint32_t cells;    // per cell: centroid[D], tensor[D*D], eigen[D], rg2, kappa (doubles)
+    uint32_t header;   // n_fragmented, n_medium (u32)
+    uint32_t frag;     // fragmented ids (n_cells u32)
+    uint32_t medium;   // medium components (n_sites u32: volume | touches << 31)
+    uint32_t ncomp;    // components per cell id (n_cells + 1 u32; not copied back)
+    uint32_t total;
+};
+__host__ __device__ inline uint32_t readout_doubles_per_cell(uint32_t D) {
+    return D + D * D + D + 2;
+}
+__host__ __device__ inline ReadoutLayout readout_layout(uint32_t D, uint32_t n_cells,
+                                                        uint32_t n_sites) {
+    ReadoutLayout L;
+    L.cells = 0;
+    L.header = n_cells * readout_doubles_per_cell(D) * 8;
+    L.frag = L.header + 8;
+    L.medium = L.frag + n_cells * 4;
+    L.ncomp = L.medium + n_sites * 4;
+    L.total = L.ncomp + (n_cells + 1) * 4;
+    return L;
+}
+
+__device__ __forceinline__ uint32_t uf_load(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ uint32_t uf_find(uint32_t* lab, uint32_t x) {
+    for (uint32_t p = uf_load(&lab[x]); p != x; p = uf_load(&lab[x])) x = p;
+    return x;
+}
+__device__ void uf_unite(uint32_t* lab, uint32_t a, uint32_t b) {
+    a = uf_find(lab, a);
+    b = uf_find(lab, b);
+    while (a != b) {
+        if (a < b) {
+            const uint32_t t = a;
+            a = b;
+            b = t;
+        }
+        // link the larger root a under b if a is still a root
+        const uint32_t old = atomicCAS(&lab[a], a, b);
+        if (old == a) return;
+        a = uf_find(lab, old);
+        b = uf_find(lab, b);
+    }
+}
+
+// Interior site q (row-major, last axis fastest) -> coordinates and padded flat index.
+__device__ __forceinline__ uint32_t interior_flat(const Rep& R, uint32_t q, uint32_t* c) {
+    uint32_t f = 0;
+    for (int d = (int)R.D - 1; d >= 0; --d) {
+        c[d] = q % R.dims[d];
+        q /= R.dims[d];
+        f += (c[d] + 1) * R.strides[d];
+    }
+    return f;
+}
+// The neighbour of interior site (c, flat x) along flat offset `off`
+// (components -1, 0, 1): its padded flat index, or its interior mirror
+// under a periodic boundary; false if it lies outside a fixed box.
+__device__ __forceinline__ bool neighbour_site(const Rep& R, const uint32_t* c, int64_t off,
+                                               uint32_t* out) {
+    int64_t rem = off, f = 0;
+    bool inside = true;
+    for (int d = 0; d < (int)R.D; ++d) {
+        const int64_t st = R.strides[d];
+        const int64_t od = rem >= 0 ? (rem + st / 2) / st : -((-rem + st / 2) / st);
+        rem -= od * st;
+        int64_t p = (int64_t)c[d] + od;  // interior coordinate
+        if (p < 0 || p >= (int64_t)R.dims[d]) {
+            inside = false;
+            p = p < 0 ? (int64_t)R.dims[d] - 1 : 0;
+        }
+        f += (p + 1) * st;
+    }
+    *out = (uint32_t)f;
+    return inside || R.boundary == CPM_BOUNDARY_PERIODIC;
+}
+
+constexpr uint32_t kRoThreads = 256;
+__global__ void __launch_bounds__(kRoThreads) k_readout(Rep* __restrict__ reps,
+                                                        const uint32_t* __restrict__ idx) {
+    __shared__ uint32_t s_wave[kRoThreads / 64];
+    __shared__ uint32_t s_base;
+    Rep& R = reps[idx[blockIdx.x]];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t D = R.D, n_sites = R.n_sites, n_cells = R.n_cells, n_padded = R.n_padded;
+    const bool periodic = R.boundary == CPM_BOUNDARY_PERIODIC;
+    const uint32_t* __restrict__ sig = R.sigma;
+    uint32_t* lab = R.labels;
+    uint32_t* cnt = R.lcount;
+    const ReadoutLayout Lo = readout_layout(D, n_cells, n_sites);
+    uint32_t* ncomp = (uint32_t*)(R.readout + Lo.ncomp);  // [n_cells + 1]
+    for (uint32_t x = tid; x < n_padded; x += kRoThreads) {
+        lab[x] = x;
+        cnt[x] = 0;
+    }
+    for (uint32_t i = tid; i <= n_cells; i += kRoThreads) ncomp[i] = 0;
+    __threadfence();
+    __syncthreads();
+    // union over each edge once: forward offsets (positive flat delta)
+    for (uint32_t q = tid; q < n_sites; q += kRoThreads) {
+        uint32_t c[3];
+        const uint32_t x = interior_flat(R, q, c), o = sig[x];
+        if (o == 0) {
+            for (uint32_t k = 0; k < R.n_topo; ++k) {
+                const int32_t off = R.topo_off[k];
+                uint32_t y;
+                if (off > 0 && neighbour_site(R, c, off, &y) && sig[y] == 0) uf_unite(lab, x, y);
+            }
+        } else {
+            for (uint32_t d = 0; d < D; ++d) {
+                uint32_t y;
+                if (neighbour_site(R, c, (int64_t)R.strides[d], &y) && sig[y] == o)
+                    uf_unite(lab, x, y);
+            }
+        }
+    }
+    __threadfence();
+    __syncthreads();
+    for (uint32_t q = tid; q < n_sites; q += kRoThreads) {
+        uint32_t c[3];
+        const uint32_t x = interior_flat(R, q, c);
+        lab[x] = uf_find(lab, x);
+    }
+    __threadfence();
+    __syncthreads();
+    for (uint32_t q = tid; q < n_sites; q += kRoThreads) {
+        uint32_t c[3];
+        const uint32_t x = interior_flat(R, q, c), o = sig[x], r = uf_load(&lab[x]);
+        if (o == 0) {
+            atomicAdd(&cnt[r], 1u);
+            bool edge = false;
+            for (uint32_t d = 0; d < D; ++d) edge |= c[d] == 0 || c[d] == R.dims[d] - 1;
+            if (!periodic && edge) atomicOr(&cnt[r], 0x80000000u);
+        } else if (r == x) {
+            atomicAdd(&ncomp[o], 1u);
+        }
+    }
+    __threadfence();
+    __syncthreads();
+    // ordered compaction: medium roots by site, then fragmented cells by id
+    uint32_t* med = (uint32_t*)(R.readout + Lo.medium);
+    uint32_t* frag = (uint32_t*)(R.readout + Lo.frag);
+    uint32_t* header = (uint32_t*)(R.readout + Lo.header);
+    for (int pass = 0; pass < 2; ++pass) {
+        const uint32_t n_items = pass == 0 ? n_sites : n_cells;
+        if (tid == 0) s_base = 0;
+        __syncthreads();
+        for (uint32_t b0 = 0; b0 < n_items; b0 += kRoThreads) {
+            const uint32_t q = b0 + tid;
+            bool keep = false;
+            uint32_t val = 0;
+            if (q < n_items) {
+                if (pass == 0) {
+                    uint32_t c[3];
+                    const uint32_t x = interior_flat(R, q, c);
+                    keep = sig[x] == 0 && uf_load(&lab[x]) == x;
+                    if (keep) val = uf_load(&cnt[x]);
+                } else {
+                    keep = uf_load(&ncomp[q + 1]) > 1;
+                    val = q + 1;
+                }
+            }
+            const uint64_t bal = __ballot(keep);
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+            if (lane == 0) s_wave[wave] = (uint32_t)__popcll(bal);
+            __syncthreads();
+            uint32_t before = s_base;
+            for (uint32_t w = 0; w < wave; ++w) before += s_wave[w];
+            if (keep) (pass == 0 ? med : frag)[before + rank] = val;
+            __syncthreads();
+            if (tid == 0)
+                for (uint32_t w = 0; w < kRoThreads / 64; ++w) s_base += s_wave[w];
+            __syncthreads();
+        }
+        if (tid == 0) header[pass == 0 ? 1 : 0] = s_base;
+        __syncthreads();
+    }
+    // per-cell shape (output.rs:147-231, 289-302)
+    const uint32_t per = readout_doubles_per_cell(D);
+    for (uint32_t i = tid; i < n_cells; i += kRoThreads) {
+        double* out = (double*)(R.readout + Lo.cells) + (size_t)i * per;
+        const double v = (double)R.volume[i];
+        double cen[3], g[9];
+        for (uint32_t d = 0; d < D; ++d) cen[d] = (double)R.sum_pos[i * D + d] / v;
+        for (uint32_t a = 0; a < D; ++a)
+            for (uint32_t b = 0; b < D; ++b)
+                g[a * D + b] = (double)R.sum_outer[(i * D + a) * D + b] / v - cen[a] * cen[b];
+        for (uint32_t d = 0; d < D; ++d) out[d] = cen[d];
+        for (uint32_t k = 0; k < D * D; ++k) out[D + k] = g[k];
+        if (D == 2) {
+            const double a = g[0], b = g[1], dd = g[3];
+            const double tr = a + dd;
+            const double t2 = (a - dd) * (a - dd) + 4.0 * b * b;
+            const double disc = sqrt(t2 < 0.0 ? 0.0 : t2);  // std::max(t2, 0.0)
+            double e0 = (tr + disc) / 2.0, e1 = (tr - disc) / 2.0;
+            if (e1 > e0) {
+                const double t = e0;
+                e0 = e1;
+                e1 = t;
+            }
+            double sum = 0.0, ssq = 0.0;
+            sum += e0;
+            sum += e1;
+            ssq += e0 * e0;
+            ssq += e1 * e1;
+            double kappa = 0.0;
+            if (sum > 0.0) kappa = (2.0 / (2.0 - 1.0)) * (ssq / (sum * sum)) - 1.0 / (2.0 - 1.0);
+            out[D + 4] = e0;
+            out[D + 5] = e1;
+            out[D + 6] = sum;
+            out[D + 7] = kappa;
+        }
+    }
+}
+
+// ===========================================================================
 // Contact counts recounted from the lattice (sublattice mode keeps no
 // running counts): the pairs of sites that differ, each unordered pair of
 // interior sites once (a halo neighbour counts as the interior site it
@@ -2029,7 +2268,7 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct Layout {
     size_t sigma, type_index, volume, interface, sum_pos, sum_outer, contact, params, targets,
-        draws, sig16, total;
+        draws, sig16, labels, lcount, readout, total;
 };
 
 static Layout layout_of(const Model& m, bool with_draws) {
@@ -2052,6 +2291,9 @@ static Layout layout_of(const Model& m, bool with_draws) {
     L.targets = take(2 * T * 4);
     L.draws = with_draws ? take((size_t)kFill * 8 + (size_t)kSegs * 32) : 0;
     L.sig16 = with_draws ? take(m.n_padded * 2) : 0;
+    L.labels = take(m.n_padded * 4);
+    L.lcount = take(m.n_padded * 4);
+    L.readout = take(readout_layout((uint32_t)D, (uint32_t)n, (uint32_t)m.n_sites).total);
     L.total = off;
     return L;
 }
@@ -2145,6 +2387,9 @@ static void upload(Engine* e, Model* m) {
     R.targets = (const uint32_t*)(d->base + L.targets);
     R.draws = lds ? (uint64_t*)(d->base + L.draws) : nullptr;
     R.sig16 = lds ? (uint16_t*)(d->base + L.sig16) : nullptr;
+    R.labels = (uint32_t*)(d->base + L.labels);
+    R.lcount = (uint32_t*)(d->base + L.lcount);
+    R.readout = (unsigned char*)(d->base + L.readout);
     const size_t T = R.n_types;
     std::vector<double> params(2 * T + (T + 1) * (T + 1));
     std::vector<uint32_t> targets(2 * T);
@@ -2254,6 +2499,16 @@ void sync_to_host(Model* m) {
     m->device_dirty = false;
 }
 
+template <class T>
+static void ensure_buf(T*& ptr, size_t& cap, size_t n) {
+    if (n <= cap) return;
+    if (ptr) HIP_OK(hipFree(ptr));
+    ptr = nullptr;
+    size_t c = std::max(n, (size_t)64);
+    HIP_OK(hipMalloc(&ptr, c * sizeof(T)));
+    cap = c;
+}
+
 // The lattice, type indices, volumes and interfaces only (adjacent at the
 // start of the allocation): what the energy needs.  The rest of the host
 // copy stays stale (device_dirty unchanged) until a full sync.
@@ -2274,15 +2529,112 @@ static void sync_lattice_tables(Model* m) {
 
 void sync_scalars(Model*) {}  // scalars are written back after every call
 
-template <class T>
-static void ensure_buf(T*& ptr, size_t& cap, size_t n) {
-    if (n <= cap) return;
-    if (ptr) HIP_OK(hipFree(ptr));
-    ptr = nullptr;
-    size_t c = std::max(n, (size_t)64);
-    HIP_OK(hipMalloc(&ptr, c * sizeof(T)));
-    cap = c;
+Engine* device_engine(const Model* m) { return m->dev ? m->dev->eng : nullptr; }
+
+// take_sample of the device-resident state of each model: the readout
+// kernel writes the labelling and per-cell shape into the model's readout
+// area; each model's volumes, interfaces, contacts and readout area (and
+// lattice when asked for) are then copied back on the copy streams, and
+// the samples assembled on the host (3D eigenvalues included).
+void device_samples(Engine* e, Model* const* models, size_t n, bool include_lattice,
+                    const uint64_t* at_mcs, const double* at_energy, Sample* out) {
+    if (!n) return;
+    {
+        std::lock_guard<std::mutex> lock(e->mu);
+        HIP_OK(hipSetDevice(e->device));
+        std::vector<Rep> reps(n);
+        std::vector<uint32_t> order(n);
+        for (size_t i = 0; i < n; ++i) {
+            if (!models[i]->dev || models[i]->dev->eng != e)
+                throw Error(CPM_ERR_ARG, "device readout of a model not resident on this engine");
+            reps[i] = models[i]->dev->rep;
+            order[i] = (uint32_t)i;
+        }
+        ensure_buf(e->reps_dev, e->reps_cap, n);
+        ensure_buf(e->idx_dev, e->idx_cap, n);
+        HIP_OK(hipMemcpyAsync(e->reps_dev, reps.data(), n * sizeof(Rep), hipMemcpyHostToDevice,
+                              e->stream));
+        HIP_OK(hipMemcpyAsync(e->idx_dev, order.data(), n * 4, hipMemcpyHostToDevice, e->stream));
+        hipLaunchKernelGGL(k_readout, dim3((uint32_t)n), dim3(kRoThreads), 0, e->stream,
+                           e->reps_dev, e->idx_dev);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipEventRecord(e->work_done, e->stream));
+        HIP_OK(hipStreamSynchronize(e->stream));
+    }
+    parallel_for(n, [&](size_t i) {
+        Model* m = models[i];
+        const Layout L = layout_of(*m, m->dev->lds);
+        const uint32_t D = (uint32_t)m->D, nc = (uint32_t)m->n_cells;
+        const ReadoutLayout Ro = readout_layout(D, nc, (uint32_t)m->n_sites);
+        // the readout area up to a first share of the medium components
+        constexpr uint32_t kFirstMedium = 1024;
+        const uint32_t first = Ro.medium + 4 * std::min<uint32_t>(kFirstMedium, (uint32_t)m->n_sites);
+        std::vector<uint8_t> ro(Ro.total);
+        std::vector<uint8_t> vi(L.interface + nc * 4 - L.volume);
+        Sample& s = out[i];
+        s.contact.resize((size_t)(nc + 1) * (nc + 1));
+        std::vector<uint32_t> lat;
+        if (include_lattice) lat.resize(m->n_padded);
+        hipStream_t st = copy_stream_for(e);
+        HIP_OK(hipStreamWaitEvent(st, e->work_done, 0));
+        HIP_OK(hipMemcpyAsync(ro.data(), m->dev->base + L.readout, first, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipMemcpyAsync(vi.data(), m->dev->base + L.volume, vi.size(), hipMemcpyDeviceToHost,
+                              st));
+        HIP_OK(hipMemcpyAsync(s.contact.data(), m->dev->base + L.contact, s.contact.size() * 4,
+                              hipMemcpyDeviceToHost, st));
+        if (include_lattice)
+            HIP_OK(hipMemcpyAsync(lat.data(), m->dev->base + L.sigma, lat.size() * 4,
+                                  hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        uint32_t hdr[2];
+        std::memcpy(hdr, ro.data() + Ro.header, 8);
+        if (hdr[1] > kFirstMedium) {
+            HIP_OK(hipMemcpyAsync(ro.data() + first, m->dev->base + L.readout + first,
+                                  4 * (size_t)(hdr[1] - kFirstMedium), hipMemcpyDeviceToHost, st));
+            HIP_OK(hipStreamSynchronize(st));
+        }
+        s.mcs = at_mcs[i];
+        s.energy = at_energy[i];
+        s.ids.resize(nc);
+        s.volume.resize(nc);
+        s.interface.resize(nc);
+        std::memcpy(s.volume.data(), vi.data(), nc * 4);
+        std::memcpy(s.interface.data(), vi.data() + (L.interface - L.volume), nc * 4);
+        s.centroid.resize((size_t)nc * D);
+        s.eigen.resize((size_t)nc * D);
+        s.rg2.resize(nc);
+        s.kappa.resize(nc);
+        const uint32_t per = readout_doubles_per_cell(D);
+        const double* cells = (const double*)(ro.data() + Ro.cells);
+        for (uint32_t c = 0; c < nc; ++c) {
+            s.ids[c] = c + 1;
+            const double* x = cells + (size_t)c * per;
+            for (uint32_t d = 0; d < D; ++d) s.centroid[(size_t)c * D + d] = x[d];
+            if (D == 2) {
+                s.eigen[(size_t)c * 2] = x[6];
+                s.eigen[(size_t)c * 2 + 1] = x[7];
+                s.rg2[c] = x[8];
+                s.kappa[c] = x[9];
+            } else {
+                shape_from_tensor((int)D, x + D, &s.eigen[(size_t)c * D], &s.rg2[c], &s.kappa[c]);
+            }
+        }
+        const uint32_t* frag = (const uint32_t*)(ro.data() + Ro.frag);
+        s.fragmented.assign(frag, frag + hdr[0]);
+        const uint32_t* med = (const uint32_t*)(ro.data() + Ro.medium);
+        s.medium_volumes.resize(hdr[1]);
+        s.medium_touch.resize(hdr[1]);
+        for (uint32_t k = 0; k < hdr[1]; ++k) {
+            s.medium_volumes[k] = med[k] & 0x7fffffffu;
+            s.medium_touch[k] = (uint8_t)(med[k] >> 31);
+        }
+        if (include_lattice) {
+            s.lattice = m->unpadded(lat.data());
+            s.has_lattice = true;
+        }
+    });
 }
+
 
 static void add_counters(cpm_counters* c, const Rep& R) {
     if (!c) return;

@@ -272,11 +272,13 @@ void Model::refresh_halo() {
         if (!is_interior(f)) sigma[f] = sigma[wrapped(f)];
 }
 
-std::vector<uint32_t> Model::unpadded_lattice() const {
+std::vector<uint32_t> Model::unpadded_lattice() const { return unpadded(sigma.data()); }
+
+std::vector<uint32_t> Model::unpadded(const uint32_t* padded) const {
     std::vector<uint32_t> out(n_sites);
     uint64_t c[3] = {0, 0, 0};
     for (uint64_t i = 0; i < n_sites; ++i) {
-        out[i] = sigma[flat(c)];
+        out[i] = padded[flat(c)];
         for (int d = D - 1; d >= 0; --d) {
             if (++c[d] < dims[d]) break;
             c[d] = 0;
@@ -645,6 +647,23 @@ static void sym_eigen(int D, const double* g, double* out) {
     out[2] = e3;
 }
 
+void shape_from_tensor(int D, const double* g, double* eigen, double* rg2, double* kappa) {
+    double ev[3];
+    sym_eigen(D, g, ev);
+    std::sort(ev, ev + D, [](double x, double y) { return x > y; });
+    double sum = 0.0, ssq = 0.0;
+    for (int d = 0; d < D; ++d) sum += ev[d];
+    for (int d = 0; d < D; ++d) ssq += ev[d] * ev[d];
+    double k = 0.0;
+    if (sum > 0.0) {
+        double dd = (double)D;
+        k = (dd / (dd - 1.0)) * (ssq / (sum * sum)) - 1.0 / (dd - 1.0);
+    }
+    for (int d = 0; d < D; ++d) eigen[d] = ev[d];
+    *rg2 = sum;
+    *kappa = k;
+}
+
 static void label(const Model& m, Sample& s) {
     const bool periodic = m.cfg.boundary == CPM_BOUNDARY_PERIODIC;
     std::vector<uint8_t> visited(m.n_padded);
@@ -704,27 +723,13 @@ Sample take_sample(const Model& m, bool include_lattice, uint64_t mcs, double en
     for (size_t i = 0; i < n; ++i) {
         s.ids[i] = (uint32_t)(i + 1);
         double v = (double)m.volume[i];
-        double cen[3], g[9], ev[3];
+        double cen[3], g[9];
         for (int d = 0; d < D; ++d) cen[d] = (double)m.sum_pos[i * D + d] / v;
         for (int a = 0; a < D; ++a)
             for (int b = 0; b < D; ++b)
                 g[a * D + b] = (double)m.sum_outer[(i * D + a) * D + b] / v - cen[a] * cen[b];
-        sym_eigen(D, g, ev);
-        std::sort(ev, ev + D, [](double x, double y) { return x > y; });
-        double sum = 0.0, ssq = 0.0;
-        for (int d = 0; d < D; ++d) sum += ev[d];
-        for (int d = 0; d < D; ++d) ssq += ev[d] * ev[d];
-        double kappa = 0.0;
-        if (sum > 0.0) {
-            double dd = (double)D;
-            kappa = (dd / (dd - 1.0)) * (ssq / (sum * sum)) - 1.0 / (dd - 1.0);
-        }
-        for (int d = 0; d < D; ++d) {
-            s.centroid[i * D + d] = cen[d];
-            s.eigen[i * D + d] = ev[d];
-        }
-        s.rg2[i] = sum;
-        s.kappa[i] = kappa;
+        for (int d = 0; d < D; ++d) s.centroid[i * D + d] = cen[d];
+        shape_from_tensor(D, g, &s.eigen[i * D], &s.rg2[i], &s.kappa[i]);
     }
     s.contact = m.contact;
     label(m, s);

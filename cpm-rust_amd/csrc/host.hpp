@@ -104,6 +104,8 @@ struct Model {
                    std::vector<int64_t>& so, std::vector<uint32_t>& ct) const;
     double global_energy() const;
     std::vector<uint32_t> unpadded_lattice() const;
+    // the interior of a padded lattice, row-major
+    std::vector<uint32_t> unpadded(const uint32_t* padded) const;
     size_t ckey(uint32_t a, uint32_t b) const {
         uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
         return (size_t)lo * (n_cells + 1) + hi;
@@ -129,6 +131,9 @@ inline Sample take_sample(const Model& m, bool include_lattice) {
     return take_sample(m, include_lattice, m.mcs, m.energy);
 }
 bool not_equilibrated(const std::vector<double>& trace);
+// shape_descriptors (output.rs:147-187) from a cell's gyration tensor g
+// (D x D): eigenvalues in descending order, R_g^2 and kappa^2.
+void shape_from_tensor(int D, const double* g, double* eigen, double* rg2, double* kappa);
 
 struct Result {
     std::string cpm_version, convention_hash, model_hash, seed_policy, status_kind;

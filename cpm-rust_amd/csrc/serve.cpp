@@ -126,14 +126,6 @@ void Server::round() {
     }
     cpm_counters cnt{};
     std::string failure;
-    // a sublattice-mode call writes its replicas' host copies when it ends
-    // (its energy is recomputed there), which the previous round's readouts
-    // may still be reading: they finish first
-    for (size_t i = 0; i < n; ++i)
-        if (active[i]->mode == CPM_MODE_SUBLATTICE) {
-            wait_readouts();
-            break;
-        }
     try {
         for (int mode : {CPM_MODE_EXACT_REPLAY, CPM_MODE_SUBLATTICE}) {
             std::vector<Model*> ms;
@@ -154,8 +146,9 @@ void Server::round() {
     std::vector<Request*> finished, sampled;
     std::vector<uint64_t> at_mcs;
     std::vector<double> at_energy;
-    // the readouts of the previous round read the host copies that this
-    // round's syncs overwrite, and may still append to results
+    std::vector<Sample> samples;
+    // the previous round's readout thread may still be appending samples and
+    // finishing results
     wait_readouts();
     std::vector<uint64_t> done_mcs(n, 0);  // whole MCS completed, published under the lock
     if (failure.empty()) {
@@ -175,7 +168,19 @@ void Server::round() {
             if (++r->step == r->sch.size()) finished.push_back(r);
         }
         try {
-            parallel_for(sampled.size(), [&](size_t j) { sync_to_host(sampled[j]->m.get()); });
+            // readouts on the device, before finished requests leave it
+            std::vector<Model*> sm(sampled.size());
+            for (size_t j = 0; j < sampled.size(); ++j) sm[j] = sampled[j]->m.get();
+            samples.resize(sampled.size());
+            bool lattice_any = false;
+            for (Request* r : sampled) lattice_any |= r->include_lattice;
+            device_samples(e, sm.data(), sm.size(), lattice_any, at_mcs.data(), at_energy.data(),
+                           samples.data());
+            for (size_t j = 0; j < sampled.size(); ++j)
+                if (!sampled[j]->include_lattice) {
+                    samples[j].lattice.clear();
+                    samples[j].has_lattice = false;
+                }
             parallel_for(finished.size(), [&](size_t j) {
                 sync_to_host(finished[j]->m.get());
                 device_release(finished[j]->m.get());
@@ -220,15 +225,11 @@ void Server::round() {
     if (sampled.empty() && finished.empty()) return;
     readouts = std::async(std::launch::async, [this, sampled = std::move(sampled),
                                                finished = std::move(finished),
-                                               at_mcs = std::move(at_mcs),
-                                               at_energy = std::move(at_energy)] {
+                                               samples = std::move(samples)]() mutable {
         std::string err;
         try {
-            parallel_for(sampled.size(), [&](size_t j) {
-                Request* r = sampled[j];
-                r->result.samples.push_back(
-                    take_sample(*r->m, r->include_lattice, at_mcs[j], at_energy[j]));
-            });
+            for (size_t j = 0; j < sampled.size(); ++j)
+                sampled[j]->result.samples.push_back(std::move(samples[j]));
             parallel_for(finished.size(), [&](size_t j) {
                 Request* r = finished[j];
                 finish_result(*r->m, r->trace, r->result);

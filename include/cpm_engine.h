@@ -231,7 +231,9 @@ cpm_status cpm_model_set_state(cpm_model* model, const cpm_model_state* state);
 
 /* take_sample (crates/cpm-core/src/output.rs:289-337, with
    label_components, labelling.rs:70-154) of the model's current state, as
-   a result holding that one sample (include_lattice as in cpm_run).  Its
+   a result holding that one sample (include_lattice as in cpm_run),
+   computed on the device when the model is resident there (it has run)
+   and on the host otherwise, with the same results.  Its
    status applies run_with's rules (output.rs:470-509) to this one sample
    with an empty burn-in trace: Fragmented if a cell is in pieces, CellLost
    if a cell has volume <= 1, Degenerate if the contact graph joins every
