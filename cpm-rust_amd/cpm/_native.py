@@ -96,7 +96,8 @@ class CpmModelState(C.Structure):
     _fields_ = [("lattice", u32p), ("n_cells", C.c_uint64), ("type_index", u32p),
                 ("volume", u32p), ("interface", u32p), ("sum_position", i64p),
                 ("sum_outer", i64p), ("contact", u32p), ("has_energy", C.c_int32),
-                ("conservative_energy", C.c_double), ("rng_state", u64p), ("mcs", C.c_uint64)]
+                ("conservative_energy", C.c_double), ("rng_state", u64p), ("mcs", C.c_uint64),
+                ("birth_mcs", u32p)]
 
 
 class CpmResultInfo(C.Structure):
@@ -220,6 +221,12 @@ def lib():
         L.cpm_engine_set_sigma_placement.restype = st
         L.cpm_metropolis_check.argtypes = [vp, f64p, f64p, C.c_size_t, f64p, f64p, u8p]
         L.cpm_metropolis_check.restype = st
+        L.cpm_act_root_check.argtypes = [vp, f64p, u32p, C.c_size_t, f64p]
+        L.cpm_act_root_check.restype = st
+        L.cpm_mh_check.argtypes = [vp, u32p, u32p, f64p, f64p, C.c_size_t, u8p]
+        L.cpm_mh_check.restype = st
+        L.cpm_model_birth_mcs.argtypes = [vp, u32p, C.c_size_t]
+        L.cpm_model_birth_mcs.restype = st
         L.cpm_engine_set_pipelining.argtypes = [vp, C.c_int32]
         L.cpm_engine_set_pipelining.restype = st
         L.cpm_engine_kernel_launches.argtypes = [vp, C.POINTER(C.c_int32), u64p, C.c_size_t,
@@ -294,6 +301,35 @@ def metropolis_check(delta_h, u=None, device: bool = True):
         None if uu is None else uu.ctypes.data_as(f64p), dh.size, pl.ctypes.data_as(f64p),
         pc.ctypes.data_as(f64p), None if acc is None else acc.ctypes.data_as(u8p)))
     return pl, pc, (None if acc is None else acc.astype(bool))
+
+
+def act_root_check(product, count, device: bool = True):
+    """cpm_act_root_check: the Act term's geometric-mean root
+    product^(1/count) as the exact-replay kernels compute it on the device
+    (or, with device=False, the same code on the host)."""
+    x = np.ascontiguousarray(np.asarray(product, dtype=np.float64))
+    n = np.ascontiguousarray(np.asarray(count, dtype=np.uint32))
+    if x.shape != n.shape:
+        raise ValueError("product and count must have the same shape")
+    out = np.empty_like(x)
+    check(lib().cpm_act_root_check(engine() if device else None, x.ctypes.data_as(f64p),
+                                   n.ctypes.data_as(u32p), x.size, out.ctypes.data_as(f64p)))
+    return out
+
+
+def mh_check(n_a, n_b, delta_h, u, device: bool = True):
+    """cpm_mh_check: the exact-replay Metropolis-Hastings decisions."""
+    a = np.ascontiguousarray(np.asarray(n_a, dtype=np.uint32))
+    b = np.ascontiguousarray(np.asarray(n_b, dtype=np.uint32))
+    dh = np.ascontiguousarray(np.asarray(delta_h, dtype=np.float64))
+    uu = np.ascontiguousarray(np.asarray(u, dtype=np.float64))
+    if not (a.shape == b.shape == dh.shape == uu.shape):
+        raise ValueError("n_a, n_b, delta_h and u must have the same shape")
+    acc = np.empty(dh.size, dtype=np.uint8)
+    check(lib().cpm_mh_check(engine() if device else None, a.ctypes.data_as(u32p),
+                             b.ctypes.data_as(u32p), dh.ctypes.data_as(f64p),
+                             uu.ctypes.data_as(f64p), dh.size, acc.ctypes.data_as(u8p)))
+    return acc.astype(bool)
 
 
 def set_pipelining(on: bool) -> None:

@@ -120,12 +120,14 @@ class Replica:
         return float(out.value)
 
     def set_state(self, lattice, type_index, volume=None, interface=None, sum_position=None,
-                  sum_outer=None, contact=None, energy=None, rng=None, mcs: int = 0) -> None:
+                  sum_outer=None, contact=None, energy=None, rng=None, mcs: int = 0,
+                  birth_mcs=None) -> None:
         """Replaces the whole state by the given one (the fields of CPM<D> and
-        State<D>, crates/cpm-core/src/model.rs:15-38, state.rs:168-203);
-        omitted per-cell fields are derived from the lattice as
-        initialization does, an omitted energy by global_energy, an
-        omitted rng keeps the current stream."""
+        State<D>, crates/cpm-core/src/model.rs:15-38, state.rs:168-203, and
+        the Act term's per-site birth MCS, energy.rs:337-344); omitted
+        per-cell fields are derived from the lattice as initialization
+        does, an omitted energy by global_energy, an omitted rng keeps the
+        current stream, omitted births are 0."""
         keep = []
 
         def arr(x, dtype, ptype):
@@ -149,9 +151,20 @@ class Replica:
         st.conservative_energy = 0.0 if energy is None else float(energy)
         st.rng_state = arr(rng, np.uint64, N.u64p)
         st.mcs = int(mcs)
+        st.birth_mcs = arr(birth_mcs, np.uint32, N.u32p)
+        if birth_mcs is not None and np.asarray(birth_mcs).size != np.asarray(lattice).size:
+            raise ValueError("birth_mcs must hold one MCS per site")
         if np.asarray(lattice).size != int(np.prod(self.spec["grid"])):
             raise ValueError("lattice must hold one id per site")
         N.check(N.lib().cpm_model_set_state(self.h, C.byref(st)))
+
+    def birth_mcs(self) -> np.ndarray:
+        """The Act term's per-site birth MCS (energy.rs:337-344), shaped like
+        the grid; zeros when the term is off."""
+        grid = tuple(self.spec["grid"])
+        out = np.zeros(int(np.prod(grid)), dtype=np.uint32)
+        N.check(N.lib().cpm_model_birth_mcs(self.h, out.ctypes.data_as(N.u32p), out.size))
+        return out.reshape(grid)
 
     def sample(self, include_lattice: bool = False):
         """take_sample of the current state (crates/cpm-core/src/

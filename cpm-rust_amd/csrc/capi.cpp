@@ -234,6 +234,21 @@ cpm_status cpm_model_get_state(cpm_model* model, uint32_t* lattice, uint32_t* vo
     });
 }
 
+cpm_status cpm_model_birth_mcs(cpm_model* model, uint32_t* out, size_t n_sites) {
+    return guarded([&] {
+        need(model && out, "model/out");
+        Model* m = model->m;
+        need(n_sites == m->n_sites, "n_sites");
+        sync_to_host(m);
+        if (!m->cfg.act) {
+            std::memset(out, 0, n_sites * 4);
+            return;
+        }
+        auto b = m->unpadded(m->birth.data());
+        std::memcpy(out, b.data(), b.size() * 4);
+    });
+}
+
 cpm_status cpm_model_hashes(cpm_model* model, char convention_hex[65], char model_hex[65]) {
     return guarded([&] {
         need(model && convention_hex && model_hex, "model/out");
@@ -269,6 +284,7 @@ cpm_status cpm_model_set_state(cpm_model* model, const cpm_model_state* state) {
         m->sum_pos.swap(next.sum_pos);
         m->sum_outer.swap(next.sum_outer);
         m->contact.swap(next.contact);
+        m->birth.swap(next.birth);
         m->energy = next.energy;
         m->rng = state->rng_state ? next.rng : m->rng;
         m->mcs = next.mcs;
@@ -567,6 +583,35 @@ cpm_status cpm_metropolis_check(cpm_engine* engine, const double* delta_h, const
                 bool tie = false;
                 accept[i] = metropolis_accept(u[i], delta_h[i], &tie) ? 1 : 0;
             }
+        }
+    });
+}
+
+cpm_status cpm_act_root_check(cpm_engine* engine, const double* product, const uint32_t* count,
+                              size_t n, double* out) {
+    return guarded([&] {
+        need((product && count && out) || n == 0, "product/count/out");
+        for (size_t i = 0; i < n; ++i) need(count[i] >= 1, "count >= 1");
+        if (engine) {
+            engine_act_root_check(engine->e, product, count, n, out);
+            return;
+        }
+        for (size_t i = 0; i < n; ++i) out[i] = act_root(product[i], count[i]);
+    });
+}
+
+cpm_status cpm_mh_check(cpm_engine* engine, const uint32_t* n_a, const uint32_t* n_b,
+                        const double* delta_h, const double* u, size_t n, uint8_t* accept) {
+    return guarded([&] {
+        need((n_a && n_b && delta_h && u && accept) || n == 0, "n_a/n_b/delta_h/u/accept");
+        for (size_t i = 0; i < n; ++i) need(n_a[i] >= 1 && n_b[i] >= 1, "n_a, n_b >= 1");
+        if (engine) {
+            engine_mh_check(engine->e, n_a, n_b, delta_h, u, n, accept);
+            return;
+        }
+        for (size_t i = 0; i < n; ++i) {
+            bool tie = false;
+            accept[i] = mh_accept(u[i], n_a[i], n_b[i], delta_h[i], &tie) ? 1 : 0;
         }
     });
 }

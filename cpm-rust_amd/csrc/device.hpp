@@ -58,6 +58,12 @@ void engine_set_placement(Engine* e, int order, double share);
 // given) the exact-replay kernels' Metropolis decision for u[i].
 void engine_metropolis_check(Engine* e, const double* dh, const double* u, size_t n,
                              double* p_libm, double* p_cr, uint8_t* acc);
+// the Act root and the Metropolis-Hastings decision as the lane path
+// computes them (device copies of the inputs, one launch)
+void engine_act_root_check(Engine* e, const double* product, const uint32_t* count, size_t n,
+                           double* out);
+void engine_mh_check(Engine* e, const uint32_t* n_a, const uint32_t* n_b, const double* dh,
+                     const double* u, size_t n, uint8_t* accept);
 // Allow (default) or forbid the pipelined two-wave exact-replay kernel.
 void engine_set_pipelining(Engine* e, bool on);
 // (instantiation key, launches) of the exact-replay wave kernels so far.

@@ -69,8 +69,11 @@ struct Rep {
     int64_t* sum_pos;
     int64_t* sum_outer;
     uint32_t* contact;
-    const double* params;     // lambda_v[T], lambda_i[T], J[(T+1)^2]
-    const uint32_t* targets;  // V*[T], I*[T]
+    const double* params;     // lambda_v[T], lambda_i[T], J[(T+1)^2], lambda_act[T]
+    const uint32_t* targets;  // V*[T], I*[T], max_act[T]
+    uint32_t acceptance;      // CPM_ACCEPT_* (the lane path takes both)
+    uint32_t act;             // Act term on (lane path only)
+    uint32_t* birth;          // Act: per padded site, the MCS it was last taken
     uint64_t* draws;          // kFill buffered draws + kSegs segment states (wave path)
     uint16_t* sig16;          // packed sigma in HBM (wave path with sigma outside LDS)
     uint32_t* labels;         // readouts: union-find parent per padded site
@@ -104,8 +107,62 @@ constexpr size_t kNoallocScratchBytes = kNoallocRepBytes + kNoallocMaxReplicas *
 static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
 
 // ===========================================================================
-// Global path: one lane per replica (lattices too large for LDS)
+// Global path: one lane per replica (lattices too large for LDS, and every
+// Metropolis-Hastings or Act model)
 // ===========================================================================
+
+// The Act term's geometric mean (crates/cpm-core/src/energy.rs:403-444)
+// around the interior site at coordinate cc (0-based): activity
+// max(0, max_act - (mcs - birth)) over the site and its copy neighbours
+// owned by `rel`, multiplied in that order, then the count-th root.  A
+// neighbour across a periodic face is read at the interior site it
+// mirrors; across a fixed face it is the halo (medium, birth 0).
+template <int D>
+__device__ double act_mean(const Rep& R, const uint32_t* cc, const int* axis, const int* sgn,
+                           uint32_t rel, uint32_t max_act, uint64_t mcs) {
+    const bool periodic = R.boundary == CPM_BOUNDARY_PERIODIC;
+    double product = 1.0;
+    uint32_t count = 0;
+    for (int k = -1; k < (int)R.n_copy; ++k) {
+        uint32_t f = 0;
+        for (int d = 0; d < D; ++d) {
+            int64_t x = (int64_t)cc[d] + 1;  // padded coordinate
+            if (k >= 0 && axis[k] == d) {
+                x += sgn[k];
+                if (periodic) x = x == 0 ? R.dims[d] : (x == (int64_t)R.dims[d] + 1 ? 1 : x);
+            }
+            f += (uint32_t)x * R.strides[d];
+        }
+        if (R.sigma[f] != rel) continue;
+        const uint64_t birth = R.birth[f];
+        const uint64_t elapsed = mcs > birth ? mcs - birth : 0;
+        const uint64_t activity = (uint64_t)max_act > elapsed ? (uint64_t)max_act - elapsed : 0;
+        product *= (double)activity;
+        count += 1;
+    }
+    return act_root(product, count);
+}
+
+// delta_H_Act (energy.rs:446-457) of copying the source (copy neighbour k
+// of the target at coordinate c, owned by G) onto the target (owned by L).
+template <int D>
+__device__ double act_delta(const Rep& R, const uint32_t* c, uint32_t k, const int* axis,
+                            const int* sgn, uint32_t L, uint32_t G, uint64_t mcs) {
+    if (G == 0) return 0.0;  // medium gaining: no type to take lambda and max_act from
+    const uint32_t T = R.n_types;
+    const uint32_t ty = R.type_index[G - 1];
+    const double lambda = R.params[2 * T + (T + 1) * (T + 1) + ty];
+    const uint32_t max_act = R.targets[2 * T + ty];
+    uint32_t cs[D];
+    for (int d = 0; d < D; ++d) cs[d] = c[d];
+    // the source's interior coordinate (a gaining cell is never a fixed halo)
+    const int64_t x = (int64_t)cs[axis[k]] + sgn[k];
+    cs[axis[k]] = x < 0 ? R.dims[axis[k]] - 1 : (x == (int64_t)R.dims[axis[k]] ? 0 : (uint32_t)x);
+    const double gm_src = act_mean<D>(R, cs, axis, sgn, G, max_act, mcs);
+    const double gm_tgt = act_mean<D>(R, c, axis, sgn, L, max_act, mcs);
+    return -(lambda / (double)max_act) * (gm_src - gm_tgt);
+}
+
 template <int D>
 __device__ void replica_chain(Rep& R, const uint64_t* __restrict__ sp3d) {
     Xoshiro rng;
@@ -133,8 +190,20 @@ __device__ void replica_chain(Rep& R, const uint64_t* __restrict__ sp3d) {
     const uint32_t* __restrict__ ti = R.targets + T;
     const uint64_t ncell1 = (uint64_t)R.n_cells + 1;
     const bool periodic = R.boundary == CPM_BOUNDARY_PERIODIC;
+    const bool mh = R.acceptance == CPM_ACCEPT_METROPOLIS_HASTINGS;
+    // the axis and direction of each copy offset (von Neumann: +-stride)
+    int axis[6], sgn[6];
+    for (uint32_t k = 0; k < n_copy; ++k) {
+        const int32_t o = R.copy_off[k];
+        for (int d = 0; d < D; ++d)
+            if ((uint32_t)(o < 0 ? -o : o) == strides[d]) {
+                axis[k] = d;
+                sgn[k] = o < 0 ? -1 : 1;
+            }
+    }
 
     for (uint64_t m = 0; m < R.todo; ++m) {
+        const uint64_t mcs = R.mcs + m;
         for (uint32_t a = 0; a < R.n_sites; ++a) {
             uint32_t c[D];
             uint32_t t = 0;
@@ -199,15 +268,36 @@ __device__ void replica_chain(Rep& R, const uint64_t* __restrict__ sp3d) {
                 ditf += li[tg] * (x1 * x1) - li[tg] * (x0 * x0);
             }
             const double dh = dvol + ditf + dadh;
-            bool accept = dh <= 0.0;
-            if (!accept) {
+            // monte_carlo.rs:80-97: the Act term enters the test, never the energy
+            const double dh_acc =
+                R.act ? dh + act_delta<D>(R, c, (uint32_t)k, axis, sgn, L, G, mcs) : dh;
+            bool accept;
+            if (mh) {
+                // monte_carlo.rs:168-222: no downhill shortcut; n_A == 0
+                // rejects before the draw
+                uint32_t na = 0, nb = 0;
+                for (uint32_t q = 0; q < n_copy; ++q) {
+                    const uint32_t v = sig[(int32_t)t + R.copy_off[q]];
+                    na += v == L;
+                    nb += v == G;
+                }
+                if (na == 0) continue;
                 const double u = gen_f64(rng);
                 bool tie = false;
-                accept = metropolis_accept(u, dh, &tie);
+                accept = mh_accept(u, na, nb, dh_acc, &tie);
                 if (tie) cnt[5]++;
+            } else {
+                accept = dh_acc <= 0.0;
+                if (!accept) {
+                    const double u = gen_f64(rng);
+                    bool tie = false;
+                    accept = metropolis_accept(u, dh_acc, &tie);
+                    if (tie) cnt[5]++;
+                }
             }
             if (!accept) continue;
             cnt[4]++;
+            if (R.act) R.birth[t] = (uint32_t)mcs;  // energy.rs:459-464
             for (uint32_t q = 0; q < n_energy; ++q) {
                 const uint32_t nb = sig[(int32_t)t + R.energy_off[q]];
                 if (nb != L) {
@@ -2268,7 +2358,7 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct Layout {
     size_t sigma, type_index, volume, interface, sum_pos, sum_outer, contact, params, targets,
-        draws, sig16, labels, lcount, readout, total;
+        birth, draws, sig16, labels, lcount, readout, total;
 };
 
 static Layout layout_of(const Model& m, bool with_draws) {
@@ -2287,8 +2377,9 @@ static Layout layout_of(const Model& m, bool with_draws) {
     L.sum_pos = take(n * D * 8);
     L.sum_outer = take(n * D * D * 8);
     L.contact = take((n + 1) * (n + 1) * 4);
-    L.params = take((2 * T + (T + 1) * (T + 1)) * 8);
-    L.targets = take(2 * T * 4);
+    L.params = take((3 * T + (T + 1) * (T + 1)) * 8);
+    L.targets = take(3 * T * 4);
+    L.birth = take(m.cfg.act ? m.n_padded * 4 : 0);
     L.draws = with_draws ? take((size_t)kFill * 8 + (size_t)kSegs * 32) : 0;
     L.sig16 = with_draws ? take(m.n_padded * 2) : 0;
     L.labels = take(m.n_padded * 4);
@@ -2303,6 +2394,8 @@ static Layout layout_of(const Model& m, bool with_draws) {
 // cells and 15 types; sigma stays in LDS when it fits beside the tables.
 static bool lds_eligible(const Engine* e, const Model& m, uint32_t* bytes, bool* gsig = nullptr) {
     if (m.n_cells > kMaxPackedCells || m.n_padded > (1u << 30)) return false;
+    // Metropolis-Hastings and the Act term run on the lane path
+    if (m.cfg.acceptance != CPM_ACCEPT_METROPOLIS || m.cfg.act) return false;
     if (m.cfg.types.size() > 15) return false;
     const uint32_t T = (uint32_t)m.cfg.types.size();
     const LdsLayout in = lds_layout((uint32_t)m.n_padded, (uint32_t)m.n_cells, T, true);
@@ -2385,19 +2478,24 @@ static void upload(Engine* e, Model* m) {
     R.contact = (uint32_t*)(d->base + L.contact);
     R.params = (const double*)(d->base + L.params);
     R.targets = (const uint32_t*)(d->base + L.targets);
+    R.acceptance = (uint32_t)m->cfg.acceptance;
+    R.act = m->cfg.act ? 1u : 0u;
+    R.birth = m->cfg.act ? (uint32_t*)(d->base + L.birth) : nullptr;
     R.draws = lds ? (uint64_t*)(d->base + L.draws) : nullptr;
     R.sig16 = lds ? (uint16_t*)(d->base + L.sig16) : nullptr;
     R.labels = (uint32_t*)(d->base + L.labels);
     R.lcount = (uint32_t*)(d->base + L.lcount);
     R.readout = (unsigned char*)(d->base + L.readout);
     const size_t T = R.n_types;
-    std::vector<double> params(2 * T + (T + 1) * (T + 1));
-    std::vector<uint32_t> targets(2 * T);
+    std::vector<double> params(3 * T + (T + 1) * (T + 1));
+    std::vector<uint32_t> targets(3 * T);
     for (size_t t = 0; t < T; ++t) {
         params[t] = m->cfg.types[t].lambda_volume;
         params[T + t] = m->cfg.types[t].lambda_interface;
+        params[2 * T + (T + 1) * (T + 1) + t] = m->cfg.types[t].lambda_act;
         targets[t] = m->cfg.types[t].target_volume;
         targets[T + t] = m->cfg.types[t].target_interface;
+        targets[2 * T + t] = m->cfg.types[t].max_act;
     }
     for (size_t i = 0; i <= T; ++i)
         for (size_t j = 0; j <= T; ++j) params[2 * T + i * (T + 1) + j] = m->cfg.adhesion[i][j];
@@ -2414,6 +2512,7 @@ static void upload(Engine* e, Model* m) {
     put(L.contact, m->contact.data(), (n + 1) * (n + 1) * 4);
     put(L.params, params.data(), params.size() * 8);
     put(L.targets, targets.data(), targets.size() * 4);
+    if (m->cfg.act) put(L.birth, m->birth.data(), m->n_padded * 4);
     HIP_OK(hipStreamSynchronize(e->stream));
     m->device_dirty = false;
 }
@@ -2437,6 +2536,7 @@ static uint32_t sublattice_lds(Engine* e, const Model& m, bool* sg) {
 }
 
 bool sublattice_supported(Engine* e, const Model& m) {
+    if (m.cfg.acceptance != CPM_ACCEPT_METROPOLIS || m.cfg.act) return false;
     uint32_t bytes = 0;
     bool gsig = false;
     if (!lds_eligible(e, m, &bytes, &gsig)) return false;  // packed sigma words
@@ -2485,6 +2585,9 @@ void sync_to_host(Model* m) {
                           hipMemcpyDeviceToHost, st));
     if (contact_apart)
         HIP_OK(hipMemcpyAsync(m->contact.data(), m->dev->base + L.contact, contact_bytes,
+                              hipMemcpyDeviceToHost, st));
+    if (m->cfg.act)
+        HIP_OK(hipMemcpyAsync(m->birth.data(), m->dev->base + L.birth, m->n_padded * 4,
                               hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
     auto put = [&](void* dst, size_t off, size_t bytes) {
@@ -2992,6 +3095,10 @@ static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint
         bool any3d = false;
         for (size_t i = 0; i < n; ++i) {
             Model* m = models[i];
+            if (m->cfg.acceptance != CPM_ACCEPT_METROPOLIS || m->cfg.act)
+                throw Error(CPM_ERR_ARG,
+                            "sublattice mode runs the Metropolis rule without the Act term; "
+                            "use exact replay for Metropolis-Hastings or Act models");
             upload(e, m);
             if (!m->dev->lds)
                 throw Error(CPM_ERR_ARG,
@@ -3264,6 +3371,63 @@ void engine_metropolis_check(Engine* e, const double* dh, const double* u, size_
     HIP_OK(hipMemcpyAsync(p_libm, d_pl, bytes, hipMemcpyDeviceToHost, e->stream));
     HIP_OK(hipMemcpyAsync(p_cr, d_pc, bytes, hipMemcpyDeviceToHost, e->stream));
     if (u) HIP_OK(hipMemcpyAsync(acc, d_acc, n, hipMemcpyDeviceToHost, e->stream));
+    HIP_OK(hipStreamSynchronize(e->stream));
+    HIP_OK(hipFree(buf));
+}
+
+__global__ void k_act_root_check(const double* x, const uint32_t* cnt, uint64_t n, double* out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = act_root(x[i], cnt[i]);
+}
+
+__global__ void k_mh_check(const uint32_t* na, const uint32_t* nb, const double* dh,
+                           const double* u, uint64_t n, uint8_t* acc) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    bool tie = false;
+    acc[i] = mh_accept(u[i], na[i], nb[i], dh[i], &tie) ? 1 : 0;
+}
+
+void engine_act_root_check(Engine* e, const double* product, const uint32_t* count, size_t n,
+                           double* out) {
+    std::lock_guard<std::mutex> lock(e->mu);
+    HIP_OK(hipSetDevice(e->device));
+    if (!n) return;
+    char* buf = nullptr;
+    HIP_OK(hipMalloc(&buf, n * 20));
+    double* d_x = (double*)buf;
+    double* d_out = (double*)(buf + n * 8);
+    uint32_t* d_c = (uint32_t*)(buf + n * 16);
+    HIP_OK(hipMemcpyAsync(d_x, product, n * 8, hipMemcpyHostToDevice, e->stream));
+    HIP_OK(hipMemcpyAsync(d_c, count, n * 4, hipMemcpyHostToDevice, e->stream));
+    hipLaunchKernelGGL(k_act_root_check, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0,
+                       e->stream, d_x, d_c, (uint64_t)n, d_out);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(out, d_out, n * 8, hipMemcpyDeviceToHost, e->stream));
+    HIP_OK(hipStreamSynchronize(e->stream));
+    HIP_OK(hipFree(buf));
+}
+
+void engine_mh_check(Engine* e, const uint32_t* n_a, const uint32_t* n_b, const double* dh,
+                     const double* u, size_t n, uint8_t* accept) {
+    std::lock_guard<std::mutex> lock(e->mu);
+    HIP_OK(hipSetDevice(e->device));
+    if (!n) return;
+    char* buf = nullptr;
+    HIP_OK(hipMalloc(&buf, n * 25));
+    double* d_dh = (double*)buf;
+    double* d_u = (double*)(buf + n * 8);
+    uint32_t* d_na = (uint32_t*)(buf + n * 16);
+    uint32_t* d_nb = (uint32_t*)(buf + n * 20);
+    uint8_t* d_acc = (uint8_t*)(buf + n * 24);
+    HIP_OK(hipMemcpyAsync(d_dh, dh, n * 8, hipMemcpyHostToDevice, e->stream));
+    HIP_OK(hipMemcpyAsync(d_u, u, n * 8, hipMemcpyHostToDevice, e->stream));
+    HIP_OK(hipMemcpyAsync(d_na, n_a, n * 4, hipMemcpyHostToDevice, e->stream));
+    HIP_OK(hipMemcpyAsync(d_nb, n_b, n * 4, hipMemcpyHostToDevice, e->stream));
+    hipLaunchKernelGGL(k_mh_check, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, e->stream,
+                       d_na, d_nb, d_dh, d_u, (uint64_t)n, d_acc);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(accept, d_acc, n, hipMemcpyDeviceToHost, e->stream));
     HIP_OK(hipStreamSynchronize(e->stream));
     HIP_OK(hipFree(buf));
 }

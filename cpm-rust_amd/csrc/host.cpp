@@ -191,10 +191,8 @@ void validate(const Config& c) {
     // silently run differently.
     if (c.proposal != CPM_PROPOSAL_UNIFORM)
         throw Error(CPM_ERR_CONFIG, "proposal = EdgeList is not supported by this engine");
-    if (c.acceptance != CPM_ACCEPT_METROPOLIS)
-        throw Error(CPM_ERR_CONFIG,
-                    "acceptance = MetropolisHastings is not supported by this engine");
-    if (c.act) throw Error(CPM_ERR_CONFIG, "the Act energy term is not supported by this engine");
+    if (c.acceptance != CPM_ACCEPT_METROPOLIS && c.acceptance != CPM_ACCEPT_METROPOLIS_HASTINGS)
+        throw Error(CPM_ERR_CONFIG, "unknown acceptance mode " + std::to_string(c.acceptance));
     if (c.types.size() > (size_t)kMaxTypes)
         throw Error(CPM_ERR_CONFIG, "this engine supports at most " + std::to_string(kMaxTypes) +
                                         " cell types");
@@ -231,6 +229,7 @@ Model::Model(const Config& c) : cfg(c) {
     n_conn = deltas(NB_VON_NEUMANN, conn_off);
     n_topo = deltas(D == 2 ? NB_MOORE : NB_TWENTY_SIX, topo_off);
     sigma.assign(n_padded, 0);
+    if (cfg.act) birth.assign(n_padded, 0);
     rng = xo_seed(cfg.seed);
 }
 
@@ -597,6 +596,12 @@ void Model::set_state(const cpm_model_state& st) {
     contact = st.contact ? std::vector<uint32_t>(st.contact, st.contact + (n + 1) * (n + 1))
                          : ct;
     energy = st.has_energy ? st.conservative_energy : global_energy();
+    if (cfg.act) {
+        std::fill(birth.begin(), birth.end(), 0u);
+        k = 0;
+        if (st.birth_mcs)
+            each_interior(*this, [&](const uint64_t* c) { birth[flat(c)] = st.birth_mcs[k++]; });
+    }
     if (st.rng_state)
         for (int q = 0; q < 4; ++q) rng.s[q] = st.rng_state[q];
     mcs = st.mcs;

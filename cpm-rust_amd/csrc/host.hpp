@@ -66,6 +66,10 @@ struct Model {
     uint64_t n_cells = 0;
     std::vector<uint32_t> type_index, volume, interface, contact;
     std::vector<int64_t> sum_pos, sum_outer;
+    // Act term (crates/cpm-core/src/energy.rs:329-344): the MCS at which
+    // each site was last taken by its owner, padded-lattice indexed (halo
+    // entries stay 0); empty when the term is off
+    std::vector<uint32_t> birth;
     double energy = 0.0;
     Xoshiro rng{};
     uint64_t mcs = 0;

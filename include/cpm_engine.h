@@ -226,8 +226,17 @@ typedef struct {
     double conservative_energy;
     const uint64_t* rng_state;    /* Xoshiro256++ s[0..4], or NULL    */
     uint64_t mcs;
+    const uint32_t* birth_mcs;    /* n_sites, row-major: the Act term's
+                                     per-site birth MCS (energy.rs:
+                                     337-344), or NULL (all 0); ignored
+                                     when the term is off             */
 } cpm_model_state;
 cpm_status cpm_model_set_state(cpm_model* model, const cpm_model_state* state);
+
+/* The Act term's per-site birth MCS (ActTerm::birth_mcs, energy.rs:
+   337-344; written by ActTerm::commit, energy.rs:459-464) for the interior
+   sites, row-major, into out[n_sites]; all 0 when the term is off. */
+cpm_status cpm_model_birth_mcs(cpm_model* model, uint32_t* out, size_t n_sites);
 
 /* take_sample (crates/cpm-core/src/output.rs:289-337, with
    label_components, labelling.rs:70-154) of the model's current state, as
@@ -376,6 +385,22 @@ cpm_status cpm_engine_sp_tables(cpm_engine* engine, uint8_t* table2d, uint64_t* 
    libm).  Any output may be NULL only together with u for accept. */
 cpm_status cpm_metropolis_check(cpm_engine* engine, const double* delta_h, const double* u,
                                 size_t n, double* exp_libm, double* exp_cr, uint8_t* accept);
+
+/* The Act term's geometric-mean root (energy.rs:443: product.powf(1.0 /
+   count as f64)) as the exact-replay kernels compute it: out[i] =
+   product[i]^(1/count[i]), correctly rounded, for products of non-negative
+   integer activities (count >= 1).  engine NULL computes the same on the
+   host. */
+cpm_status cpm_act_root_check(cpm_engine* engine, const double* product, const uint32_t* count,
+                              size_t n, double* out);
+
+/* The Metropolis-Hastings decision (monte_carlo.rs:168-196) as the
+   exact-replay kernels take it: accept[i] = u[i] < min((n_a[i] / n_b[i])
+   exp(-delta_h[i]), 1), with the ratio recomputed from the correctly
+   rounded exp when u[i] lies within 2^-40 (relative) of the libm one.
+   n_a[i] >= 1, n_b[i] >= 1.  engine NULL computes the same on the host. */
+cpm_status cpm_mh_check(cpm_engine* engine, const uint32_t* n_a, const uint32_t* n_b,
+                        const double* delta_h, const double* u, size_t n, uint8_t* accept);
 
 /* Exact-replay sigma placement (no counterpart in the reference, whose
    lattices live in host memory).  A batch keeps each lattice's sigma in

@@ -439,6 +439,13 @@ struct ocpm {
     /* optional record of the Metropolis draws: (dH, u) per uphill test */
     double* mlog;
     uint64_t mlog_cap, mlog_n;
+    /* acceptance rule (OC_METROPOLIS / OC_METROPOLIS_HASTINGS), the Act
+       term (config.rs:84-103) and the connectivity check of run_mcs
+       (off only for the exact-Boltzmann checks: monte_carlo.rs:30-42) */
+    int acceptance, act, check_conn;
+    double* lambda_act;
+    uint32_t* max_act;
+    uint32_t* birth; /* energy.rs:337-344: padded-lattice indexed, all 0 */
 };
 
 static ptrdiff_t off_delta(const ocpm* m, const int32_t* o) {
@@ -566,6 +573,7 @@ ocpm* oc_model_new(int dim, const uint64_t* dims, int boundary, int copy_nb,
     }
     oc_sp_table2d(m->table2d);
     oc_rng_seed(seed, m->rng);
+    m->check_conn = 1;
     return m;
 }
 
@@ -583,8 +591,32 @@ void oc_model_free(ocpm* m) {
     free(m->target_v);
     free(m->target_i);
     free(m->J);
+    free(m->lambda_act);
+    free(m->max_act);
+    free(m->birth);
     free(m);
 }
+
+void oc_set_modes(ocpm* m, int acceptance, int act, const double* lambda_act,
+                  const uint32_t* max_act) {
+    m->acceptance = acceptance;
+    m->act = act;
+    free(m->lambda_act);
+    free(m->max_act);
+    free(m->birth);
+    m->lambda_act = NULL;
+    m->max_act = NULL;
+    m->birth = NULL;
+    if (act) {
+        m->lambda_act = (double*)malloc(sizeof(double) * m->n_types);
+        m->max_act = (uint32_t*)malloc(sizeof(uint32_t) * m->n_types);
+        memcpy(m->lambda_act, lambda_act, sizeof(double) * m->n_types);
+        memcpy(m->max_act, max_act, sizeof(uint32_t) * m->n_types);
+        m->birth = (uint32_t*)calloc(m->n_padded, sizeof(uint32_t));
+    }
+}
+
+void oc_set_connectivity_check(ocpm* m, int on) { m->check_conn = on; }
 
 void oc_set_halo_mode(ocpm* m, int mode) { m->halo_mode = mode; }
 
@@ -750,6 +782,59 @@ static int topo_ok(ocpm* m, size_t t, uint32_t L) {
     return (int)((t3[pattern >> 6] >> (pattern & 63)) & 1);
 }
 
+/* energy.rs:364-376: a flat index canonicalised to the interior site it
+   stands for (periodic only; a fixed halo stays where it is) */
+static inline size_t act_canonical(const ocpm* m, size_t flat) {
+    return m->boundary == OC_PERIODIC ? wrapped_flat(m, flat) : flat;
+}
+
+/* energy.rs:403-444: geometric mean of activity over {site} and its copy
+   neighbours, restricted to sites owned by rel, with the gaining type's
+   max_act for every retained site */
+static double act_geometric_mean(const ocpm* m, size_t site, uint32_t rel, uint32_t max_act,
+                                 uint64_t mcs) {
+    const size_t centre = act_canonical(m, site);
+    double product = 1.0;
+    uint32_t count = 0;
+    for (int k = -1; k < m->n_copy; ++k) {
+        const size_t f = act_canonical(
+            m, k < 0 ? centre : (size_t)((ptrdiff_t)centre + m->copy_off[k]));
+        if (m->cells[f] != rel) continue;
+        const uint64_t birth = m->birth[f];
+        const uint64_t elapsed = mcs > birth ? mcs - birth : 0;
+        const uint64_t activity = (uint64_t)max_act > elapsed ? (uint64_t)max_act - elapsed : 0;
+        product *= (double)activity;
+        count += 1;
+    }
+    return pow(product, 1.0 / (double)count);
+}
+
+/* energy.rs:446-457: delta_H_Act = -(lambda_Act / Max_Act) (GM_src - GM_tgt),
+   0 when medium gains */
+static double act_delta(const ocpm* m, size_t t, size_t s, uint32_t L, uint32_t G,
+                        uint64_t mcs) {
+    if (G == 0) return 0.0;
+    const uint32_t ty = m->type_index[G - 1];
+    const double lambda = m->lambda_act[ty];
+    const uint32_t max_act = m->max_act[ty];
+    const double gm_src = act_geometric_mean(m, s, G, max_act, mcs);
+    const double gm_tgt = act_geometric_mean(m, t, L, max_act, mcs);
+    return -(lambda / (double)max_act) * (gm_src - gm_tgt);
+}
+
+/* monte_carlo.rs:200-222: copy neighbours of t owned by L (n_A) and by G (n_B) */
+static void mh_counts(const ocpm* m, size_t t, uint32_t L, uint32_t G, uint32_t* na,
+                      uint32_t* nb) {
+    uint32_t a = 0, b = 0;
+    for (int k = 0; k < m->n_copy; ++k) {
+        const uint32_t c = m->cells[(size_t)((ptrdiff_t)t + m->copy_off[k])];
+        a += c == L;
+        b += c == G;
+    }
+    *na = a;
+    *nb = b;
+}
+
 /* monte_carlo.rs:37-145, 147-158 */
 static inline void attempt(ocpm* m, uint64_t* restrict rng, uint64_t* restrict cnt) {
     size_t c[3];
@@ -761,25 +846,41 @@ static inline void attempt(ocpm* m, uint64_t* restrict rng, uint64_t* restrict c
     uint32_t L = m->cells[t], G = m->cells[s];
     if (L == G) return;
     cnt[1]++;
-    if (!topo_ok(m, t, L)) return;
+    if (m->check_conn && !topo_ok(m, t, L)) return;
     if (L != 0 && m->volume[L - 1] <= 1) return;
     cnt[3]++;
-    double dh = total_delta(m, t, L, G);
-    double dh_acc = dh + 0.0; /* act term inactive */
-    int accept = dh_acc <= 0.0;
-    if (!accept) {
+    /* monte_carlo.rs:80-97: Act enters the acceptance test only */
+    const double dh = total_delta(m, t, L, G);
+    const double dh_acc = dh + (m->act ? act_delta(m, t, s, L, G, m->mcs) : 0.0);
+    int accept;
+    if (m->acceptance == OC_METROPOLIS_HASTINGS) {
+        /* monte_carlo.rs:168-196: no downhill shortcut; n_A == 0 rejects
+           without a draw */
+        uint32_t na, nb;
+        mh_counts(m, t, L, G, &na, &nb);
+        if (na == 0) return;
+        const double ratio = ((double)na / (double)nb) * exp(-dh_acc);
         const double u = oc_rng_gen_f64(rng);
-        accept = u < exp(-dh_acc);
-        if (m->mlog) {
-            if (m->mlog_n < m->mlog_cap) {
-                m->mlog[2 * m->mlog_n] = dh_acc;
-                m->mlog[2 * m->mlog_n + 1] = u;
+        accept = u < fmin(ratio, 1.0);
+    } else {
+        accept = dh_acc <= 0.0;
+        if (!accept) {
+            const double u = oc_rng_gen_f64(rng);
+            accept = u < exp(-dh_acc);
+            if (m->mlog) {
+                if (m->mlog_n < m->mlog_cap) {
+                    m->mlog[2 * m->mlog_n] = dh_acc;
+                    m->mlog[2 * m->mlog_n + 1] = u;
+                }
+                m->mlog_n++;
             }
-            m->mlog_n++;
         }
     }
     if (!accept) return;
     cnt[4]++;
+    /* energy.rs:459-464 then dynamics.rs; only the conservative part
+       reaches the energy (monte_carlo.rs:265-270) */
+    if (m->act) m->birth[t] = (uint32_t)m->mcs;
     double dl, dg;
     interface_deltas(m, t, L, G, &dl, &dg);
     commit(m, t, L, G, dl, dg);
@@ -1200,6 +1301,31 @@ uint64_t oc_metropolis_log(ocpm* m, double* buf, uint64_t cap) {
     m->mlog_cap = cap;
     m->mlog_n = 0;
     return seen;
+}
+
+/* Act state and probes (energy.rs:329-471) */
+void oc_birth_mcs(const ocpm* m, uint32_t* out) {
+    size_t idx = 0;
+    FOR_INTERIOR(m, c, { out[idx++] = m->birth ? m->birth[flat_of(m, c)] : 0; });
+}
+void oc_set_birth_mcs(ocpm* m, const uint32_t* in) {
+    if (!m->birth) return;
+    size_t idx = 0;
+    FOR_INTERIOR(m, c, { m->birth[flat_of(m, c)] = in[idx++]; });
+}
+/* delta_H_Act of copying the copy neighbour k of the interior site at
+   coordinate tc onto it, at the given MCS (the model's Act parameters) */
+double oc_act_delta(const ocpm* m, const uint64_t* tc, int k, uint64_t mcs) {
+    size_t c[3];
+    for (int d = 0; d < m->D; ++d) c[d] = (size_t)tc[d];
+    const size_t t = flat_of(m, c);
+    const size_t s = (size_t)((ptrdiff_t)t + m->copy_off[k]);
+    return act_delta(m, t, s, m->cells[t], m->cells[s], mcs);
+}
+/* pow(x[i], 1 / n[i]) with this platform's libm (the reference's
+   f64::powf in energy.rs:443) */
+void oc_root(const double* x, const uint32_t* n, uint64_t len, double* out) {
+    for (uint64_t i = 0; i < len; ++i) out[i] = pow(x[i], 1.0 / (double)n[i]);
 }
 
 /* exp(-x[i]) with this platform's libm (the reference's f64::exp). */

@@ -45,6 +45,7 @@ enum {
     OC_NB_TWENTY_SIX = 3
 };
 enum { OC_HALO_FULL_RESCAN = 0, OC_HALO_MIRROR = 1 };
+enum { OC_METROPOLIS = 0, OC_METROPOLIS_HASTINGS = 1 };
 
 typedef struct ocpm ocpm;
 
@@ -76,6 +77,18 @@ ocpm* oc_model_new(int dim, const uint64_t* dims, int boundary, int copy_nb,
                    uint64_t seed);
 void oc_model_free(ocpm* m);
 void oc_set_halo_mode(ocpm* m, int mode);
+/* acceptance rule and Act term (config.rs:84-103, 140-200); Act parameters
+   per type, ignored when act == 0 */
+void oc_set_modes(ocpm* m, int acceptance, int act, const double* lambda_act,
+                  const uint32_t* max_act);
+/* the connectivity check of each attempt (default on; the exact-Boltzmann
+   checks switch it off as exact_boltzmann.rs does) */
+void oc_set_connectivity_check(ocpm* m, int on);
+/* per-site birth MCS of the Act term, interior row-major */
+void oc_birth_mcs(const ocpm* m, uint32_t* out);
+void oc_set_birth_mcs(ocpm* m, const uint32_t* in);
+double oc_act_delta(const ocpm* m, const uint64_t* target_coord, int copy_k, uint64_t mcs);
+void oc_root(const double* x, const uint32_t* n, uint64_t len, double* out);
 /* initialization.rs:61-137 (explicit) — returns 0 or a nonzero error code */
 int oc_init_explicit(ocpm* m, const uint32_t* lattice, uint64_t n_cells,
                      const uint64_t* cell_type_of);

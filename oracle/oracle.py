@@ -110,6 +110,13 @@ def lib():
     L.oc_metropolis_log.argtypes = [C.c_void_p, f64p, C.c_uint64]
     L.oc_metropolis_log.restype = C.c_uint64
     L.oc_exp_neg.argtypes = [f64p, C.c_uint64, f64p]
+    L.oc_set_modes.argtypes = [C.c_void_p, C.c_int, C.c_int, f64p, u32p]
+    L.oc_set_connectivity_check.argtypes = [C.c_void_p, C.c_int]
+    L.oc_birth_mcs.argtypes = [C.c_void_p, u32p]
+    L.oc_set_birth_mcs.argtypes = [C.c_void_p, u32p]
+    L.oc_act_delta.argtypes = [C.c_void_p, u64p, C.c_int, C.c_uint64]
+    L.oc_act_delta.restype = C.c_double
+    L.oc_root.argtypes = [f64p, u32p, C.c_uint64, f64p]
     _lib = L
     return L
 
@@ -152,6 +159,15 @@ def exp_neg(x: np.ndarray) -> np.ndarray:
     x = np.ascontiguousarray(x, dtype=np.float64)
     out = np.empty_like(x)
     lib().oc_exp_neg(_ptr(x, C.c_double), x.size, _ptr(out, C.c_double))
+    return out
+
+
+def root(x: np.ndarray, n: np.ndarray) -> np.ndarray:
+    """pow(x, 1 / n) with this platform's libm (the reference's f64::powf)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = np.ascontiguousarray(n, dtype=np.uint32)
+    out = np.empty_like(x)
+    lib().oc_root(_ptr(x, C.c_double), _ptr(n, C.c_uint32), x.size, _ptr(out, C.c_double))
     return out
 
 
@@ -215,6 +231,8 @@ class Config:
     burn_in_mcs: int = 0
     readout_mcs: int = 1
     sampling_interval_mcs: int = 1
+    acceptance: str = "metropolis"  # or "metropolis_hastings" (config.rs:140-200)
+    act: bool = False               # active_terms.act (config.rs:84-103)
 
     @property
     def dim(self) -> int:
@@ -298,11 +316,13 @@ def convention_json(cfg: Config) -> bytes:
         "energy_neighborhood": _nb_json(cfg.energy_nb),
         "connectivity_neighborhood": _nb_json(cfg.connectivity_neighborhood),
         "boundary": "Periodic" if cfg.boundary == "periodic" else "Fixed",
-        "acceptance": "Metropolis",
+        "acceptance": ("MetropolisHastings" if cfg.acceptance == "metropolis_hastings"
+                       else "Metropolis"),
         "proposal": "Uniform",
         "cell_types": [{"name": t.name, "target_volume": int(t.target_volume),
                         "target_interface": int(t.target_interface)} for t in cfg.cell_types],
-        "active_terms": {"volume": True, "interface": True, "adhesion": True, "act": False},
+        "active_terms": {"volume": True, "interface": True, "adhesion": True,
+                         "act": bool(cfg.act)},
         "initialization": init,
         "burn_in_mcs": int(cfg.burn_in_mcs),
         "readout_mcs": int(cfg.readout_mcs),
@@ -351,6 +371,12 @@ class Model:
                                 _ptr(J, C.c_double), cfg.seed)
         if halo_mirror:
             L.oc_set_halo_mode(self.h, 1)
+        if cfg.acceptance not in ("metropolis", "metropolis_hastings"):
+            raise ValueError(f"unknown acceptance {cfg.acceptance!r}")
+        la = np.array([t.lambda_act for t in cfg.cell_types], dtype=np.float64)
+        ma = np.array([t.max_act for t in cfg.cell_types], dtype=np.uint32)
+        L.oc_set_modes(self.h, int(cfg.acceptance == "metropolis_hastings"), int(bool(cfg.act)),
+                       _ptr(la, C.c_double), _ptr(ma, C.c_uint32))
         self.used_default = cfg.init_lattice is None
         if cfg.init_lattice is not None:
             lat = np.ascontiguousarray(np.asarray(cfg.init_lattice, dtype=np.uint32).ravel())
@@ -472,8 +498,26 @@ class Model:
             "lattice": self.lattice(), "volume": self.volume(), "interface": self.interface(),
             "sum_position": self.sum_position(), "sum_outer": self.sum_outer(),
             "contact": self.contact(), "energy": self.energy, "rng": self.rng_state(),
-            "mcs": self.mcs,
+            "mcs": self.mcs, "birth_mcs": self.birth_mcs(),
         }
+
+    # --- Act term and acceptance probes (energy.rs:329-471; monte_carlo.rs)
+    def birth_mcs(self) -> np.ndarray:
+        out = np.zeros(self.n_sites, dtype=np.uint32)
+        lib().oc_birth_mcs(self.h, _ptr(out, C.c_uint32))
+        return out.reshape(self.cfg.grid)
+
+    def set_birth_mcs(self, birth) -> None:
+        b = np.ascontiguousarray(np.asarray(birth, dtype=np.uint32).ravel())
+        assert b.size == self.n_sites
+        lib().oc_set_birth_mcs(self.h, _ptr(b, C.c_uint32))
+
+    def act_delta(self, target, copy_k: int, mcs: int) -> float:
+        tc = np.array(target, dtype=np.uint64)
+        return lib().oc_act_delta(self.h, _ptr(tc, C.c_uint64), copy_k, mcs)
+
+    def set_connectivity_check(self, on: bool) -> None:
+        lib().oc_set_connectivity_check(self.h, int(bool(on)))
 
     def model_hash(self) -> str:
         return model_hash(self.cfg, self.lattice().ravel(), self.type_index())
