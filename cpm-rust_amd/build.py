@@ -29,7 +29,7 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math"
 
 HIP_SOURCES = ["engine.hip"]
 CXX_SOURCES = ["host.cpp", "capi.cpp", "rng_jump.cpp", "serve.cpp"]
-HEADERS = ["core.hpp", "metropolis.hpp", "host.hpp", "device.hpp", "rng_jump.hpp", "driver.hpp", "serve.hpp"]
+HEADERS = ["core.hpp", "binomial.hpp", "metropolis.hpp", "host.hpp", "device.hpp", "rng_jump.hpp", "driver.hpp", "serve.hpp"]
 
 
 def _run(cmd):

@@ -97,7 +97,7 @@ class CpmModelState(C.Structure):
                 ("volume", u32p), ("interface", u32p), ("sum_position", i64p),
                 ("sum_outer", i64p), ("contact", u32p), ("has_energy", C.c_int32),
                 ("conservative_energy", C.c_double), ("rng_state", u64p), ("mcs", C.c_uint64),
-                ("birth_mcs", u32p)]
+                ("birth_mcs", u32p), ("edge_sites", u64p), ("n_edge_sites", C.c_uint64)]
 
 
 class CpmResultInfo(C.Structure):
@@ -227,6 +227,10 @@ def lib():
         L.cpm_mh_check.restype = st
         L.cpm_model_birth_mcs.argtypes = [vp, u32p, C.c_size_t]
         L.cpm_model_birth_mcs.restype = st
+        L.cpm_model_edge_sites.argtypes = [vp, u64p, C.c_size_t, C.POINTER(C.c_size_t)]
+        L.cpm_model_edge_sites.restype = st
+        L.cpm_binomial_check.argtypes = [vp, u64p, u64p, f64p, C.c_size_t, u64p]
+        L.cpm_binomial_check.restype = st
         L.cpm_engine_set_pipelining.argtypes = [vp, C.c_int32]
         L.cpm_engine_set_pipelining.restype = st
         L.cpm_engine_kernel_launches.argtypes = [vp, C.POINTER(C.c_int32), u64p, C.c_size_t,
@@ -330,6 +334,22 @@ def mh_check(n_a, n_b, delta_h, u, device: bool = True):
                              b.ctypes.data_as(u32p), dh.ctypes.data_as(f64p),
                              uu.ctypes.data_as(f64p), dh.size, acc.ctypes.data_as(u8p)))
     return acc.astype(bool)
+
+
+def binomial_check(states, n, p, device: bool = True):
+    """cpm_binomial_check: one Binomial(n[i], p[i]) sample per Xoshiro256++
+    state (rows of `states`, shape (count, 4)) as the exact-replay kernels
+    draw it; returns (k, advanced states)."""
+    st = np.ascontiguousarray(np.array(states, dtype=np.uint64).reshape(-1, 4))
+    nn = np.ascontiguousarray(np.asarray(n, dtype=np.uint64))
+    pp = np.ascontiguousarray(np.asarray(p, dtype=np.float64))
+    if not (len(st) == nn.size == pp.size):
+        raise ValueError("states, n and p must have the same length")
+    k = np.zeros(nn.size, dtype=np.uint64)
+    check(lib().cpm_binomial_check(engine() if device else None, st.ctypes.data_as(u64p),
+                                   nn.ctypes.data_as(u64p), pp.ctypes.data_as(f64p), nn.size,
+                                   k.ctypes.data_as(u64p)))
+    return k, st
 
 
 def set_pipelining(on: bool) -> None:

@@ -121,13 +121,15 @@ class Replica:
 
     def set_state(self, lattice, type_index, volume=None, interface=None, sum_position=None,
                   sum_outer=None, contact=None, energy=None, rng=None, mcs: int = 0,
-                  birth_mcs=None) -> None:
+                  birth_mcs=None, edge_sites=None) -> None:
         """Replaces the whole state by the given one (the fields of CPM<D> and
         State<D>, crates/cpm-core/src/model.rs:15-38, state.rs:168-203, and
-        the Act term's per-site birth MCS, energy.rs:337-344); omitted
-        per-cell fields are derived from the lattice as initialization
-        does, an omitted energy by global_energy, an omitted rng keeps the
-        current stream, omitted births are 0."""
+        the Act term's per-site birth MCS, energy.rs:337-344, and the
+        EdgeList proposal's edge list in its order, edge_list.rs:42-45, as
+        interior row-major indices); omitted per-cell fields are derived
+        from the lattice as initialization does, an omitted energy by
+        global_energy, an omitted rng keeps the current stream, omitted
+        births are 0, an omitted edge list is rebuilt from the lattice."""
         keep = []
 
         def arr(x, dtype, ptype):
@@ -152,6 +154,8 @@ class Replica:
         st.rng_state = arr(rng, np.uint64, N.u64p)
         st.mcs = int(mcs)
         st.birth_mcs = arr(birth_mcs, np.uint32, N.u32p)
+        st.edge_sites = arr(edge_sites, np.uint64, N.u64p)
+        st.n_edge_sites = 0 if edge_sites is None else int(np.asarray(edge_sites).size)
         if birth_mcs is not None and np.asarray(birth_mcs).size != np.asarray(lattice).size:
             raise ValueError("birth_mcs must hold one MCS per site")
         if np.asarray(lattice).size != int(np.prod(self.spec["grid"])):
@@ -165,6 +169,16 @@ class Replica:
         out = np.zeros(int(np.prod(grid)), dtype=np.uint32)
         N.check(N.lib().cpm_model_birth_mcs(self.h, out.ctypes.data_as(N.u32p), out.size))
         return out.reshape(grid)
+
+    def edge_sites(self) -> np.ndarray:
+        """The EdgeList proposal's edge list in its order (interior row-major
+        site indices); empty under the uniform proposal."""
+        n = C.c_size_t(0)
+        cap = int(np.prod(self.spec["grid"]))
+        out = np.zeros(cap, dtype=np.uint64)
+        N.check(N.lib().cpm_model_edge_sites(self.h, out.ctypes.data_as(N.u64p), cap,
+                                             C.byref(n)))
+        return out[:n.value]
 
     def sample(self, include_lattice: bool = False):
         """take_sample of the current state (crates/cpm-core/src/

@@ -16,6 +16,7 @@
 #include "device.hpp"
 #include "driver.hpp"
 #include "host.hpp"
+#include "binomial.hpp"
 #include "metropolis.hpp"
 #include "serve.hpp"
 
@@ -249,6 +250,22 @@ cpm_status cpm_model_birth_mcs(cpm_model* model, uint32_t* out, size_t n_sites) 
     });
 }
 
+cpm_status cpm_model_edge_sites(cpm_model* model, uint64_t* out, size_t cap, size_t* n) {
+    return guarded([&] {
+        need(model && n && (cap == 0 || out), "model/out/n");
+        Model* m = model->m;
+        sync_to_host(m);
+        *n = (size_t)m->n_edge;
+        for (size_t i = 0; i < m->n_edge && i < cap; ++i) {
+            uint64_t c[3];
+            m->interior_coord(m->edge_sites[i], c);
+            uint64_t r = 0;
+            for (int d = 0; d < m->D; ++d) r = r * m->dims[d] + c[d];
+            out[i] = r;
+        }
+    });
+}
+
 cpm_status cpm_model_hashes(cpm_model* model, char convention_hex[65], char model_hex[65]) {
     return guarded([&] {
         need(model && convention_hex && model_hex, "model/out");
@@ -285,6 +302,9 @@ cpm_status cpm_model_set_state(cpm_model* model, const cpm_model_state* state) {
         m->sum_outer.swap(next.sum_outer);
         m->contact.swap(next.contact);
         m->birth.swap(next.birth);
+        m->edge_sites.swap(next.edge_sites);
+        m->edge_pos.swap(next.edge_pos);
+        m->n_edge = next.n_edge;
         m->energy = next.energy;
         m->rng = state->rng_state ? next.rng : m->rng;
         m->mcs = next.mcs;
@@ -612,6 +632,25 @@ cpm_status cpm_mh_check(cpm_engine* engine, const uint32_t* n_a, const uint32_t*
         for (size_t i = 0; i < n; ++i) {
             bool tie = false;
             accept[i] = mh_accept(u[i], n_a[i], n_b[i], delta_h[i], &tie) ? 1 : 0;
+        }
+    });
+}
+
+cpm_status cpm_binomial_check(cpm_engine* engine, uint64_t* rng_state, const uint64_t* n,
+                              const double* p, size_t count, uint64_t* k) {
+    return guarded([&] {
+        need((rng_state && n && p && k) || count == 0, "rng_state/n/p/k");
+        for (size_t i = 0; i < count; ++i) need(p[i] >= 0.0 && p[i] <= 1.0, "0 <= p <= 1");
+        if (engine) {
+            engine_binomial_check(engine->e, rng_state, n, p, count, k);
+            return;
+        }
+        for (size_t i = 0; i < count; ++i) {
+            Xoshiro r;
+            for (int j = 0; j < 4; ++j) r.s[j] = rng_state[4 * i + j];
+            if (!binomial_sample(r, n[i], p[i], &k[i]))
+                throw Error(CPM_ERR_DEVICE, "the binomial sampler ran away");
+            for (int j = 0; j < 4; ++j) rng_state[4 * i + j] = r.s[j];
         }
     });
 }

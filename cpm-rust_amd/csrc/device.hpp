@@ -64,6 +64,8 @@ void engine_act_root_check(Engine* e, const double* product, const uint32_t* cou
                            double* out);
 void engine_mh_check(Engine* e, const uint32_t* n_a, const uint32_t* n_b, const double* dh,
                      const double* u, size_t n, uint8_t* accept);
+void engine_binomial_check(Engine* e, uint64_t* rng_state, const uint64_t* n, const double* p,
+                           size_t count, uint64_t* k);
 // Allow (default) or forbid the pipelined two-wave exact-replay kernel.
 void engine_set_pipelining(Engine* e, bool on);
 // (instantiation key, launches) of the exact-replay wave kernels so far.

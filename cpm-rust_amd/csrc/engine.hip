@@ -36,6 +36,7 @@
 #include "device.hpp"
 #include "driver.hpp"
 #include "host.hpp"
+#include "binomial.hpp"
 #include "metropolis.hpp"
 #include "rng_jump.hpp"
 
@@ -74,6 +75,10 @@ struct Rep {
     uint32_t acceptance;      // CPM_ACCEPT_* (the lane path takes both)
     uint32_t act;             // Act term on (lane path only)
     uint32_t* birth;          // Act: per padded site, the MCS it was last taken
+    uint32_t proposal;        // CPM_PROPOSAL_* (EdgeList: lane path only)
+    uint32_t* edge_sites;     // EdgeList: edge sites (padded indices) in list order
+    uint32_t* edge_pos;       // EdgeList: each padded site's list position, or ~0
+    uint64_t n_edge;          // EdgeList: list length at the start / end of a call
     uint64_t* draws;          // kFill buffered draws + kSegs segment states (wave path)
     uint16_t* sig16;          // packed sigma in HBM (wave path with sigma outside LDS)
     uint32_t* labels;         // readouts: union-find parent per padded site
@@ -202,25 +207,62 @@ __device__ void replica_chain(Rep& R, const uint64_t* __restrict__ sp3d) {
             }
     }
 
+    // EdgeList (edge_list.rs:42-186): the list lives in HBM; its length here
+    const bool edges = R.proposal == CPM_PROPOSAL_EDGE_LIST;
+    uint64_t n_edge = R.n_edge;
+    uint32_t* __restrict__ esites = R.edge_sites;
+    uint32_t* __restrict__ epos = R.edge_pos;
+    // EdgeList::update_around (edge_list.rs:150-174): the target and its
+    // copy neighbours (a periodic halo site canonicalised, a fixed one
+    // skipped), each inserted or swap-removed by its new edge status
+    auto update_edges = [&](uint32_t t, const uint32_t* c) {
+        for (int k = -1; k < (int)n_copy; ++k) {
+            uint32_t site = t;
+            if (k >= 0) {
+                const int ax = axis[k];
+                const int64_t x = (int64_t)c[ax] + sgn[k];
+                if (x < 0 || x >= (int64_t)dims[ax]) {
+                    if (!periodic) continue;
+                    const uint32_t w = x < 0 ? dims[ax] - 1 : 0;
+                    site = (uint32_t)((int64_t)t + ((int64_t)w - (int64_t)c[ax]) * strides[ax]);
+                } else {
+                    site = (uint32_t)((int32_t)t + R.copy_off[k]);
+                }
+            }
+            const uint32_t own = sig[site];
+            bool edge = false;
+            for (uint32_t q = 0; q < n_copy; ++q) edge |= sig[(int32_t)site + R.copy_off[q]] != own;
+            const uint32_t at = epos[site];
+            if (edge) {
+                if (at == 0xffffffffu) {
+                    epos[site] = (uint32_t)n_edge;
+                    esites[n_edge++] = site;
+                }
+            } else if (at != 0xffffffffu) {
+                const uint32_t last = (uint32_t)(n_edge - 1);
+                if (at != last) {
+                    const uint32_t moved = esites[last];
+                    esites[at] = moved;
+                    epos[moved] = at;
+                }
+                n_edge -= 1;
+                epos[site] = 0xffffffffu;
+            }
+        }
+    };
+
     for (uint64_t m = 0; m < R.todo; ++m) {
         const uint64_t mcs = R.mcs + m;
-        for (uint32_t a = 0; a < R.n_sites; ++a) {
-            uint32_t c[D];
-            uint32_t t = 0;
-            for (int d = 0; d < D; ++d) {
-                uint64_t v;
-                while (!range_trial(xo_next(rng), dims[d], zone[d], &v)) {
-                }
-                c[d] = (uint32_t)v;
-                t += (c[d] + 1) * strides[d];
-            }
+        // one attempt at target t (coordinate c): monte_carlo.rs:52-145;
+        // `return` ends the attempt
+        auto attempt_at = [&](const uint32_t t, const uint32_t* c) {
             uint64_t k;
             while (!range_trial(xo_next(rng), n_copy, zone_k, &k)) {
             }
             const uint32_t s = (uint32_t)((int32_t)t + R.copy_off[k]);
             cnt[0]++;
             const uint32_t L = sig[t], G = sig[s];
-            if (L == G) continue;
+            if (L == G) return;
             cnt[1]++;
             if (L != 0) {
                 uint32_t pat = 0;
@@ -229,8 +271,8 @@ __device__ void replica_chain(Rep& R, const uint64_t* __restrict__ sp3d) {
                 cnt[2]++;
                 const bool simple = D == 2 ? ((c_sp2d[pat >> 5] >> (pat & 31)) & 1u)
                                            : ((sp3d[pat >> 6] >> (pat & 63)) & 1ull);
-                if (!simple) continue;
-                if (vol[L - 1] <= 1) continue;
+                if (!simple) return;
+                if (vol[L - 1] <= 1) return;
             }
             cnt[3]++;
             const uint32_t tl = L ? typ[L - 1] : 0, tg = G ? typ[G - 1] : 0;
@@ -281,7 +323,7 @@ __device__ void replica_chain(Rep& R, const uint64_t* __restrict__ sp3d) {
                     na += v == L;
                     nb += v == G;
                 }
-                if (na == 0) continue;
+                if (na == 0) return;
                 const double u = gen_f64(rng);
                 bool tie = false;
                 accept = mh_accept(u, na, nb, dh_acc, &tie);
@@ -295,7 +337,7 @@ __device__ void replica_chain(Rep& R, const uint64_t* __restrict__ sp3d) {
                     if (tie) cnt[5]++;
                 }
             }
-            if (!accept) continue;
+            if (!accept) return;
             cnt[4]++;
             if (R.act) R.birth[t] = (uint32_t)mcs;  // energy.rs:459-464
             for (uint32_t q = 0; q < n_energy; ++q) {
@@ -373,7 +415,39 @@ __device__ void replica_chain(Rep& R, const uint64_t* __restrict__ sp3d) {
                     }
                 }
             }
+            if (edges) update_edges(t, c);  // monte_carlo.rs:257-264
             energy += dh;
+        };
+        if (!edges) {
+            for (uint32_t a = 0; a < R.n_sites; ++a) {
+                uint32_t c[D];
+                uint32_t t = 0;
+                for (int d = 0; d < D; ++d) {
+                    uint64_t v;
+                    while (!range_trial(xo_next(rng), dims[d], zone[d], &v)) {
+                    }
+                    c[d] = (uint32_t)v;
+                    t += (c[d] + 1) * strides[d];
+                }
+                attempt_at(t, c);
+            }
+        } else {
+            // monte_carlo.rs:336-351: k ~ Binomial(N_sites, |E| / N_sites),
+            // |E| read once; k targets drawn from the list as it changes
+            uint64_t todo_k = 0;
+            if (n_edge && !binomial_sample(rng, R.n_sites, (double)n_edge / (double)R.n_sites,
+                                           &todo_k))
+                R.error = 2;
+            for (uint64_t a = 0; a < todo_k; ++a) {
+                if (n_edge == 0) break;
+                const uint32_t t = esites[gen_range(rng, n_edge)];
+                uint32_t c[D], f = t;
+                for (int d = 0; d < D; ++d) {
+                    c[d] = f / strides[d] - 1;
+                    f %= strides[d];
+                }
+                attempt_at(t, c);
+            }
         }
     }
     for (int i = 0; i < 4; ++i) R.rng[i] = rng.s[i];
@@ -381,6 +455,7 @@ __device__ void replica_chain(Rep& R, const uint64_t* __restrict__ sp3d) {
     R.mcs += R.todo;
     R.todo = 0;
     R.done = 1;
+    R.n_edge = n_edge;
     for (int i = 0; i < 6; ++i) R.cnt[i] += cnt[i];
 }
 
@@ -2358,7 +2433,7 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct Layout {
     size_t sigma, type_index, volume, interface, sum_pos, sum_outer, contact, params, targets,
-        birth, draws, sig16, labels, lcount, readout, total;
+        birth, edge_sites, edge_pos, draws, sig16, labels, lcount, readout, total;
 };
 
 static Layout layout_of(const Model& m, bool with_draws) {
@@ -2380,6 +2455,9 @@ static Layout layout_of(const Model& m, bool with_draws) {
     L.params = take((3 * T + (T + 1) * (T + 1)) * 8);
     L.targets = take(3 * T * 4);
     L.birth = take(m.cfg.act ? m.n_padded * 4 : 0);
+    const bool edges = m.cfg.proposal == CPM_PROPOSAL_EDGE_LIST;
+    L.edge_sites = take(edges ? m.n_sites * 4 : 0);
+    L.edge_pos = take(edges ? m.n_padded * 4 : 0);
     L.draws = with_draws ? take((size_t)kFill * 8 + (size_t)kSegs * 32) : 0;
     L.sig16 = with_draws ? take(m.n_padded * 2) : 0;
     L.labels = take(m.n_padded * 4);
@@ -2394,8 +2472,11 @@ static Layout layout_of(const Model& m, bool with_draws) {
 // cells and 15 types; sigma stays in LDS when it fits beside the tables.
 static bool lds_eligible(const Engine* e, const Model& m, uint32_t* bytes, bool* gsig = nullptr) {
     if (m.n_cells > kMaxPackedCells || m.n_padded > (1u << 30)) return false;
-    // Metropolis-Hastings and the Act term run on the lane path
-    if (m.cfg.acceptance != CPM_ACCEPT_METROPOLIS || m.cfg.act) return false;
+    // Metropolis-Hastings, the Act term and the EdgeList proposal run on
+    // the lane path
+    if (m.cfg.acceptance != CPM_ACCEPT_METROPOLIS || m.cfg.act ||
+        m.cfg.proposal != CPM_PROPOSAL_UNIFORM)
+        return false;
     if (m.cfg.types.size() > 15) return false;
     const uint32_t T = (uint32_t)m.cfg.types.size();
     const LdsLayout in = lds_layout((uint32_t)m.n_padded, (uint32_t)m.n_cells, T, true);
@@ -2481,6 +2562,10 @@ static void upload(Engine* e, Model* m) {
     R.acceptance = (uint32_t)m->cfg.acceptance;
     R.act = m->cfg.act ? 1u : 0u;
     R.birth = m->cfg.act ? (uint32_t*)(d->base + L.birth) : nullptr;
+    const bool edges = m->cfg.proposal == CPM_PROPOSAL_EDGE_LIST;
+    R.proposal = (uint32_t)m->cfg.proposal;
+    R.edge_sites = edges ? (uint32_t*)(d->base + L.edge_sites) : nullptr;
+    R.edge_pos = edges ? (uint32_t*)(d->base + L.edge_pos) : nullptr;
     R.draws = lds ? (uint64_t*)(d->base + L.draws) : nullptr;
     R.sig16 = lds ? (uint16_t*)(d->base + L.sig16) : nullptr;
     R.labels = (uint32_t*)(d->base + L.labels);
@@ -2513,6 +2598,10 @@ static void upload(Engine* e, Model* m) {
     put(L.params, params.data(), params.size() * 8);
     put(L.targets, targets.data(), targets.size() * 4);
     if (m->cfg.act) put(L.birth, m->birth.data(), m->n_padded * 4);
+    if (edges) {
+        put(L.edge_sites, m->edge_sites.data(), m->n_sites * 4);
+        put(L.edge_pos, m->edge_pos.data(), m->n_padded * 4);
+    }
     HIP_OK(hipStreamSynchronize(e->stream));
     m->device_dirty = false;
 }
@@ -2536,7 +2625,9 @@ static uint32_t sublattice_lds(Engine* e, const Model& m, bool* sg) {
 }
 
 bool sublattice_supported(Engine* e, const Model& m) {
-    if (m.cfg.acceptance != CPM_ACCEPT_METROPOLIS || m.cfg.act) return false;
+    if (m.cfg.acceptance != CPM_ACCEPT_METROPOLIS || m.cfg.act ||
+        m.cfg.proposal != CPM_PROPOSAL_UNIFORM)
+        return false;
     uint32_t bytes = 0;
     bool gsig = false;
     if (!lds_eligible(e, m, &bytes, &gsig)) return false;  // packed sigma words
@@ -2589,6 +2680,12 @@ void sync_to_host(Model* m) {
     if (m->cfg.act)
         HIP_OK(hipMemcpyAsync(m->birth.data(), m->dev->base + L.birth, m->n_padded * 4,
                               hipMemcpyDeviceToHost, st));
+    if (m->cfg.proposal == CPM_PROPOSAL_EDGE_LIST) {
+        HIP_OK(hipMemcpyAsync(m->edge_sites.data(), m->dev->base + L.edge_sites,
+                              m->n_sites * 4, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipMemcpyAsync(m->edge_pos.data(), m->dev->base + L.edge_pos, m->n_padded * 4,
+                              hipMemcpyDeviceToHost, st));
+    }
     HIP_OK(hipStreamSynchronize(st));
     auto put = [&](void* dst, size_t off, size_t bytes) {
         if (bytes) std::memcpy(dst, stage.data() + (off - L.sigma), bytes);
@@ -2910,6 +3007,7 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
         for (int k = 0; k < 4; ++k) R.rng[k] = m->rng.s[k];
         R.energy = m->energy;
         R.mcs = m->mcs;
+        R.n_edge = m->n_edge;
         R.todo = todo[i];
         R.attempt_in_mcs = m->attempt_in_mcs;
         R.done = todo[i] == 0;
@@ -3070,13 +3168,16 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
     }
     for (size_t i = 0; i < n; ++i)
         if (reps[i].error)
-            throw Error(CPM_ERR_DEVICE, "an attempt needed more than 128 stream draws");
+            throw Error(CPM_ERR_DEVICE,
+                        reps[i].error == 2 ? "the binomial sampler of the EdgeList proposal ran away"
+                                           : "an attempt needed more than 128 stream draws");
     for (size_t i = 0; i < n; ++i) {
         Model* m = models[i];
         if (todo[i]) {
             for (int k = 0; k < 4; ++k) m->rng.s[k] = reps[i].rng[k];
             m->energy = reps[i].energy;
             m->mcs = reps[i].mcs;
+            m->n_edge = reps[i].n_edge;
             m->attempt_in_mcs = reps[i].attempt_in_mcs;
             m->device_dirty = true;
             // the chain's own speed in this call (wave path), for schedulers
@@ -3095,10 +3196,12 @@ static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint
         bool any3d = false;
         for (size_t i = 0; i < n; ++i) {
             Model* m = models[i];
-            if (m->cfg.acceptance != CPM_ACCEPT_METROPOLIS || m->cfg.act)
+            if (m->cfg.acceptance != CPM_ACCEPT_METROPOLIS || m->cfg.act ||
+                m->cfg.proposal != CPM_PROPOSAL_UNIFORM)
                 throw Error(CPM_ERR_ARG,
-                            "sublattice mode runs the Metropolis rule without the Act term; "
-                            "use exact replay for Metropolis-Hastings or Act models");
+                            "sublattice mode runs the Metropolis rule without the Act term, "
+                            "with its own proposal; use exact replay for Metropolis-Hastings, "
+                            "Act or EdgeList models");
             upload(e, m);
             if (!m->dev->lds)
                 throw Error(CPM_ERR_ARG,
@@ -3386,6 +3489,45 @@ __global__ void k_mh_check(const uint32_t* na, const uint32_t* nb, const double*
     if (i >= n) return;
     bool tie = false;
     acc[i] = mh_accept(u[i], na[i], nb[i], dh[i], &tie) ? 1 : 0;
+}
+
+__global__ void k_binomial_check(uint64_t* st, const uint64_t* n, const double* p, uint64_t cnt,
+                                 uint64_t* k, uint32_t* bad) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cnt) return;
+    Xoshiro r;
+    for (int j = 0; j < 4; ++j) r.s[j] = st[4 * i + j];
+    if (!binomial_sample(r, n[i], p[i], &k[i])) atomicOr(bad, 1u);
+    for (int j = 0; j < 4; ++j) st[4 * i + j] = r.s[j];
+}
+
+void engine_binomial_check(Engine* e, uint64_t* rng_state, const uint64_t* n, const double* p,
+                           size_t count, uint64_t* k) {
+    std::lock_guard<std::mutex> lock(e->mu);
+    HIP_OK(hipSetDevice(e->device));
+    if (!count) return;
+    char* buf = nullptr;
+    HIP_OK(hipMalloc(&buf, count * 56 + 256));
+    uint64_t* d_st = (uint64_t*)buf;
+    uint64_t* d_n = (uint64_t*)(buf + count * 32);
+    double* d_p = (double*)(buf + count * 40);
+    uint64_t* d_k = (uint64_t*)(buf + count * 48);
+    uint32_t* d_bad = (uint32_t*)(buf + count * 56);
+    const uint32_t zero = 0;
+    HIP_OK(hipMemcpyAsync(d_st, rng_state, count * 32, hipMemcpyHostToDevice, e->stream));
+    HIP_OK(hipMemcpyAsync(d_n, n, count * 8, hipMemcpyHostToDevice, e->stream));
+    HIP_OK(hipMemcpyAsync(d_p, p, count * 8, hipMemcpyHostToDevice, e->stream));
+    HIP_OK(hipMemcpyAsync(d_bad, &zero, 4, hipMemcpyHostToDevice, e->stream));
+    hipLaunchKernelGGL(k_binomial_check, dim3((uint32_t)((count + 255) / 256)), dim3(256), 0,
+                       e->stream, d_st, d_n, d_p, (uint64_t)count, d_k, d_bad);
+    HIP_OK(hipGetLastError());
+    uint32_t bad = 0;
+    HIP_OK(hipMemcpyAsync(rng_state, d_st, count * 32, hipMemcpyDeviceToHost, e->stream));
+    HIP_OK(hipMemcpyAsync(k, d_k, count * 8, hipMemcpyDeviceToHost, e->stream));
+    HIP_OK(hipMemcpyAsync(&bad, d_bad, 4, hipMemcpyDeviceToHost, e->stream));
+    HIP_OK(hipStreamSynchronize(e->stream));
+    HIP_OK(hipFree(buf));
+    if (bad) throw Error(CPM_ERR_DEVICE, "the binomial sampler ran away");
 }
 
 void engine_act_root_check(Engine* e, const double* product, const uint32_t* count, size_t n,

@@ -187,10 +187,8 @@ void validate(const Config& c) {
                     "the exact-Boltzmann validation's reversibility argument depends on the "
                     "uniform proposal's forward/reverse symmetry, which EdgeList does not "
                     "establish");
-    // What this engine does not implement is refused loudly rather than
-    // silently run differently.
-    if (c.proposal != CPM_PROPOSAL_UNIFORM)
-        throw Error(CPM_ERR_CONFIG, "proposal = EdgeList is not supported by this engine");
+    if (c.proposal != CPM_PROPOSAL_UNIFORM && c.proposal != CPM_PROPOSAL_EDGE_LIST)
+        throw Error(CPM_ERR_CONFIG, "unknown proposal mode " + std::to_string(c.proposal));
     if (c.acceptance != CPM_ACCEPT_METROPOLIS && c.acceptance != CPM_ACCEPT_METROPOLIS_HASTINGS)
         throw Error(CPM_ERR_CONFIG, "unknown acceptance mode " + std::to_string(c.acceptance));
     if (c.types.size() > (size_t)kMaxTypes)
@@ -230,6 +228,10 @@ Model::Model(const Config& c) : cfg(c) {
     n_topo = deltas(D == 2 ? NB_MOORE : NB_TWENTY_SIX, topo_off);
     sigma.assign(n_padded, 0);
     if (cfg.act) birth.assign(n_padded, 0);
+    if (cfg.proposal == CPM_PROPOSAL_EDGE_LIST) {
+        edge_sites.assign(n_sites, 0);
+        edge_pos.assign(n_padded, kNotEdge);
+    }
     rng = xo_seed(cfg.seed);
 }
 
@@ -546,7 +548,24 @@ void Model::initialize() {
     }
     recompute(interface, sum_pos, sum_outer, contact);
     energy = global_energy();
+    build_edge_list();  // initialization.rs:427-435
     initialized = true;
+}
+
+void Model::build_edge_list() {
+    if (cfg.proposal != CPM_PROPOSAL_EDGE_LIST) return;
+    std::fill(edge_pos.begin(), edge_pos.end(), kNotEdge);
+    n_edge = 0;
+    each_interior(*this, [&](const uint64_t* c) {
+        const uint64_t f = flat(c);
+        bool edge = false;
+        for (int k = 0; k < n_copy && !edge; ++k)
+            edge = sigma[(uint64_t)((int64_t)f + copy_off[k])] != sigma[f];
+        if (edge) {
+            edge_pos[f] = (uint32_t)n_edge;
+            edge_sites[n_edge++] = (uint32_t)f;
+        }
+    });
 }
 
 // The fields of CPM<D> and State<D> (model.rs:15-38, state.rs:168-203)
@@ -601,6 +620,27 @@ void Model::set_state(const cpm_model_state& st) {
         k = 0;
         if (st.birth_mcs)
             each_interior(*this, [&](const uint64_t* c) { birth[flat(c)] = st.birth_mcs[k++]; });
+    }
+    if (cfg.proposal == CPM_PROPOSAL_EDGE_LIST) {
+        if (!st.edge_sites) {
+            build_edge_list();
+        } else {
+            std::fill(edge_pos.begin(), edge_pos.end(), kNotEdge);
+            n_edge = 0;
+            for (uint64_t i = 0; i < st.n_edge_sites; ++i) {
+                uint64_t r = st.edge_sites[i], c[3] = {0, 0, 0};
+                if (r >= n_sites) bad("edge site " + std::to_string(r) + " is not a site");
+                for (int d = D - 1; d >= 0; --d) {
+                    c[d] = r % dims[d];
+                    r /= dims[d];
+                }
+                const uint64_t f = flat(c);
+                if (edge_pos[f] != kNotEdge)
+                    bad("edge site " + std::to_string(st.edge_sites[i]) + " is listed twice");
+                edge_pos[f] = (uint32_t)n_edge;
+                edge_sites[n_edge++] = (uint32_t)f;
+            }
+        }
     }
     if (st.rng_state)
         for (int q = 0; q < 4; ++q) rng.s[q] = st.rng_state[q];

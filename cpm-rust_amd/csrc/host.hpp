@@ -70,6 +70,13 @@ struct Model {
     // each site was last taken by its owner, padded-lattice indexed (halo
     // entries stay 0); empty when the term is off
     std::vector<uint32_t> birth;
+    // EdgeList proposal (crates/cpm-core/src/edge_list.rs:42-186): the edge
+    // sites (padded flat indices) in list order, the first n_edge entries
+    // of edge_sites, and each padded site's position in the list
+    // (kNotEdge when absent); empty under the uniform proposal
+    std::vector<uint32_t> edge_sites, edge_pos;
+    uint64_t n_edge = 0;
+    static constexpr uint32_t kNotEdge = 0xffffffffu;
     double energy = 0.0;
     Xoshiro rng{};
     uint64_t mcs = 0;
@@ -100,6 +107,9 @@ struct Model {
     bool is_interior(uint64_t flat) const;
     uint64_t wrapped(uint64_t flat) const;
     void refresh_halo();
+    // EdgeList::build (edge_list.rs:121-131): interior sites in row-major
+    // order whose copy neighbourhood holds another owner
+    void build_edge_list();
     void initialize();  // initialization::initialize, throws Error(INIT)
     // Replaces the state by a caller's (cpm_model_set_state); throws
     // Error(ARG) and leaves the model unchanged on invalid input.

@@ -230,6 +230,13 @@ typedef struct {
                                      per-site birth MCS (energy.rs:
                                      337-344), or NULL (all 0); ignored
                                      when the term is off             */
+    const uint64_t* edge_sites;   /* EdgeList proposal: the edge list in
+                                     its order (edge_list.rs:42-45), as
+                                     interior row-major site indices,
+                                     each at most once; NULL rebuilds
+                                     it from the lattice (EdgeList::
+                                     build); ignored under Uniform    */
+    uint64_t n_edge_sites;
 } cpm_model_state;
 cpm_status cpm_model_set_state(cpm_model* model, const cpm_model_state* state);
 
@@ -237,6 +244,11 @@ cpm_status cpm_model_set_state(cpm_model* model, const cpm_model_state* state);
    337-344; written by ActTerm::commit, energy.rs:459-464) for the interior
    sites, row-major, into out[n_sites]; all 0 when the term is off. */
 cpm_status cpm_model_birth_mcs(cpm_model* model, uint32_t* out, size_t n_sites);
+
+/* The EdgeList proposal's edge list (edge_list.rs:42-45) in its order, as
+   interior row-major site indices: writes up to cap entries and sets *n to
+   its length (0 under the uniform proposal). */
+cpm_status cpm_model_edge_sites(cpm_model* model, uint64_t* out, size_t cap, size_t* n);
 
 /* take_sample (crates/cpm-core/src/output.rs:289-337, with
    label_components, labelling.rs:70-154) of the model's current state, as
@@ -401,6 +413,15 @@ cpm_status cpm_act_root_check(cpm_engine* engine, const double* product, const u
    n_a[i] >= 1, n_b[i] >= 1.  engine NULL computes the same on the host. */
 cpm_status cpm_mh_check(cpm_engine* engine, const uint32_t* n_a, const uint32_t* n_b,
                         const double* delta_h, const double* u, size_t n, uint8_t* accept);
+
+/* The EdgeList proposal's per-MCS draw k ~ Binomial(n, p) (monte_carlo.rs:
+   359-366, rand_distr 0.4.3 Binomial::sample) as the exact-replay kernels
+   take it: for each i, one sample from the Xoshiro256++ stream whose state
+   is rng_state[4i..4i+4], written to k[i], and the advanced state written
+   back to rng_state.  0 <= p[i] <= 1.  engine NULL computes the same on
+   the host. */
+cpm_status cpm_binomial_check(cpm_engine* engine, uint64_t* rng_state, const uint64_t* n,
+                              const double* p, size_t count, uint64_t* k);
 
 /* Exact-replay sigma placement (no counterpart in the reference, whose
    lattices live in host memory).  A batch keeps each lattice's sigma in

@@ -446,6 +446,12 @@ struct ocpm {
     double* lambda_act;
     uint32_t* max_act;
     uint32_t* birth; /* energy.rs:337-344: padded-lattice indexed, all 0 */
+    /* ProposalMode::EdgeList (edge_list.rs:42-186): the edge sites in
+       their list order and each padded site's position in it */
+    int edge_list;
+    size_t* edge_sites;
+    uint32_t* edge_pos;
+    size_t n_edge;
 };
 
 static ptrdiff_t off_delta(const ocpm* m, const int32_t* o) {
@@ -594,6 +600,8 @@ void oc_model_free(ocpm* m) {
     free(m->lambda_act);
     free(m->max_act);
     free(m->birth);
+    free(m->edge_sites);
+    free(m->edge_pos);
     free(m);
 }
 
@@ -617,6 +625,205 @@ void oc_set_modes(ocpm* m, int acceptance, int act, const double* lambda_act,
 }
 
 void oc_set_connectivity_check(ocpm* m, int on) { m->check_conn = on; }
+
+void oc_set_proposal(ocpm* m, int edge_list) {
+    m->edge_list = edge_list;
+    free(m->edge_sites);
+    free(m->edge_pos);
+    m->edge_sites = NULL;
+    m->edge_pos = NULL;
+    m->n_edge = 0;
+    if (edge_list) {
+        m->edge_sites = (size_t*)malloc(sizeof(size_t) * m->n_sites);
+        m->edge_pos = (uint32_t*)malloc(sizeof(uint32_t) * m->n_padded);
+        for (size_t i = 0; i < m->n_padded; ++i) m->edge_pos[i] = UINT32_MAX;
+    }
+}
+
+/* ---- edge list (edge_list.rs:77-186) ---- */
+static void edge_insert(ocpm* m, size_t site) {
+    if (m->edge_pos[site] != UINT32_MAX) return;
+    m->edge_pos[site] = (uint32_t)m->n_edge;
+    m->edge_sites[m->n_edge++] = site;
+}
+static void edge_remove(ocpm* m, size_t site) {
+    const uint32_t idx = m->edge_pos[site];
+    if (idx == UINT32_MAX) return;
+    const size_t last = m->n_edge - 1;
+    if (idx != last) {
+        const size_t moved = m->edge_sites[last];
+        m->edge_sites[idx] = moved;
+        m->edge_pos[moved] = idx;
+    }
+    m->n_edge--;
+    m->edge_pos[site] = UINT32_MAX;
+}
+static int edge_site(const ocpm* m, size_t site) {
+    const uint32_t owner = m->cells[site];
+    for (int k = 0; k < m->n_copy; ++k)
+        if (m->cells[(size_t)((ptrdiff_t)site + m->copy_off[k])] != owner) return 1;
+    return 0;
+}
+/* from-scratch build, interior sites in row-major order */
+static void edge_build(ocpm* m) {
+    for (size_t i = 0; i < m->n_padded; ++i) m->edge_pos[i] = UINT32_MAX;
+    m->n_edge = 0;
+    for (size_t f = 0; f < m->n_padded; ++f)
+        if (is_interior(m, f) && edge_site(m, f)) edge_insert(m, f);
+}
+/* after an accepted copy onto t: t, then each copy neighbour (a periodic
+   halo site canonicalised, a fixed one skipped) */
+static void edge_update_around(ocpm* m, size_t t) {
+    for (int k = -1; k < m->n_copy; ++k) {
+        size_t site = k < 0 ? t : (size_t)((ptrdiff_t)t + m->copy_off[k]);
+        if (!is_interior(m, site)) {
+            if (m->boundary != OC_PERIODIC) continue;
+            site = wrapped_flat(m, site);
+        }
+        if (edge_site(m, site)) edge_insert(m, site);
+        else edge_remove(m, site);
+    }
+}
+
+/* ---- rand 0.8 UniformFloat<f64> (uniform.rs): low + scale * [0, 1) with
+   the [0, 1) value from 52 random bits as (1.x) - 1 ---- */
+static double uniform_f64_scale(double low, double high) {
+    const double max_rand = 1.0 - 0x1p-52;
+    double scale = high - low;
+    while (scale * max_rand + low >= high) scale = nextafter(scale, 0.0);
+    return scale;
+}
+static double uniform_f64_sample(uint64_t* rng, double low, double scale) {
+    const uint64_t bits = (oc_rng_next(rng) >> 12) | 0x3FF0000000000000ull;
+    double v12;
+    memcpy(&v12, &bits, 8);
+    return (v12 - 1.0) * scale + low;
+}
+
+/* compiler-rt __powidf2: x^n by binary powering (Rust's f64::powi) */
+static double powi_f64(double a, int32_t b) {
+    const int recip = b < 0;
+    double r = 1.0;
+    for (;;) {
+        if (b & 1) r *= a;
+        b /= 2;
+        if (b == 0) break;
+        a *= a;
+    }
+    return recip ? 1.0 / r : r;
+}
+
+/* Rust's `x as i64`: truncation toward zero, saturating, NaN -> 0 (a
+   sample of v = 0 gives log(v) = -inf in the tails) */
+static inline int64_t f64_to_i64(double x) {
+    if (x != x) return 0;
+    if (x <= -0x1p63) return INT64_MIN;
+    if (x >= 0x1p63) return INT64_MAX;
+    return (int64_t)x;
+}
+
+static double btpe_stirling(double a) {
+    const double a2 = a * a;
+    return (13860.0 - (462.0 - (132.0 - (99.0 - 140.0 / a2) / a2) / a2) / a2) / a / 166320.0;
+}
+
+/* rand_distr 0.4.3 Binomial::sample (binomial.rs): BINV inversion when
+   n * min(p, 1 - p) < 10, else BTPE (Kachitvichyanukul & Schmeiser 1988,
+   with GSL's signs in the final test); results for p > 1/2 mirrored.
+   Returns 0 without a draw for p == 0 and n for p == 1. */
+uint64_t oc_binomial(uint64_t rng[4], uint64_t n_u, double p_in) {
+    if (p_in == 0.0) return 0;
+    if (p_in == 1.0) return n_u;
+    const double p = p_in <= 0.5 ? p_in : 1.0 - p_in;
+    const double q = 1.0 - p;
+    uint64_t result;
+    if ((double)n_u * p < 10.0 && n_u <= (uint64_t)INT32_MAX) {
+        const double s = p / q;
+        const double a = (double)(n_u + 1) * s;
+        double r = powi_f64(q, (int32_t)n_u);
+        double u = oc_rng_gen_f64(rng);
+        uint64_t x = 0;
+        while (u > r) {
+            u -= r;
+            x += 1;
+            r *= a / (double)x - s;
+        }
+        result = x;
+    } else {
+        const double n = (double)n_u;
+        const double np = n * p;
+        const double npq = np * q;
+        const double f_m = np + p;
+        const int64_t m = f64_to_i64(f_m);
+        const double p1 = floor(2.195 * sqrt(npq) - 4.6 * q) + 0.5;
+        const double x_m = (double)m + 0.5;
+        const double x_l = x_m - p1;
+        const double x_r = x_m + p1;
+        const double c = 0.134 + 20.5 / (15.3 + (double)m);
+        const double p2 = p1 * (1.0 + 2.0 * c);
+        const double al = (f_m - x_l) / (f_m - x_l * p);
+        const double lambda_l = al * (1.0 + 0.5 * al);
+        const double ar = (x_r - f_m) / (x_r * q);
+        const double lambda_r = ar * (1.0 + 0.5 * ar);
+        const double p3 = p2 + c / lambda_l;
+        const double p4 = p3 + c / lambda_r;
+        const double scale_u = uniform_f64_scale(0.0, p4);
+        const double scale_v = uniform_f64_scale(0.0, 1.0);
+        int64_t y;
+        for (;;) {
+            const double u = uniform_f64_sample(rng, 0.0, scale_u);
+            double v = uniform_f64_sample(rng, 0.0, scale_v);
+            if (!(u > p1)) {
+                y = f64_to_i64(x_m - p1 * v + u);
+                break;
+            }
+            if (!(u > p2)) {
+                const double x = x_l + (u - p1) / c;
+                v = v * c + 1.0 - fabs(x - x_m) / p1;
+                if (v > 1.0) continue;
+                y = f64_to_i64(x);
+            } else if (!(u > p3)) {
+                y = f64_to_i64(x_l + log(v) / lambda_l);
+                if (y < 0) continue;
+                v *= (u - p2) * lambda_l;
+            } else {
+                y = f64_to_i64(x_r - log(v) / lambda_r);
+                if (y > 0 && (uint64_t)y > n_u) continue;
+                v *= (u - p3) * lambda_r;
+            }
+            const int64_t k = y > m ? y - m : m - y;
+            if (!(k > 20 && (double)k < 0.5 * npq - 1.0)) {
+                const double s = p / q;
+                const double a = s * (n + 1.0);
+                double f = 1.0;
+                if (m < y) {
+                    for (int64_t i = m + 1; i <= y; ++i) f *= a / (double)i - s;
+                } else if (m > y) {
+                    for (int64_t i = y + 1; i <= m; ++i) f /= a / (double)i - s;
+                }
+                if (v > f) continue;
+                break;
+            }
+            const double kf = (double)k;
+            const double rho = (kf / npq) * ((kf * (kf / 3.0 + 0.625) + 1.0 / 6.0) / npq + 0.5);
+            const double t = -0.5 * kf * kf / npq;
+            const double alpha = log(v);
+            if (alpha < t - rho) break;
+            if (alpha > t + rho) continue;
+            const double x1 = (double)(y + 1);
+            const double f1 = (double)(m + 1);
+            const double z = (double)(f64_to_i64(n) + 1 - m);
+            const double w = (double)(f64_to_i64(n) - y + 1);
+            if (alpha > x_m * log(f1 / x1) + (n - (double)m + 0.5) * log(z / w) +
+                            (double)(y - m) * log(w * p / (x1 * q)) + btpe_stirling(f1) +
+                            btpe_stirling(z) - btpe_stirling(x1) - btpe_stirling(w))
+                continue;
+            break;
+        }
+        result = (uint64_t)y;
+    }
+    return p != p_in ? n_u - result : result;
+}
 
 void oc_set_halo_mode(ocpm* m, int mode) { m->halo_mode = mode; }
 
@@ -835,11 +1042,8 @@ static void mh_counts(const ocpm* m, size_t t, uint32_t L, uint32_t G, uint32_t*
     *nb = b;
 }
 
-/* monte_carlo.rs:37-145, 147-158 */
-static inline void attempt(ocpm* m, uint64_t* restrict rng, uint64_t* restrict cnt) {
-    size_t c[3];
-    for (int d = 0; d < m->D; ++d) c[d] = (size_t)oc_rng_gen_range(rng, m->dims[d]);
-    size_t t = flat_of(m, c);
+/* monte_carlo.rs:52-145, 147-158: one attempt at target t */
+static inline void attempt_at(ocpm* m, uint64_t* restrict rng, uint64_t* restrict cnt, size_t t) {
     size_t k = (size_t)oc_rng_gen_range(rng, (uint64_t)m->n_copy);
     size_t s = (size_t)((ptrdiff_t)t + m->copy_off[k]);
     cnt[0]++;
@@ -884,7 +1088,15 @@ static inline void attempt(ocpm* m, uint64_t* restrict rng, uint64_t* restrict c
     double dl, dg;
     interface_deltas(m, t, L, G, &dl, &dg);
     commit(m, t, L, G, dl, dg);
+    if (m->edge_list) edge_update_around(m, t); /* monte_carlo.rs:257-264 */
     m->energy += dh;
+}
+
+/* monte_carlo.rs:27-42: a uniformly drawn interior target */
+static inline void attempt(ocpm* m, uint64_t* restrict rng, uint64_t* restrict cnt) {
+    size_t c[3];
+    for (int d = 0; d < m->D; ++d) c[d] = (size_t)oc_rng_gen_range(rng, m->dims[d]);
+    attempt_at(m, rng, cnt, flat_of(m, c));
 }
 
 void oc_run_mcs(ocpm* m, uint64_t n) {
@@ -894,7 +1106,21 @@ void oc_run_mcs(ocpm* m, uint64_t n) {
     memcpy(rng, m->rng, sizeof rng);
     memcpy(cnt, m->counters, sizeof cnt);
     for (uint64_t i = 0; i < n; ++i) {
-        for (size_t a = 0; a < m->n_sites; ++a) attempt(m, rng, cnt);
+        if (!m->edge_list) {
+            for (size_t a = 0; a < m->n_sites; ++a) attempt(m, rng, cnt);
+        } else {
+            /* monte_carlo.rs:336-351: k ~ Binomial(N_sites, |E| / N_sites)
+               with |E| read once, then k targets drawn from E */
+            const uint64_t k = m->n_edge == 0
+                                   ? 0
+                                   : oc_binomial(rng, m->n_sites,
+                                                 (double)m->n_edge / (double)m->n_sites);
+            for (uint64_t a = 0; a < k; ++a) {
+                if (m->n_edge == 0) break;
+                const size_t t = m->edge_sites[oc_rng_gen_range(rng, m->n_edge)];
+                attempt_at(m, rng, cnt, t);
+            }
+        }
         m->mcs += 1;
     }
     memcpy(m->rng, rng, sizeof rng);
@@ -992,6 +1218,8 @@ static void alloc_cells(ocpm* m, size_t n) {
 static void finish_init(ocpm* m) {
     oc_recompute(m, m->interface, m->sum_pos, m->sum_outer, m->contact);
     m->energy = oc_global_energy(m);
+    /* initialization.rs:427-435 */
+    if (m->edge_list) edge_build(m);
 }
 
 /* initialization.rs:145-178: BFS connectivity under von Neumann */
@@ -1322,6 +1550,34 @@ double oc_act_delta(const ocpm* m, const uint64_t* tc, int k, uint64_t mcs) {
     const size_t s = (size_t)((ptrdiff_t)t + m->copy_off[k]);
     return act_delta(m, t, s, m->cells[t], m->cells[s], mcs);
 }
+/* the edge list in its order, as interior row-major indices; returns |E| */
+uint64_t oc_edge_sites(const ocpm* m, uint64_t* out, uint64_t cap) {
+    if (!m->edge_list) return 0;
+    for (size_t i = 0; i < m->n_edge && i < cap; ++i) {
+        size_t c[3];
+        interior_coord(m, m->edge_sites[i], c);
+        uint64_t r = 0;
+        for (int d = 0; d < m->D; ++d) r = r * m->dims[d] + c[d];
+        out[i] = r;
+    }
+    return m->n_edge;
+}
+/* replaces the edge list by the given order (interior row-major indices) */
+void oc_set_edge_sites(ocpm* m, const uint64_t* in, uint64_t n) {
+    if (!m->edge_list) return;
+    for (size_t i = 0; i < m->n_padded; ++i) m->edge_pos[i] = UINT32_MAX;
+    m->n_edge = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        size_t c[3];
+        uint64_t r = in[i];
+        for (int d = m->D - 1; d >= 0; --d) {
+            c[d] = (size_t)(r % m->dims[d]);
+            r /= m->dims[d];
+        }
+        edge_insert(m, flat_of(m, c));
+    }
+}
+
 /* pow(x[i], 1 / n[i]) with this platform's libm (the reference's
    f64::powf in energy.rs:443) */
 void oc_root(const double* x, const uint32_t* n, uint64_t len, double* out) {

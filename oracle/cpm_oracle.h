@@ -84,6 +84,13 @@ void oc_set_modes(ocpm* m, int acceptance, int act, const double* lambda_act,
 /* the connectivity check of each attempt (default on; the exact-Boltzmann
    checks switch it off as exact_boltzmann.rs does) */
 void oc_set_connectivity_check(ocpm* m, int on);
+/* ProposalMode::EdgeList (monte_carlo.rs:336-351; edge_list.rs): call
+   before initialisation */
+void oc_set_proposal(ocpm* m, int edge_list);
+uint64_t oc_edge_sites(const ocpm* m, uint64_t* out, uint64_t cap);
+void oc_set_edge_sites(ocpm* m, const uint64_t* in, uint64_t n);
+/* rand_distr 0.4.3 Binomial(n, p).sample on the given stream */
+uint64_t oc_binomial(uint64_t state[4], uint64_t n, double p);
 /* per-site birth MCS of the Act term, interior row-major */
 void oc_birth_mcs(const ocpm* m, uint32_t* out);
 void oc_set_birth_mcs(ocpm* m, const uint32_t* in);

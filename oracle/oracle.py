@@ -117,6 +117,12 @@ def lib():
     L.oc_act_delta.argtypes = [C.c_void_p, u64p, C.c_int, C.c_uint64]
     L.oc_act_delta.restype = C.c_double
     L.oc_root.argtypes = [f64p, u32p, C.c_uint64, f64p]
+    L.oc_set_proposal.argtypes = [C.c_void_p, C.c_int]
+    L.oc_edge_sites.argtypes = [C.c_void_p, u64p, C.c_uint64]
+    L.oc_edge_sites.restype = C.c_uint64
+    L.oc_set_edge_sites.argtypes = [C.c_void_p, u64p, C.c_uint64]
+    L.oc_binomial.argtypes = [u64p, C.c_uint64, C.c_double]
+    L.oc_binomial.restype = C.c_uint64
     _lib = L
     return L
 
@@ -145,6 +151,10 @@ class Rng:
 
     def gen_f64(self) -> float:
         return float(lib().oc_rng_gen_f64(_ptr(self.s, C.c_uint64)))
+
+    def binomial(self, n: int, p: float) -> int:
+        """rand_distr 0.4.3 Binomial::new(n, p).sample on this stream."""
+        return int(lib().oc_binomial(_ptr(self.s, C.c_uint64), n, p))
 
 
 def blake3_hex(data: bytes) -> str:
@@ -233,6 +243,7 @@ class Config:
     sampling_interval_mcs: int = 1
     acceptance: str = "metropolis"  # or "metropolis_hastings" (config.rs:140-200)
     act: bool = False               # active_terms.act (config.rs:84-103)
+    proposal: str = "uniform"       # or "edge_list" (config.rs:42-70)
 
     @property
     def dim(self) -> int:
@@ -318,7 +329,7 @@ def convention_json(cfg: Config) -> bytes:
         "boundary": "Periodic" if cfg.boundary == "periodic" else "Fixed",
         "acceptance": ("MetropolisHastings" if cfg.acceptance == "metropolis_hastings"
                        else "Metropolis"),
-        "proposal": "Uniform",
+        "proposal": "EdgeList" if cfg.proposal == "edge_list" else "Uniform",
         "cell_types": [{"name": t.name, "target_volume": int(t.target_volume),
                         "target_interface": int(t.target_interface)} for t in cfg.cell_types],
         "active_terms": {"volume": True, "interface": True, "adhesion": True,
@@ -377,6 +388,9 @@ class Model:
         ma = np.array([t.max_act for t in cfg.cell_types], dtype=np.uint32)
         L.oc_set_modes(self.h, int(cfg.acceptance == "metropolis_hastings"), int(bool(cfg.act)),
                        _ptr(la, C.c_double), _ptr(ma, C.c_uint32))
+        if cfg.proposal not in ("uniform", "edge_list"):
+            raise ValueError(f"unknown proposal {cfg.proposal!r}")
+        L.oc_set_proposal(self.h, int(cfg.proposal == "edge_list"))
         self.used_default = cfg.init_lattice is None
         if cfg.init_lattice is not None:
             lat = np.ascontiguousarray(np.asarray(cfg.init_lattice, dtype=np.uint32).ravel())
@@ -515,6 +529,17 @@ class Model:
     def act_delta(self, target, copy_k: int, mcs: int) -> float:
         tc = np.array(target, dtype=np.uint64)
         return lib().oc_act_delta(self.h, _ptr(tc, C.c_uint64), copy_k, mcs)
+
+    def edge_sites(self) -> np.ndarray:
+        """The EdgeList proposal's edge sites in list order (interior
+        row-major indices); empty under the uniform proposal."""
+        out = np.zeros(self.n_sites, dtype=np.uint64)
+        n = lib().oc_edge_sites(self.h, _ptr(out, C.c_uint64), out.size)
+        return out[:n]
+
+    def set_edge_sites(self, sites) -> None:
+        a = np.ascontiguousarray(np.asarray(sites, dtype=np.uint64))
+        lib().oc_set_edge_sites(self.h, _ptr(a, C.c_uint64), a.size)
 
     def set_connectivity_check(self, on: bool) -> None:
         lib().oc_set_connectivity_check(self.h, int(bool(on)))
