@@ -1986,7 +1986,7 @@ __device__ __forceinline__ bool neighbour_site(const Rep& R, const uint32_t* c, 
     return inside || R.boundary == CPM_BOUNDARY_PERIODIC;
 }
 
-constexpr uint32_t kRoThreads = 256;
+constexpr uint32_t kRoThreads = 1024;
 __global__ void __launch_bounds__(kRoThreads) k_readout(Rep* __restrict__ reps,
                                                         const uint32_t* __restrict__ idx) {
     __shared__ uint32_t s_wave[kRoThreads / 64];
