@@ -2246,7 +2246,8 @@ __global__ void __launch_bounds__(256) k_sp3d(SpMasks M, uint64_t* __restrict__ 
 struct Device {
     char* base = nullptr;
     Rep rep{};  // device pointers + static fields
-    bool lds = false;     // wave path
+    bool lds = false;     // packed sigma words fit (LDS or HBM copy): wave and sublattice paths
+    bool wave = false;    // exact replay on the wave path (else the lane path)
     bool gsig = false;    // sigma does not fit in LDS: always in HBM
     uint32_t lds_bytes = 0;       // LDS with sigma in LDS (when it fits)
     uint32_t lds_bytes_hbm = 0;   // LDS with sigma in HBM
@@ -2472,11 +2473,8 @@ static Layout layout_of(const Model& m, bool with_draws) {
 // cells and 15 types; sigma stays in LDS when it fits beside the tables.
 static bool lds_eligible(const Engine* e, const Model& m, uint32_t* bytes, bool* gsig = nullptr) {
     if (m.n_cells > kMaxPackedCells || m.n_padded > (1u << 30)) return false;
-    // Metropolis-Hastings, the Act term and the EdgeList proposal run on
-    // the lane path
-    if (m.cfg.acceptance != CPM_ACCEPT_METROPOLIS || m.cfg.act ||
-        m.cfg.proposal != CPM_PROPOSAL_UNIFORM)
-        return false;
+    // Metropolis-Hastings and the Act term run on the lane path only
+    if (m.cfg.acceptance != CPM_ACCEPT_METROPOLIS || m.cfg.act) return false;
     if (m.cfg.types.size() > 15) return false;
     const uint32_t T = (uint32_t)m.cfg.types.size();
     const LdsLayout in = lds_layout((uint32_t)m.n_padded, (uint32_t)m.n_cells, T, true);
@@ -2510,6 +2508,9 @@ static void upload(Engine* e, Model* m) {
     d->device = e->device;
     d->eng = e;
     d->lds = lds;
+    // the EdgeList proposal runs exact replay on the lane path (the
+    // sublattice mode takes it on the packed words)
+    d->wave = lds && m->cfg.proposal == CPM_PROPOSAL_UNIFORM;
     d->gsig = lds && gsig;
     d->lds_bytes = lds_bytes;
     d->lds_bytes_hbm = lds_layout((uint32_t)m->n_padded, (uint32_t)m->n_cells,
@@ -2625,9 +2626,7 @@ static uint32_t sublattice_lds(Engine* e, const Model& m, bool* sg) {
 }
 
 bool sublattice_supported(Engine* e, const Model& m) {
-    if (m.cfg.acceptance != CPM_ACCEPT_METROPOLIS || m.cfg.act ||
-        m.cfg.proposal != CPM_PROPOSAL_UNIFORM)
-        return false;
+    if (m.cfg.acceptance != CPM_ACCEPT_METROPOLIS || m.cfg.act) return false;
     uint32_t bytes = 0;
     bool gsig = false;
     if (!lds_eligible(e, m, &bytes, &gsig)) return false;  // packed sigma words
@@ -2920,7 +2919,7 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
         std::vector<uint32_t> cand;
         for (size_t i = 0; i < n; ++i) {
             const Model* m = models[i];
-            if (!todo[i] || !m->dev->lds) continue;
+            if (!todo[i] || !m->dev->wave) continue;
             ++waves;
             max_work = std::max(max_work, (double)todo[i] * (double)m->n_sites);
             if (m->dev->lds_bytes_hbm + pipe_extra_bytes(m->D) + kLdsStaticReserve <= e->max_lds)
@@ -2951,7 +2950,7 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
         double demand = 0.0;
         for (size_t i = 0; i < n; ++i) {
             const Model* m = models[i];
-            if (!todo[i] || !m->dev->lds) continue;
+            if (!todo[i] || !m->dev->wave) continue;
             in_hbm[i] = 1;
             demand += (double)lds_class(e, lds_need(i, true));
             if (!m->dev->gsig && lds_need(i, false) + kLdsStaticReserve <= e->max_lds)
@@ -2987,7 +2986,7 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
         std::set<int> keys;
         bool any_piped = false;
         for (size_t i = 0; i < n; ++i)
-            if (todo[i] && models[i]->dev->lds) {
+            if (todo[i] && models[i]->dev->wave) {
                 keys.insert((piped[i] ? 2000 : 0) + (in_hbm[i] ? 1000 : 0) + models[i]->D * 100 +
                             models[i]->n_energy);
                 any_piped |= piped[i] != 0;
@@ -3016,13 +3015,13 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
         R.t_run = 0;
         // 100 MHz ticks (s_memrealtime); the lane path runs whole MCS only,
         // one per time-limited call
-        R.budget = budget_s > 0 && m->dev->lds ? std::max<uint64_t>(1, (uint64_t)(budget_s * 1e8)) : 0;
-        if (budget_s > 0 && !m->dev->lds) R.todo = std::min<uint64_t>(R.todo, 1);
-        if (m->attempt_in_mcs && !m->dev->lds)
+        R.budget = budget_s > 0 && m->dev->wave ? std::max<uint64_t>(1, (uint64_t)(budget_s * 1e8)) : 0;
+        if (budget_s > 0 && !m->dev->wave) R.todo = std::min<uint64_t>(R.todo, 1);
+        if (m->attempt_in_mcs && !m->dev->wave)
             throw Error(CPM_ERR_DEVICE, "a lane-path model stopped mid-MCS");
         reps[i] = R;
         if (!todo[i]) continue;
-        if (m->dev->lds)
+        if (m->dev->wave)
             by_kernel[(budget_s > 0 ? 4000 : 0) + (piped[i] ? 2000 : 0) + (in_hbm[i] ? 1000 : 0) +
                       m->D * 100 + m->n_energy]
                 .push_back((uint32_t)i);
@@ -3196,12 +3195,10 @@ static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint
         bool any3d = false;
         for (size_t i = 0; i < n; ++i) {
             Model* m = models[i];
-            if (m->cfg.acceptance != CPM_ACCEPT_METROPOLIS || m->cfg.act ||
-                m->cfg.proposal != CPM_PROPOSAL_UNIFORM)
+            if (m->cfg.acceptance != CPM_ACCEPT_METROPOLIS || m->cfg.act)
                 throw Error(CPM_ERR_ARG,
-                            "sublattice mode runs the Metropolis rule without the Act term, "
-                            "with its own proposal; use exact replay for Metropolis-Hastings, "
-                            "Act or EdgeList models");
+                            "sublattice mode runs the Metropolis rule without the Act term; "
+                            "use exact replay for Metropolis-Hastings or Act models");
             upload(e, m);
             if (!m->dev->lds)
                 throw Error(CPM_ERR_ARG,
@@ -3303,8 +3300,23 @@ static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint
     // launch)
     parallel_for(n, [&](size_t i) {
         if (!todo[i]) return;
-        sync_lattice_tables(models[i]);
-        models[i]->energy = models[i]->global_energy();
+        Model* m = models[i];
+        sync_lattice_tables(m);
+        m->energy = m->global_energy();
+        // EdgeList models: the mode's proposal is the uniform one, which the
+        // edge list's is statistically (config.rs:42-70: the same chain
+        // without its null attempts); the list is rebuilt from the new
+        // lattice as EdgeList::build would (edge_list.rs:121-131)
+        if (m->cfg.proposal == CPM_PROPOSAL_EDGE_LIST) {
+            m->build_edge_list();
+            const Layout L = layout_of(*m, m->dev->lds);
+            hipStream_t st = copy_stream_for(e);
+            HIP_OK(hipMemcpyAsync(m->dev->base + L.edge_sites, m->edge_sites.data(),
+                                  m->n_sites * 4, hipMemcpyHostToDevice, st));
+            HIP_OK(hipMemcpyAsync(m->dev->base + L.edge_pos, m->edge_pos.data(),
+                                  m->n_padded * 4, hipMemcpyHostToDevice, st));
+            HIP_OK(hipStreamSynchronize(st));
+        }
     });
 }
 
@@ -3315,7 +3327,7 @@ static uint64_t batch_signature(Engine* e, Model* const* models, size_t n, const
     double demand = 0.0;
     uint64_t dims = 0, count = 0;
     for (size_t i = 0; i < n; ++i) {
-        if (!todo[i] || !models[i]->dev || !models[i]->dev->lds) continue;
+        if (!todo[i] || !models[i]->dev || !models[i]->dev->wave) continue;
         demand += (double)models[i]->dev->lds_bytes;
         dims |= 1ull << models[i]->D;
         ++count;

@@ -157,7 +157,7 @@ cpm_status cpm_run_mcs_plain(cpm_engine* engine, cpm_model* model, uint64_t n_mc
    built by that call), and the call's replica descriptors and launch order
    live in a fixed device array of the library, at most
    CPM_NOALLOC_MAX_REPLICAS models per call (larger n returns CPM_ERR_ARG;
-   the array takes about 2.2 MiB of each device the library runs on, until
+   the array takes about 2.5 MiB of each device the library runs on, until
    the process ends).  The results are bit-identical to cpm_run_mcs.  Every
    call on one device shares that array: calls that overlap on one device
    race and must not be made.  counters->replay_launch_ms stays 0 (no

@@ -143,8 +143,48 @@ def test_device_binomial_equals_the_oracle():
         assert np.array_equal(r.s, st_dev[i]), i
 
 
-def test_sublattice_mode_refuses_edge_list():
-    sim, _ = pair(seed=141, grid=(32, 32))
-    with pytest.raises(ValueError, match="use exact replay for Metropolis-Hastings, Act or "
-                                         "EdgeList models"):
-        cpm.Replica(sim).run_mcs(2, mode="sublattice")
+def test_sublattice_mode_runs_edge_list_models_and_rebuilds_the_list():
+    """The sublattice mode takes an EdgeList model with its own (uniform)
+    proposal, which the reference defines as the same chain with its null
+    attempts kept (config.rs:42-70); after the call the edge list is the
+    rebuild of the new lattice, in EdgeList::build's order."""
+    sim, cfg = pair(seed=141, grid=(24, 24, 24), boundary="fixed", phi=0.1, tv=60, ti=90)
+    rep = cpm.Replica(sim)
+    rep.run_mcs(12, mode="sublattice")
+    snap = rep.snapshot()
+    ref = O.Model(cfg)
+    ref.load_state(snap["lattice"].ravel(), snap["type_index"])
+    assert np.array_equal(rep.edge_sites(), ref.edge_sites())
+    itf, sp, so, ct = ref.recompute()
+    assert np.array_equal(snap["volume"], ref.volume())
+    assert np.array_equal(snap["interface"], itf)
+    assert np.array_equal(snap["contact"], ct)
+    assert abs(snap["energy"] - ref.global_energy()) <= 1e-9 * abs(ref.global_energy())
+    # exact replay carries on from the rebuilt list, keeping it consistent
+    rep.run_mcs(5)
+    s2 = rep.snapshot()
+    assert s2["mcs"] == 17
+    ref.load_state(s2["lattice"].ravel(), s2["type_index"])
+    assert np.array_equal(np.sort(rep.edge_sites()), np.sort(ref.edge_sites()))
+
+
+def test_sublattice_edge_list_ks_against_exact_edge_list():
+    """256 seeds of a sparse 32^3 fixed box (three types, phi = 0.2), every
+    replica first 20 MCS of exact EdgeList replay, then 40 MCS either of
+    exact EdgeList replay (lane path) or of sublattice mode: every summary
+    statistic passes a two-sample KS test at p > 0.01."""
+    from tests.test_gpu_sublattice import SEEDS, _ks, organoid3d_sim, trajectory_stats
+
+    seeds = [cpm.derive_seed_u64(2031, i) for i in range(SEEDS)]
+    out = {}
+    for mode in ("exact_replay", "sublattice"):
+        reps = [cpm.Replica(organoid3d_sim(sd, phi=0.2, proposal="edge_list")) for sd in seeds]
+        cpm.Replica.run_mcs_many(reps, 20)
+        first = [r.sample() for r in reps]
+        cpm.Replica.run_mcs_many(reps, 40, mode=mode)
+        out[mode] = []
+        for r, f in zip(reps, first):
+            last = r.sample()
+            last.samples.insert(0, f.samples[0])
+            out[mode].append(trajectory_stats(last, 50.0, box=(32, 32, 32)))
+    _ks(out["exact_replay"], out["sublattice"], "32^3 fixed EdgeList, 20 exact + 40 MCS")

@@ -220,7 +220,7 @@ def c2_sim(seed):
     return sim
 
 
-def organoid3d_sim(seed, side=32):
+def organoid3d_sim(seed, side=32, phi=0.5, proposal=None):
     """3D sorting in a fixed box: 3 types, V* = 50, I* = 360 (about the
     eighteen-neighbour interface of a compact 50-site cell: the organoid
     configuration's I*/V*^(2/3)), lambda_V = 5, lambda_I = 1, phi = 0.5, J
@@ -232,8 +232,9 @@ def organoid3d_sim(seed, side=32):
     for a in range(1, T + 1):
         for b in range(a, T + 1):
             J[a, b] = J[b, a] = float(rng.uniform(2, 16))
-    n = round(0.5 * side ** 3 / 50)
-    sim = cpm.CPM(grid=(side, side, side), boundary="fixed", seed=seed)
+    n = round(phi * side ** 3 / 50)
+    kw = {} if proposal is None else {"proposal": proposal}
+    sim = cpm.CPM(grid=(side, side, side), boundary="fixed", seed=seed, **kw)
     for t in range(T):
         sim.register_cell_type(f"t{t}", target_volume=50, target_interface=360,
                                lambda_volume=5.0, lambda_interface=1.0)
