@@ -524,10 +524,27 @@ def main():
     from concurrent.futures import ThreadPoolExecutor
 
     sims = [make_sim(cpm, p, p["seed"]) for p in params]
-    with ThreadPoolExecutor(min(os.cpu_count() or 1, HOST_THREADS)) as ex:  # host C++ init
-        reps = list(ex.map(lambda s: cpm.Replica(s, 0, 10000, INTERVAL), sims))
+    threads = min(os.cpu_count() or 1, HOST_THREADS)
+    # CPM::new of every replica on the host threads, then scatter_and_grow
+    # of all of them on the device (one wave per replica)
+    with ThreadPoolExecutor(threads) as ex:
+        reps = list(ex.map(lambda s: cpm.Replica(s, 0, 10000, INTERVAL, initialize=False), sims))
+    t_create = time.perf_counter() - t_init0
+    t_d0 = time.perf_counter()
+    cpm.Replica.initialize_many(reps)
+    t_dev_init = time.perf_counter() - t_d0
     t_init = time.perf_counter() - t_init0
-    print(f"[bench] {len(reps)} replicas initialised in {t_init:.1f} s", flush=True)
+    # the same initialisation on the host threads, for comparison
+    t_h0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        host_reps = list(ex.map(lambda s: cpm.Replica(s, 0, 10000, INTERVAL), sims))
+    t_host_init = time.perf_counter() - t_h0
+    del host_reps
+    init_info = {"replicas": len(reps), "create_s": t_create, "device_scatter_and_grow_s": t_dev_init,
+                 "host_threads_create_and_initialize_s": t_host_init, "host_threads": threads}
+    print(f"[bench] {len(reps)} replicas initialised in {t_init:.2f} s (create {t_create:.2f} s, "
+          f"scatter_and_grow on the device {t_dev_init:.3f} s; on {threads} host threads "
+          f"create + initialize {t_host_init:.2f} s)", flush=True)
 
     for w in range(args.warmup):
         c = cpm.Replica.run_mcs_many(reps, args.interval)
@@ -614,6 +631,7 @@ def main():
         "sims_per_hour": None if full is None else full["sims_per_hour"],
         "sims_per_hour_projected_from_steps": sims_per_hour,
         "run_batch": full,
+        "initialization": init_info,
         "counters": {k: totals[k] for k in totals},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),

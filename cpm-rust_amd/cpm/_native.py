@@ -166,6 +166,9 @@ def lib():
         L.cpm_model_destroy.argtypes = [vp]
         L.cpm_model_initialize.argtypes = [vp, C.POINTER(C.c_int32)]
         L.cpm_model_initialize.restype = st
+        L.cpm_model_initialize_batch.argtypes = [vp, C.POINTER(vp), C.c_size_t,
+                                                 C.POINTER(C.c_int32)]
+        L.cpm_model_initialize_batch.restype = st
         L.cpm_run_mcs.argtypes = [vp, C.POINTER(vp), C.c_size_t, C.c_uint64, C.c_int32,
                                   C.POINTER(CpmCounters)]
         L.cpm_run_mcs.restype = st
@@ -233,6 +236,8 @@ def lib():
         L.cpm_binomial_check.restype = st
         L.cpm_engine_set_pipelining.argtypes = [vp, C.c_int32]
         L.cpm_engine_set_pipelining.restype = st
+        L.cpm_engine_set_device_init.argtypes = [vp, C.c_int32]
+        L.cpm_engine_set_device_init.restype = st
         L.cpm_engine_kernel_launches.argtypes = [vp, C.POINTER(C.c_int32), u64p, C.c_size_t,
                                                  C.POINTER(C.c_size_t)]
         L.cpm_engine_kernel_launches.restype = st
@@ -355,6 +360,12 @@ def binomial_check(states, n, p, device: bool = True):
 def set_pipelining(on: bool) -> None:
     """Allows (default) or forbids the two-wave pipelined exact-replay kernel."""
     check(lib().cpm_engine_set_pipelining(engine(), 1 if on else 0))
+
+
+def set_device_init(on: bool) -> None:
+    """Batches run scatter_and_grow on the device (default) or on the host
+    threads; the initial states are the same."""
+    check(lib().cpm_engine_set_device_init(engine(), 1 if on else 0))
 
 
 def kernel_launches() -> dict:

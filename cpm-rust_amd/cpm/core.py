@@ -52,6 +52,17 @@ class Replica:
             N.lib().cpm_model_destroy(self.h)
             self.h = None
 
+    @staticmethod
+    def initialize_many(replicas) -> None:
+        """initialization::initialize of replicas created with
+        initialize=False, in one call (as run_batch's workers do):
+        scatter_and_grow runs on the device, one wave per replica."""
+        arr = (C.c_void_p * max(len(replicas), 1))(*[r.h for r in replicas])
+        used = (C.c_int32 * max(len(replicas), 1))()
+        N.check(N.lib().cpm_model_initialize_batch(N.engine(), arr, len(replicas), used))
+        for r, u in zip(replicas, used):
+            r.used_default = bool(u)
+
     # ------------------------------------------------------------ dynamics
     def run_mcs(self, n: int, mode: str = "exact_replay") -> dict:
         return Replica.run_mcs_many([self], n, mode)

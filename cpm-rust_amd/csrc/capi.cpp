@@ -24,6 +24,8 @@ using namespace cpm;
 
 struct cpm_engine {
     Engine* e;
+    // batches place and grow their default-initialised models on the device
+    bool device_init = true;
 };
 struct cpm_model {
     Model* m;
@@ -152,6 +154,22 @@ cpm_status cpm_model_initialize(cpm_model* model, int32_t* used_default) {
         device_release(model->m);
         model->m->initialize();
         if (used_default) *used_default = model->m->used_default ? 1 : 0;
+    });
+}
+
+cpm_status cpm_model_initialize_batch(cpm_engine* engine, cpm_model* const* models, size_t n,
+                                      int32_t* used_default) {
+    return guarded([&] {
+        need(engine && (models || n == 0), "engine/models");
+        std::vector<Model*> ms(n);
+        for (size_t i = 0; i < n; ++i) {
+            need(models[i], "models[i]");
+            ms[i] = models[i]->m;
+            device_release(ms[i]);
+        }
+        device_initialize(engine->e, ms.data(), n);
+        if (used_default)
+            for (size_t i = 0; i < n; ++i) used_default[i] = ms[i]->used_default ? 1 : 0;
     });
 }
 
@@ -361,17 +379,18 @@ static void batch_models(cpm_engine* engine, std::vector<Config>& cfgs, int32_t 
                          cpm_counters* counters) {
     const size_t n = cfgs.size();
     std::vector<std::unique_ptr<Model>> owned(n);
-    // CPM::new + initialize per replica, over the host cores (batch.rs:126-145)
+    std::vector<Model*> ms(n);
+    // CPM::new + initialize per replica (batch.rs:126-145): construction
+    // over the host cores, scatter_and_grow on the device (one wave per
+    // model) unless it is switched off
     parallel_for(n, [&](size_t i) {
         owned[i] = std::make_unique<Model>(cfgs[i]);
-        owned[i]->initialize();
-    });
-    std::vector<Model*> ms(n);
-    bool any = false;
-    for (size_t i = 0; i < n; ++i) {
         ms[i] = owned[i].get();
-        any |= ms[i]->used_default;
-    }
+        if (!engine->device_init) owned[i]->initialize();
+    });
+    if (engine->device_init) device_initialize(engine->e, ms.data(), n);
+    bool any = false;
+    for (size_t i = 0; i < n; ++i) any |= ms[i]->used_default;
     if (counters) *counters = cpm_counters{};
     std::vector<Result> rs;
     run_lockstep(engine->e, ms, include_lattice != 0, mode, rs, counters);
@@ -659,6 +678,13 @@ cpm_status cpm_engine_set_pipelining(cpm_engine* engine, int32_t on) {
     return guarded([&] {
         need(engine, "engine");
         engine_set_pipelining(engine->e, on != 0);
+    });
+}
+
+cpm_status cpm_engine_set_device_init(cpm_engine* engine, int32_t on) {
+    return guarded([&] {
+        need(engine, "engine");
+        engine->device_init = on != 0;
     });
 }
 

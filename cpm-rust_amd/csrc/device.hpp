@@ -71,6 +71,12 @@ void engine_set_pipelining(Engine* e, bool on);
 // (instantiation key, launches) of the exact-replay wave kernels so far.
 std::vector<std::pair<int, uint64_t>> engine_launch_counts(Engine* e);
 
+// initialization::initialize of n models (initialization.rs:39-55): the
+// default scatter_and_grow runs on the device, one wave per model, for
+// models of at most 4096 cells; the others initialise on the host.
+// Throws the lowest failing index's error.
+void device_initialize(Engine* e, Model* const* models, size_t n);
+
 // Device bytes a model occupies once uploaded.
 size_t device_footprint(Engine* e, const Model& m);
 // Whether sublattice mode can run the model (LDS-resident lattice, even

@@ -2241,6 +2241,203 @@ __global__ void __launch_bounds__(256) k_sp3d(SpMasks M, uint64_t* __restrict__ 
 }
 
 // ===========================================================================
+// scatter_and_grow on the device (initialization.rs:202-330): one wave per
+// model, every draw in the reference's order.  The wave places the seeds
+// (lanes test the separation from 64 earlier seeds at a time), then grows
+// the cells round-robin in the shuffled order: each claim prunes the cell's
+// frontier of claimed sites keeping its order (lane-parallel compaction),
+// draws the pick, swap-removes it and appends the new site's unclaimed
+// neighbours in copy-stencil order.  Draws and the uniform steps run on
+// every lane alike, so each lane's reads of the lattice follow its own
+// writes; the frontier moves between lanes and is fenced.
+// ===========================================================================
+struct InitJob {
+    uint32_t* sigma;           // padded lattice, zero on entry
+    uint32_t* frontier;        // cell i's frontier at fr_start[i]
+    uint32_t* seeds;           // 3 coordinates per cell
+    const uint32_t* fr_start;  // frontier offset of each cell
+    const uint32_t* target;    // target volume of each cell
+    uint32_t* volume;          // out: volume of each cell
+    uint64_t rng[4];           // in: the model's state; out: after the last draw
+    uint64_t sep_sq;           // seeds closer than sqrt(sep_sq) are refused
+    uint32_t dims[3], strides[3];
+    int32_t copy_vec[26][3];   // copy stencil, as per-axis offsets
+    uint32_t n_cells, n_copy, D, periodic;
+    uint32_t status;           // out: 0 grown; 1 + i: seed i found no room
+    uint32_t pad_;
+};
+// cells per model the device places (order, volume and frontier length in LDS)
+constexpr uint32_t kInitMaxCells = 4096;
+
+__device__ __forceinline__ void init_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__global__ void __launch_bounds__(64) k_scatter_grow(InitJob* __restrict__ jobs) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    InitJob& J = jobs[blockIdx.x];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t n = J.n_cells, D = J.D, n_copy = J.n_copy;
+    const bool periodic = J.periodic != 0;
+    uint32_t dims[3], strides[3];
+    for (int d = 0; d < 3; ++d) {
+        dims[d] = J.dims[d];
+        strides[d] = J.strides[d];
+    }
+    uint32_t* const order = (uint32_t*)smem;
+    uint32_t* const vol = order + n;
+    uint32_t* const flen = vol + n;
+    uint32_t* const seeds = J.seeds;
+    uint32_t* const sigma = J.sigma;
+    uint32_t* const fr = J.frontier;
+    Xoshiro r{{J.rng[0], J.rng[1], J.rng[2], J.rng[3]}};
+    auto flat_of = [&](const uint32_t* c) {
+        uint32_t f = 0;
+        for (uint32_t d = 0; d < D; ++d) f += (c[d] + 1) * strides[d];
+        return f;
+    };
+    // canonical_growable_neighbour (initialization.rs:183-196): the
+    // neighbour of interior site c along copy offset k, wrapped on a
+    // periodic box; none past a fixed boundary
+    auto growable = [&](const uint32_t* c, uint32_t k, uint32_t* out) {
+        uint32_t q[3];
+        for (uint32_t d = 0; d < D; ++d) {
+            const int64_t x = (int64_t)c[d] + J.copy_vec[k][d];
+            if (x < 0 || x >= (int64_t)dims[d]) {
+                if (!periodic) return false;
+                q[d] = x < 0 ? dims[d] - 1 : 0u;
+            } else {
+                q[d] = (uint32_t)x;
+            }
+        }
+        *out = flat_of(q);
+        return true;
+    };
+    auto coords_of = [&](uint32_t f, uint32_t* c) {
+        for (uint32_t d = 0; d < D; ++d) {
+            c[d] = f / strides[d] - 1;
+            f %= strides[d];
+        }
+    };
+
+    // ---- seeds (initialization.rs:226-249)
+    for (uint32_t i = 0; i < n; ++i) {
+        bool placed = false;
+        for (uint32_t a = 0; a < 20000 && !placed; ++a) {
+            uint32_t cand[3] = {0, 0, 0};
+            for (uint32_t d = 0; d < D; ++d) cand[d] = (uint32_t)gen_range(r, dims[d]);
+            bool far = true;
+            for (uint32_t j0 = 0; j0 < i && far; j0 += 64) {
+                const uint32_t j = j0 + lane;
+                bool near = false;
+                if (j < i) {
+                    uint64_t sq = 0;
+                    for (uint32_t d = 0; d < D; ++d) {
+                        const uint32_t sd = seeds[3 * j + d];
+                        const uint32_t raw = sd > cand[d] ? sd - cand[d] : cand[d] - sd;
+                        const uint32_t del = periodic ? min(raw, dims[d] - raw) : raw;
+                        sq += (uint64_t)del * del;
+                    }
+                    near = sq < J.sep_sq;
+                }
+                far = __ballot(near) == 0;
+            }
+            if (far) {
+                for (uint32_t d = 0; d < 3; ++d) seeds[3 * i + d] = cand[d];
+                placed = true;
+            }
+        }
+        if (!placed) {
+            if (lane == 0) J.status = 1 + i;
+            return;
+        }
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t c[3];
+        for (uint32_t d = 0; d < 3; ++d) c[d] = seeds[3 * i + d];
+        sigma[flat_of(c)] = i + 1;
+    }
+    for (uint32_t i = lane; i < n; i += 64) {
+        order[i] = i;
+        vol[i] = 1;
+    }
+    init_fence();
+    // frontier_of each seed once every seed is placed (initialization.rs:332-343)
+    for (uint32_t i = lane; i < n; i += 64) {
+        uint32_t c[3], len = 0;
+        for (uint32_t d = 0; d < 3; ++d) c[d] = seeds[3 * i + d];
+        const uint32_t base = J.fr_start[i];
+        for (uint32_t k = 0; k < n_copy; ++k) {
+            uint32_t nb;
+            if (growable(c, k, &nb) && sigma[nb] == 0) fr[base + len++] = nb;
+        }
+        flen[i] = len;
+    }
+    init_fence();
+
+    // ---- growth rounds (initialization.rs:262-294)
+    for (;;) {
+        bool grew = false;
+        for (uint32_t idx = 0; idx < n; ++idx) {
+            const uint32_t i = order[idx];
+            if (vol[i] >= J.target[i]) continue;
+            const uint32_t base = J.fr_start[i];
+            // retain the unclaimed sites, in order
+            const uint32_t len0 = flen[i];
+            uint32_t len = 0;
+            for (uint32_t c0 = 0; c0 < len0; c0 += 64) {
+                const bool in = c0 + lane < len0;
+                const uint32_t v = in ? fr[base + c0 + lane] : 0u;
+                const bool keep = in && sigma[v] == 0;
+                const uint64_t m = __ballot(keep);
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+                    (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                if (keep) fr[base + len + rank] = v;
+                len += (uint32_t)__popcll(m);
+            }
+            if (len == 0) {
+                flen[i] = 0;
+                continue;
+            }
+            init_fence();
+            const uint32_t pick = (uint32_t)gen_range(r, len);
+            const uint32_t claimed = fr[base + pick];
+            const uint32_t last = fr[base + len - 1];
+            init_fence();
+            if (lane == 0) fr[base + pick] = last;
+            --len;
+            sigma[claimed] = i + 1;
+            vol[i] += 1;
+            uint32_t c[3];
+            coords_of(claimed, c);
+            uint32_t nb = 0;
+            const bool ok = lane < n_copy && growable(c, lane, &nb) && sigma[nb] == 0;
+            const uint64_t m = __ballot(ok);
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+                (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            if (ok) fr[base + len + rank] = nb;
+            flen[i] = len + (uint32_t)__popcll(m);
+            init_fence();
+            grew = true;
+        }
+        if (!grew) break;
+        // the per-round shuffle (initialization.rs:289-293)
+        for (uint32_t i = n - 1; i >= 1; --i) {
+            const uint32_t j = (uint32_t)gen_range_incl(r, i);
+            const uint32_t a = order[i], b = order[j];
+            order[i] = b;
+            order[j] = a;
+        }
+    }
+    for (uint32_t i = lane; i < n; i += 64) J.volume[i] = vol[i];
+    if (lane == 0) {
+        for (int k = 0; k < 4; ++k) J.rng[k] = r.s[k];
+        J.status = 0;
+    }
+}
+
+// ===========================================================================
 // Host side
 // ===========================================================================
 struct Device {
@@ -3626,6 +3823,150 @@ void run_mcs_plain(Engine* e, Model* m, uint64_t n_mcs, cpm_counters* counters) 
         uint64_t one = 1;
         run_mcs_each(e, &m, 1, &one, CPM_MODE_EXACT_REPLAY, counters);
     }
+}
+
+}  // namespace cpm
+
+namespace cpm {
+
+// initialization::initialize of many models: scatter_and_grow on the device
+// for the default-initialised ones (at most kInitMaxCells cells), placed
+// and grown in one launch (one wave per model); explicit placements and
+// larger models initialise on the host threads.  The host then finishes
+// each model as initialization.rs:296-305 does (halo, derived state,
+// energy).  On failure the error of the lowest failing index is thrown,
+// with the reference's message.
+void device_initialize(Engine* e, Model* const* ms, size_t n) {
+    std::vector<std::exception_ptr> errs(n);
+    std::vector<ScatterPlan> plans(n);
+    std::vector<size_t> on_dev, on_host;
+    std::vector<uint64_t> cap_of(n, 0);
+    for (size_t i = 0; i < n; ++i) {
+        Model& m = *ms[i];
+        bool dev = !m.cfg.init_explicit && m.n_padded < (1ull << 31);
+        if (dev) {
+            try {
+                plans[i] = m.scatter_plan();
+            } catch (...) {
+                errs[i] = std::current_exception();
+                continue;
+            }
+            dev = plans[i].type_of.size() <= kInitMaxCells;
+            // a cell's frontier holds at most n_copy sites per site it claims
+            uint64_t cap = 0;
+            for (uint32_t t : plans[i].type_of)
+                cap += (uint64_t)m.n_copy * std::max<uint64_t>(m.cfg.types[t].target_volume, 1);
+            cap_of[i] = cap;
+            dev = dev && cap < (1ull << 28);
+        }
+        (dev ? on_dev : on_host).push_back(i);
+    }
+    parallel_for(on_host.size(), [&](size_t k) {
+        try {
+            ms[on_host[k]]->initialize();
+        } catch (...) {
+            errs[on_host[k]] = std::current_exception();
+        }
+    });
+    if (!on_dev.empty()) {
+        // one allocation: jobs, then the lattices (cleared), then per model
+        // seeds, frontier offsets, targets, volumes and frontiers
+        const size_t nj = on_dev.size();
+        std::vector<size_t> sig_off(nj), aux_off(nj), fr_off(nj);
+        size_t off = align256(nj * sizeof(InitJob));
+        const size_t sig_begin = off;
+        for (size_t k = 0; k < nj; ++k) {
+            sig_off[k] = off;
+            off += align256(ms[on_dev[k]]->n_padded * 4);
+        }
+        const size_t sig_end = off;
+        for (size_t k = 0; k < nj; ++k) {
+            aux_off[k] = off;
+            off += align256(plans[on_dev[k]].type_of.size() * 4 * 6);
+        }
+        const size_t aux_end = off;  // copied back up to here; frontiers after
+        for (size_t k = 0; k < nj; ++k) {
+            fr_off[k] = off;
+            off += align256(cap_of[on_dev[k]] * 4);
+        }
+        std::lock_guard<std::mutex> lock(e->mu);
+        HIP_OK(hipSetDevice(e->device));
+        char* buf = nullptr;
+        HIP_OK(hipMalloc(&buf, off));
+        std::vector<InitJob> jobs(nj);
+        std::vector<char> host_aux(aux_end - sig_end);  // fr_start and target, staged
+        uint32_t max_cells = 1;
+        for (size_t k = 0; k < nj; ++k) {
+            const Model& m = *ms[on_dev[k]];
+            const ScatterPlan& p = plans[on_dev[k]];
+            const uint32_t cells = (uint32_t)p.type_of.size();
+            max_cells = std::max(max_cells, cells);
+            InitJob& J = jobs[k];
+            std::memset(&J, 0, sizeof J);
+            char* a = buf + aux_off[k];
+            J.sigma = (uint32_t*)(buf + sig_off[k]);
+            J.seeds = (uint32_t*)a;
+            J.fr_start = (const uint32_t*)(a + cells * 12);
+            J.target = (const uint32_t*)(a + cells * 16);
+            J.volume = (uint32_t*)(a + cells * 20);
+            J.frontier = (uint32_t*)(buf + fr_off[k]);
+            for (int q = 0; q < 4; ++q) J.rng[q] = m.rng.s[q];
+            J.sep_sq = p.sep_sq;
+            for (int d = 0; d < 3; ++d) {
+                J.dims[d] = d < m.D ? (uint32_t)m.dims[d] : 1u;
+                J.strides[d] = d < m.D ? (uint32_t)m.strides[d] : 1u;
+            }
+            int32_t o[26 * 3];
+            const int nc = stencil(m.D, m.cfg.copy_nb, o);
+            for (int q = 0; q < nc; ++q)
+                for (int d = 0; d < m.D; ++d) J.copy_vec[q][d] = o[q * m.D + d];
+            J.n_cells = cells;
+            J.n_copy = (uint32_t)nc;
+            J.D = (uint32_t)m.D;
+            J.periodic = m.cfg.boundary == CPM_BOUNDARY_PERIODIC ? 1u : 0u;
+            char* h = host_aux.data() + (aux_off[k] - sig_end);
+            uint32_t* fs = (uint32_t*)(h + cells * 12);
+            uint32_t* tg = (uint32_t*)(h + cells * 16);
+            uint64_t at = 0;
+            for (uint32_t c = 0; c < cells; ++c) {
+                const uint32_t t = m.cfg.types[p.type_of[c]].target_volume;
+                fs[c] = (uint32_t)at;
+                tg[c] = t;
+                at += (uint64_t)nc * std::max<uint32_t>(t, 1u);
+            }
+        }
+        HIP_OK(hipMemsetAsync(buf + sig_begin, 0, sig_end - sig_begin, e->stream));
+        HIP_OK(hipMemcpyAsync(buf, jobs.data(), nj * sizeof(InitJob), hipMemcpyHostToDevice,
+                              e->stream));
+        if (!host_aux.empty())
+            HIP_OK(hipMemcpyAsync(buf + sig_end, host_aux.data(), host_aux.size(),
+                                  hipMemcpyHostToDevice, e->stream));
+        hipLaunchKernelGGL(k_scatter_grow, dim3((uint32_t)nj), dim3(64), max_cells * 12, e->stream,
+                           (InitJob*)buf);
+        HIP_OK(hipGetLastError());
+        std::vector<char> back(aux_end);
+        HIP_OK(hipMemcpyAsync(back.data(), buf, aux_end, hipMemcpyDeviceToHost, e->stream));
+        HIP_OK(hipStreamSynchronize(e->stream));
+        HIP_OK(hipFree(buf));
+        const InitJob* out = (const InitJob*)back.data();
+        parallel_for(nj, [&](size_t k) {
+            Model& m = *ms[on_dev[k]];
+            const ScatterPlan& p = plans[on_dev[k]];
+            try {
+                if (out[k].status) m.crowded_error(p);
+                const size_t cells = p.type_of.size();
+                std::memcpy(m.sigma.data(), back.data() + sig_off[k], m.n_padded * 4);
+                const uint32_t* v = (const uint32_t*)(back.data() + aux_off[k] + cells * 20);
+                m.volume.assign(v, v + cells);
+                for (int q = 0; q < 4; ++q) m.rng.s[q] = out[k].rng[q];
+                m.finish_scatter(p);
+            } catch (...) {
+                errs[on_dev[k]] = std::current_exception();
+            }
+        });
+    }
+    for (auto& err : errs)
+        if (err) std::rethrow_exception(err);
 }
 
 }  // namespace cpm

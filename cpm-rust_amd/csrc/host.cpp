@@ -377,7 +377,7 @@ double Model::global_energy() const {
 // initialization.rs
 // ----------------------------------------------------------------------
 
-static void init_error(const std::string& m) { throw Error(CPM_ERR_INIT, m); }
+[[noreturn]] static void init_error(const std::string& m) { throw Error(CPM_ERR_INIT, m); }
 
 void Model::initialize() {
     if (cfg.init_explicit) {
@@ -447,15 +447,10 @@ void Model::initialize() {
         used_default = false;
     } else {
         // scatter_and_grow (initialization.rs:202-330)
-        std::vector<uint32_t> type_of;
-        for (size_t t = 0; t < cfg.counts.size(); ++t)
-            type_of.insert(type_of.end(), cfg.counts[t], (uint32_t)t);
+        const ScatterPlan plan = scatter_plan();
+        const std::vector<uint32_t>& type_of = plan.type_of;
         const size_t n = type_of.size();
-        if (!n) init_error("cell_counts sums to zero cells \xE2\x80\x94 nothing to place");
-        double avg = 0.0;
-        for (auto& t : cfg.types) avg += (double)t.target_volume;
-        avg /= (double)cfg.types.size();
-        const double min_sep = std::sqrt(avg / 3.141592653589793);
+        const double min_sep = plan.min_sep;
         const bool periodic = cfg.boundary == CPM_BOUNDARY_PERIODIC;
         std::vector<uint64_t> seeds;
         seeds.reserve(3 * n);
@@ -479,14 +474,7 @@ void Model::initialize() {
                     placed = true;
                 }
             }
-            if (!placed) {
-                char sep[64];
-                snprintf(sep, sizeof sep, "%.2f", min_sep);
-                init_error("could not place all " + std::to_string(n) +
-                           " seed points with minimum separation " + sep + " on a grid of " +
-                           grid_debug(cfg) +
-                           " \xE2\x80\x94 too crowded for this configuration");
-            }
+            if (!placed) crowded_error(plan);
         }
         n_cells = n;
         std::fill(sigma.begin(), sigma.end(), 0u);
@@ -543,9 +531,53 @@ void Model::initialize() {
                 std::swap(order[i], order[j]);
             }
         }
-        refresh_halo();
-        used_default = true;
+        finish_scatter(plan);
+        return;
     }
+    recompute(interface, sum_pos, sum_outer, contact);
+    energy = global_energy();
+    build_edge_list();  // initialization.rs:427-435
+    initialized = true;
+}
+
+// What scatter_and_grow derives from the configuration before its first
+// draw (initialization.rs:203-224): the type of each cell in id order and
+// the minimum seed separation.  Seed coordinates are integers, so every
+// separation the placement tests is the square root of an exact integer
+// sum of squares (periodic_distance, initialization.rs:369-386); since
+// sqrt is monotone, `sqrt(s) >= min_sep` holds exactly for the integers
+// s >= sep_sq, the smallest such integer.
+ScatterPlan Model::scatter_plan() const {
+    ScatterPlan p;
+    for (size_t t = 0; t < cfg.counts.size(); ++t)
+        p.type_of.insert(p.type_of.end(), cfg.counts[t], (uint32_t)t);
+    if (p.type_of.empty()) init_error("cell_counts sums to zero cells \xE2\x80\x94 nothing to place");
+    double avg = 0.0;
+    for (auto& t : cfg.types) avg += (double)t.target_volume;
+    avg /= (double)cfg.types.size();
+    p.min_sep = std::sqrt(avg / 3.141592653589793);
+    const double sq = p.min_sep * p.min_sep;
+    uint64_t s = sq > 2.0 ? (uint64_t)sq - 2 : 0;
+    while (std::sqrt((double)s) < p.min_sep) ++s;
+    p.sep_sq = s;
+    return p;
+}
+
+void Model::crowded_error(const ScatterPlan& plan) const {
+    char sep[64];
+    snprintf(sep, sizeof sep, "%.2f", plan.min_sep);
+    init_error("could not place all " + std::to_string(plan.type_of.size()) +
+               " seed points with minimum separation " + sep + " on a grid of " +
+               grid_debug(cfg) + " \xE2\x80\x94 too crowded for this configuration");
+}
+
+// The end of scatter_and_grow once sigma and the volumes hold the grown
+// cells (initialization.rs:296-305, 396-436): halo, derived state, energy.
+void Model::finish_scatter(const ScatterPlan& plan) {
+    n_cells = plan.type_of.size();
+    type_index = plan.type_of;
+    refresh_halo();
+    used_default = true;
     recompute(interface, sum_pos, sum_outer, contact);
     energy = global_energy();
     build_edge_list();  // initialization.rs:427-435

@@ -54,6 +54,15 @@ std::string grid_debug(const Config& c);
 
 struct Device;  // device residency of one model (engine.hip)
 
+// scatter_and_grow's inputs derived from the configuration: the type of
+// each cell in id order, the minimum seed separation and the smallest
+// integer sum of squares whose square root reaches it.
+struct ScatterPlan {
+    std::vector<uint32_t> type_of;
+    double min_sep = 0.0;
+    uint64_t sep_sq = 0;
+};
+
 // CPM<D> + State<D> (crates/cpm-core/src/model.rs:15-38, state.rs:168-203)
 struct Model {
     Config cfg;
@@ -111,6 +120,13 @@ struct Model {
     // order whose copy neighbourhood holds another owner
     void build_edge_list();
     void initialize();  // initialization::initialize, throws Error(INIT)
+    // scatter_and_grow in parts, for a device that places and grows the
+    // cells: the plan (throws Error(INIT) for zero cells), the error for a
+    // seed that finds no room, and the steps after growth (sigma and
+    // volume hold the grown cells, rng the state after the last draw)
+    ScatterPlan scatter_plan() const;
+    [[noreturn]] void crowded_error(const ScatterPlan& plan) const;
+    void finish_scatter(const ScatterPlan& plan);
     // Replaces the state by a caller's (cpm_model_set_state); throws
     // Error(ARG) and leaves the model unchanged on invalid input.
     void set_state(const cpm_model_state& st);

@@ -135,6 +135,16 @@ void cpm_model_destroy(cpm_model* model);
    *used_default = 1 iff scatter-and-grow ran. Host-side, as in the
    reference. */
 cpm_status cpm_model_initialize(cpm_model* model, int32_t* used_default);
+/* initialization::initialize for n models at once, as run_batch's workers
+   do (crates/cpm-core/src/batch.rs:126-145, initialization.rs:39-55): the
+   default scatter_and_grow (initialization.rs:202-330) runs on the engine's
+   device, one wave per model of at most 4096 cells, every draw in the
+   reference's order; explicit placements and larger models initialise on
+   the host threads.  Each model ends in the state cpm_model_initialize
+   gives it.  On failure returns the error (and message) of the lowest
+   failing index; used_default (n entries) may be NULL. */
+cpm_status cpm_model_initialize_batch(cpm_engine* engine, cpm_model* const* models, size_t n,
+                                      int32_t* used_default);
 
 /* run_mcs (crates/cpm-core/src/monte_carlo.rs:327-357) for n models at
    once: every model advances by n_mcs MCS on the device.  In
@@ -439,6 +449,14 @@ cpm_status cpm_engine_set_sigma_placement(cpm_engine* engine, int32_t order, dou
    batch leaves SIMDs free; on = 0 runs every replica on one wave.
    Results do not depend on it. */
 cpm_status cpm_engine_set_pipelining(cpm_engine* engine, int32_t on);
+
+/* Where batches initialise (no counterpart in the reference; the result
+   is the same either way): by default cpm_run_batch and
+   cpm_run_batch_configs run scatter_and_grow (initialization.rs:202-330)
+   on the device, one wave per model of at most 4096 cells, drawing from
+   each model's stream in the reference's order; on = 0 initialises every
+   model on the host threads, as cpm_model_initialize does. */
+cpm_status cpm_engine_set_device_init(cpm_engine* engine, int32_t on);
 
 /* How often each exact-replay wave-kernel variant has been launched by
    this engine: writes up to cap (key, launches) pairs and sets *n to the
