@@ -2144,42 +2144,50 @@ __global__ void __launch_bounds__(256) k_contact_clear(Rep* __restrict__ reps,
 
 __global__ void __launch_bounds__(256) k_contact_count(Rep* __restrict__ reps,
                                                        const uint32_t* __restrict__ idx) {
+    __shared__ int32_t s_off[26][3];
     Rep& R = reps[idx[blockIdx.y]];
     const uint32_t D = R.D;
     const uint64_t n1 = (uint64_t)R.n_cells + 1;
     const bool periodic = R.boundary == CPM_BOUNDARY_PERIODIC;
-    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < R.n_sites;
-         q += (uint64_t)gridDim.x * blockDim.x) {
+    // the offsets' components (each -1, 0 or 1: the flat offset rounded per axis)
+    if (threadIdx.x < R.n_energy) {
+        int64_t off = R.energy_off[threadIdx.x];
+        for (int d = 0; d < 3; ++d) {
+            int64_t od = 0;
+            if (d < (int)D) {
+                const int64_t st = R.strides[d];
+                od = off >= 0 ? (off + st / 2) / st : -((-off + st / 2) / st);
+                off -= od * st;
+            }
+            s_off[threadIdx.x][d] = (int32_t)od;
+        }
+    }
+    __syncthreads();
+    const uint32_t dimv[3] = {R.dims[0], R.dims[1], R.dims[2]};
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < R.n_sites;
+         q += gridDim.x * blockDim.x) {
         uint32_t c[3] = {0, 0, 0};
-        uint64_t rem = q, fi = 0;
+        uint32_t rem = q;
+        uint64_t fi = 0;
         for (int d = (int)D - 1; d >= 0; --d) {
-            c[d] = (uint32_t)(rem % R.dims[d]);
-            rem /= R.dims[d];
+            c[d] = rem % dimv[d];
+            rem /= dimv[d];
             fi += (uint64_t)(c[d] + 1) * R.strides[d];
         }
         const uint32_t ci = R.sigma[fi];
         for (uint32_t k = 0; k < R.n_energy; ++k) {
-            // the offset's components (each -1, 0 or 1: the flat offset
-            // rounded per axis), the neighbour's padded coordinates and the
-            // interior site it is or mirrors
-            int32_t o[3] = {0, 0, 0};
-            int64_t off = R.energy_off[k];
-            uint64_t fj = (uint64_t)((int64_t)fi + off), key = 0;
+            // the neighbour's padded coordinates and the interior site it
+            // is or mirrors
+            uint64_t fj = (uint64_t)((int64_t)fi + R.energy_off[k]), key = 0;
             bool interior = true;
             for (int d = 0; d < (int)D; ++d) {
-                const int64_t st = R.strides[d];
-                int64_t od = off >= 0 ? (off + st / 2) / st : -((-off + st / 2) / st);
-                o[d] = (int32_t)od;
-                off -= od * st;
-            }
-            for (int d = 0; d < (int)D; ++d) {
-                const int64_t p = (int64_t)c[d] + 1 + o[d];  // padded coordinate
-                uint64_t w = (uint64_t)p;
-                if (p == 0 || p == (int64_t)R.dims[d] + 1) {
+                const int32_t p = (int32_t)c[d] + 1 + s_off[k][d];  // padded coordinate
+                uint32_t w = (uint32_t)p;
+                if (p == 0 || p == (int32_t)dimv[d] + 1) {
                     interior = false;
-                    w = p == 0 ? R.dims[d] : 1;  // the mirrored interior coordinate
+                    w = p == 0 ? dimv[d] : 1u;  // the mirrored interior coordinate
                 }
-                key += w * R.strides[d];
+                key += (uint64_t)w * R.strides[d];
             }
             if (interior) key = fj;
             else if (!periodic) key = ~0ull;  // Fixed: a distinct, permanently medium site
@@ -2191,6 +2199,148 @@ __global__ void __launch_bounds__(256) k_contact_count(Rep* __restrict__ reps,
             }
         }
     }
+}
+
+// ===========================================================================
+// Sublattice mode's energy after a call.
+// ===========================================================================
+// global_energy (energy.rs:561-565) of each listed replica from its device
+// state, one wave per replica, in the host's summation order: the volume
+// terms cell by cell, the interface terms cell by cell, then the adhesion
+// over interior sites in row-major order and, per site, over the energy
+// stencil in order (each unordered pair once, keyed by the interior site it
+// is or mirrors; a fixed halo is a distinct medium site).  The lanes compute
+// 64 cells' or sites' terms at a time and stage them in LDS in that order;
+// every lane then adds the staged terms one after another, so the FP64
+// rounding is the host's.
+constexpr uint32_t kEnergyStage = 64 * 26;  // terms staged per pass (doubles)
+
+__device__ __forceinline__ uint32_t lane_prefix(uint32_t v, uint32_t lane) {
+    // exclusive prefix sum over the wave
+    uint32_t x = v;
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, d, 64);
+        if (lane >= d) x += y;
+    }
+    return x - v;
+}
+
+__global__ void __launch_bounds__(64) k_energy(Rep* __restrict__ reps,
+                                               const uint32_t* __restrict__ idx) {
+    __shared__ double stage[kEnergyStage];
+    __shared__ int32_t s_off[26][3];  // the energy stencil, per-axis offsets
+    Rep& R = reps[idx[blockIdx.x]];
+    const uint32_t lane = threadIdx.x, T = R.n_types, T1 = T + 1, D = R.D;
+    if (lane < R.n_energy) {
+        // each component is -1, 0 or 1: the flat offset rounded per axis
+        int64_t off = R.energy_off[lane];
+        for (int d = 0; d < 3; ++d) {
+            int64_t od = 0;
+            if (d < (int)D) {
+                const int64_t st = R.strides[d];
+                od = off >= 0 ? (off + st / 2) / st : -((-off + st / 2) / st);
+                off -= od * st;
+            }
+            s_off[lane][d] = (int32_t)od;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t dims0 = R.dims[0], dims1 = R.dims[1], dims2 = R.dims[2];
+    const uint32_t dimv[3] = {dims0, dims1, dims2};
+    const double* lv = R.params;
+    const double* li = R.params + T;
+    const double* J = R.params + 2 * T;
+    const uint32_t* tv = R.targets;
+    const uint32_t* ti = R.targets + T;
+    const bool periodic = R.boundary == CPM_BOUNDARY_PERIODIC;
+    double sums[2] = {0.0, 0.0};
+    // volume, then interface: lambda * (x - target)^2 per cell
+    for (int which = 0; which < 2; ++which) {
+        const uint32_t* x = which ? R.interface : R.volume;
+        for (uint32_t c0 = 0; c0 < R.n_cells; c0 += 64) {
+            const uint32_t c = c0 + lane;
+            if (c < R.n_cells) {
+                const uint32_t t = R.type_index[c];
+                const double d = (double)x[c] - (double)(which ? ti[t] : tv[t]);
+                stage[lane] = (which ? li[t] : lv[t]) * (d * d);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            const uint32_t n = min(64u, R.n_cells - c0);
+            double acc = sums[which];
+            for (uint32_t k = 0; k < n; ++k) acc += stage[k];
+            sums[which] = acc;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    // adhesion: sum over unlike keyed pairs of J[type(ci)][type(cj)]; each
+    // lane counts its site's terms, then writes them at its prefix offset
+    double adh = 0.0;
+    for (uint32_t q0 = 0; q0 < R.n_sites; q0 += 64) {
+        const uint32_t q = q0 + lane;
+        const bool site = q < R.n_sites;
+        uint32_t c[3] = {0, 0, 0};
+        uint64_t fi = 0;
+        if (site) {
+            uint32_t rem = q;
+            for (int d = (int)D - 1; d >= 0; --d) {
+                c[d] = rem % dimv[d];
+                rem /= dimv[d];
+                fi += (uint64_t)(c[d] + 1) * R.strides[d];
+            }
+        }
+        const uint32_t ci = site ? R.sigma[fi] : 0u;
+        const uint32_t ai = ci ? R.type_index[ci - 1] + 1 : 0u;
+        // the pair (site, stencil entry k) counts: the neighbour's adhesion index in *aj
+        auto pair = [&](uint32_t k, uint32_t* aj) {
+            const uint64_t fj = (uint64_t)((int64_t)fi + R.energy_off[k]);
+            uint64_t key = 0;
+            bool interior = true;
+            for (int d = 0; d < (int)D; ++d) {
+                const int32_t p = (int32_t)c[d] + 1 + s_off[k][d];  // padded coordinate
+                uint32_t w = (uint32_t)p;
+                if (p == 0 || p == (int32_t)dimv[d] + 1) {
+                    interior = false;
+                    w = p == 0 ? dimv[d] : 1u;  // the mirrored interior coordinate
+                }
+                key += (uint64_t)w * R.strides[d];
+            }
+            if (interior) key = fj;
+            else if (!periodic) key = ~0ull;
+            if (key != ~0ull && key <= fi) return false;
+            const uint32_t cj = R.sigma[fj];
+            if (cj == ci) return false;
+            *aj = cj ? R.type_index[cj - 1] + 1 : 0u;
+            return true;
+        };
+        uint32_t cnt = 0, aj = 0;
+        if (site)
+            for (uint32_t k = 0; k < R.n_energy; ++k) cnt += pair(k, &aj) ? 1u : 0u;
+        const uint32_t at = lane_prefix(cnt, lane);
+        if (site)
+            for (uint32_t k = 0, r = 0; k < R.n_energy; ++k)
+                if (pair(k, &aj)) stage[at + r++] = J[ai * T1 + aj];
+        const uint32_t total = (uint32_t)__shfl((int)(at + cnt), 63, 64);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        // eight staged terms loaded at a time, added one after another
+        double acc = adh;
+        uint32_t k = 0;
+        for (; k + 8 <= total; k += 8) {
+            double v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = stage[k + j];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) acc += 1.0 * v[j];
+        }
+        for (; k < total; ++k) acc += 1.0 * stage[k];
+        adh = acc;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (lane == 0) R.energy = sums[0] + sums[1] + adh;
 }
 
 // ===========================================================================
@@ -3476,6 +3626,14 @@ static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint
                                e->idx_dev + at);
             HIP_OK(hipGetLastError());
         }
+        // the running energy is recomputed from scratch (a running FP64 sum
+        // of concurrently accepted moves would depend on their order), on
+        // the device in the host's summation order
+        if (!order.empty()) {
+            hipLaunchKernelGGL(k_energy, dim3((uint32_t)order.size()), dim3(64), 0, e->stream,
+                               e->reps_dev, e->idx_dev);
+            HIP_OK(hipGetLastError());
+        }
         HIP_OK(hipEventRecord(e->ev1, e->stream));
         HIP_OK(hipEventRecord(e->work_done, e->stream));
         HIP_OK(hipMemcpyAsync(reps.data(), e->reps_dev, n * sizeof(Rep), hipMemcpyDeviceToHost,
@@ -3487,24 +3645,20 @@ static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint
         for (size_t i = 0; i < n; ++i) {
             if (!todo[i]) continue;
             models[i]->mcs = reps[i].mcs;
+            models[i]->energy = reps[i].energy;
             models[i]->device_dirty = true;
             add_counters(counters, reps[i]);
         }
     }
-    // the running energy is recomputed from scratch: a running FP64 sum of
-    // concurrently accepted moves would depend on their order (one host
-    // thread per replica: with many replicas this sum costs more than the
-    // launch)
     parallel_for(n, [&](size_t i) {
         if (!todo[i]) return;
         Model* m = models[i];
-        sync_lattice_tables(m);
-        m->energy = m->global_energy();
         // EdgeList models: the mode's proposal is the uniform one, which the
         // edge list's is statistically (config.rs:42-70: the same chain
         // without its null attempts); the list is rebuilt from the new
         // lattice as EdgeList::build would (edge_list.rs:121-131)
         if (m->cfg.proposal == CPM_PROPOSAL_EDGE_LIST) {
+            sync_lattice_tables(m);
             m->build_edge_list();
             const Layout L = layout_of(*m, m->dev->lds);
             hipStream_t st = copy_stream_for(e);

@@ -339,3 +339,23 @@ def test_sublattice_ks_3d_after_a_common_exact_burn_in():
             last.samples.insert(0, f.samples[0])
             out[mode].append(trajectory_stats(last, 50.0, box=(32, 32, 32)))
     _ks(out["exact_replay"], out["sublattice"], "32^3 fixed, 3 types, 20 exact + 40 MCS")
+
+
+def test_sublattice_energy_is_the_host_recompute_bit_for_bit():
+    """After a sublattice call the conservative energy is recomputed on the
+    device in the host's summation order: with non-integer J (the 256x256
+    sorting configuration's U[2,14] draws) the order fixes the last bits,
+    and the device value must equal the host's global_energy of the same
+    state exactly, in 2D (sigma in LDS and in HBM) and 3D, alone and
+    batched."""
+    sims = [c2_sim(31), c2_sim(32), organoid3d_sim(33), organoid3d_sim(34, side=24)]
+    reps = [cpm.Replica(s) for s in sims]
+    for mcs in (3, 2):
+        cpm.Replica.run_mcs_many(reps, mcs, mode="sublattice")
+        for r in reps:
+            e_dev = r.energy
+            assert e_dev == r.global_energy(), (e_dev, r.global_energy())
+    alone = cpm.Replica(c2_sim(31))
+    alone.run_mcs(3, mode="sublattice")
+    alone.run_mcs(2, mode="sublattice")
+    assert alone.energy == reps[0].energy
