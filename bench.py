@@ -184,16 +184,19 @@ def copy_calibration(mib: int = 256, reps: int = 4):
 
 
 def serving_trace(cpm, n_requests: int = 1500, rate: float = 80.0, seed: int = 7,
-                  quantum: int = 1 << 17, max_active: int = 4096, round_ms: float = 30.0):
+                  quantum: int = 1 << 17, max_active: int = 4096, round_ms: float = 30.0,
+                  side3d=(32, 40), mcs_range=(40, 160)):
     """The serving workload: a Poisson stream (mean `rate` requests/s) of
     mixed run requests — 70 % 2D periodic lattices of side U{64..256}, 30 %
-    3D periodic lattices of side U{32..40} (the LDS-resident range) — with
-    2-4 cell types, V* = 40, I* = 30 (2D) / 120 (3D), lambda_V = 5,
-    lambda_I = 1, J: medium row 16, type pairs U[2,14], density 0.8 (2D) /
-    0.5 (3D), and U[40,160] MCS each (readout, one sample per 20 MCS).
-    Requests are initialised on 16 host threads as they arrive and queued
-    on one server; reported: completion latency (server queue + device +
-    readouts) p50/p99, sims/hour and copy attempts/s over the trace."""
+    3D periodic lattices of side U{side3d} (by default 32..40, the
+    LDS-resident range; 32..64 is the stated mix, whose larger lattices keep
+    sigma in HBM) — with 2-4 cell types, V* = 40, I* = 30 (2D) / 120 (3D),
+    lambda_V = 5, lambda_I = 1, J: medium row 16, type pairs U[2,14],
+    density 0.8 (2D) / 0.5 (3D), and U[mcs_range] MCS each, rounded down to
+    a multiple of 20 (readout, one sample per 20 MCS).  Requests are
+    initialised on 16 host threads as they arrive and queued on one server;
+    reported: completion latency (server queue + device + readouts)
+    p50/p99, sims/hour and copy attempts/s over the trace."""
     from concurrent.futures import ThreadPoolExecutor
 
     rng = np.random.default_rng(seed)
@@ -204,7 +207,7 @@ def serving_trace(cpm, n_requests: int = 1500, rate: float = 80.0, seed: int = 7
             side = int(rng.integers(64, 257))
             grid, it, phi = (side, side), 30, 0.8
         else:
-            side = int(rng.integers(32, 41))
+            side = int(rng.integers(side3d[0], side3d[1] + 1))
             grid, it, phi = (side, side, side), 120, 0.5
         T = int(rng.integers(2, 5))
         J = np.zeros((T + 1, T + 1))
@@ -212,7 +215,7 @@ def serving_trace(cpm, n_requests: int = 1500, rate: float = 80.0, seed: int = 7
         for a in range(1, T + 1):
             for b in range(a, T + 1):
                 J[a, b] = J[b, a] = float(rng.uniform(2, 14))
-        mcs = int(rng.integers(40, 161)) // 20 * 20
+        mcs = max(20, int(rng.integers(mcs_range[0], mcs_range[1] + 1)) // 20 * 20)
         specs.append(dict(at=t, grid=grid, T=T, J=J.tolist(), it=it, phi=phi, mcs=mcs,
                           seed=int(cpm.derive_seed_u64(seed, i))))
 
@@ -252,8 +255,8 @@ def serving_trace(cpm, n_requests: int = 1500, rate: float = 80.0, seed: int = 7
             "attempts_per_s": st["attempts"] / wall, "rounds": st["rounds"],
             "device_s": st["device_s"], "host_s": st["busy_s"] - st["device_s"],
             "mode": "exact_replay",
-            "mix": "70% 2D side U{64..256}, 30% 3D side U{32..40}, 2-4 types, "
-                   "U[40,160] MCS per request"}
+            "mix": f"70% 2D side U{{64..256}}, 30% 3D side U{{{side3d[0]}..{side3d[1]}}}, "
+                   f"2-4 types, U[{mcs_range[0]},{mcs_range[1]}] MCS per request"}
 
 
 def sorting_256_parts(seed: int = 1):
@@ -535,16 +538,25 @@ def main():
     t_dev_init = time.perf_counter() - t_d0
     t_init = time.perf_counter() - t_init0
     # the same initialisation on the host threads, for comparison
-    t_h0 = time.perf_counter()
     with ThreadPoolExecutor(threads) as ex:
-        host_reps = list(ex.map(lambda s: cpm.Replica(s, 0, 10000, INTERVAL), sims))
-    t_host_init = time.perf_counter() - t_h0
+        host_reps = list(ex.map(lambda s: cpm.Replica(s, 0, 10000, INTERVAL, initialize=False),
+                                sims))
+    from cpm import _native
+    _native.set_device_init(False)
+    try:
+        t_h0 = time.perf_counter()
+        cpm.Replica.initialize_many(host_reps)
+        t_host_init = time.perf_counter() - t_h0
+    finally:
+        _native.set_device_init(True)
     del host_reps
-    init_info = {"replicas": len(reps), "create_s": t_create, "device_scatter_and_grow_s": t_dev_init,
-                 "host_threads_create_and_initialize_s": t_host_init, "host_threads": threads}
+    init_info = {"replicas": len(reps), "create_s": t_create,
+                 "device_scatter_and_grow_s": t_dev_init,
+                 "host_threads_scatter_and_grow_s": t_host_init,
+                 "host_threads": min(os.cpu_count() or 1, 16)}
     print(f"[bench] {len(reps)} replicas initialised in {t_init:.2f} s (create {t_create:.2f} s, "
-          f"scatter_and_grow on the device {t_dev_init:.3f} s; on {threads} host threads "
-          f"create + initialize {t_host_init:.2f} s)", flush=True)
+          f"scatter_and_grow on the device {t_dev_init:.3f} s; the same on "
+          f"{init_info['host_threads']} host threads {t_host_init:.3f} s)", flush=True)
 
     for w in range(args.warmup):
         c = cpm.Replica.run_mcs_many(reps, args.interval)
@@ -601,6 +613,10 @@ def main():
     serving = None
     if not args.no_serving:
         serving = [serving_trace(cpm, rate=r) for r in (80.0, 200.0)]
+        # the stated mix's shapes (3D sides up to 64, sigma in HBM for the
+        # larger ones) with its 5k-20k MCS scaled by 1/100
+        serving.append(serving_trace(cpm, n_requests=400, rate=20.0, side3d=(32, 64),
+                                     mcs_range=(50, 200)))
         for sv in serving:
             print(f"[bench] serving at {sv['offered_rate_per_s']:.0f} requests/s: "
                   f"{sv['requests']} requests in {sv['wall_s']:.1f} s, "

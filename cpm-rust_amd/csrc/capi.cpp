@@ -167,7 +167,10 @@ cpm_status cpm_model_initialize_batch(cpm_engine* engine, cpm_model* const* mode
             ms[i] = models[i]->m;
             device_release(ms[i]);
         }
-        device_initialize(engine->e, ms.data(), n);
+        if (engine->device_init)
+            device_initialize(engine->e, ms.data(), n);
+        else
+            parallel_for(n, [&](size_t i) { ms[i]->initialize(); });
         if (used_default)
             for (size_t i = 0; i < n; ++i) used_default[i] = ms[i]->used_default ? 1 : 0;
     });

@@ -140,9 +140,11 @@ cpm_status cpm_model_initialize(cpm_model* model, int32_t* used_default);
    default scatter_and_grow (initialization.rs:202-330) runs on the engine's
    device, one wave per model of at most 4096 cells, every draw in the
    reference's order; explicit placements and larger models initialise on
-   the host threads.  Each model ends in the state cpm_model_initialize
-   gives it.  On failure returns the error (and message) of the lowest
-   failing index; used_default (n entries) may be NULL. */
+   the host threads (all of them on the host threads when
+   cpm_engine_set_device_init(engine, 0) was called).  Each model ends in
+   the state cpm_model_initialize gives it.  On failure returns the error
+   (and message) of the lowest failing index on the device path, of some
+   failing index on the host path; used_default (n entries) may be NULL. */
 cpm_status cpm_model_initialize_batch(cpm_engine* engine, cpm_model* const* models, size_t n,
                                       int32_t* used_default);
 
@@ -454,8 +456,9 @@ cpm_status cpm_engine_set_pipelining(cpm_engine* engine, int32_t on);
    is the same either way): by default cpm_run_batch and
    cpm_run_batch_configs run scatter_and_grow (initialization.rs:202-330)
    on the device, one wave per model of at most 4096 cells, drawing from
-   each model's stream in the reference's order; on = 0 initialises every
-   model on the host threads, as cpm_model_initialize does. */
+   each model's stream in the reference's order, and so does
+   cpm_model_initialize_batch; on = 0 initialises every model on the host
+   threads, as cpm_model_initialize does. */
 cpm_status cpm_engine_set_device_init(cpm_engine* engine, int32_t on);
 
 /* How often each exact-replay wave-kernel variant has been launched by
