@@ -537,26 +537,37 @@ def main():
     cpm.Replica.initialize_many(reps)
     t_dev_init = time.perf_counter() - t_d0
     t_init = time.perf_counter() - t_init0
-    # the same initialisation on the host threads, for comparison
-    with ThreadPoolExecutor(threads) as ex:
-        host_reps = list(ex.map(lambda s: cpm.Replica(s, 0, 10000, INTERVAL, initialize=False),
-                                sims))
+    # initialisation and the first 1-MCS call (which uploads the state of
+    # host-initialised replicas; device-initialised ones are resident) on
+    # the device and on the host threads, on copies of the same replicas
     from cpm import _native
-    _native.set_device_init(False)
-    try:
-        t_h0 = time.perf_counter()
-        cpm.Replica.initialize_many(host_reps)
-        t_host_init = time.perf_counter() - t_h0
-    finally:
-        _native.set_device_init(True)
-    del host_reps
+
+    def init_and_first_call(on_device):
+        with ThreadPoolExecutor(threads) as ex:
+            rs = list(ex.map(lambda s: cpm.Replica(s, 0, 10000, INTERVAL, initialize=False),
+                             sims))
+        _native.set_device_init(on_device)
+        try:
+            ta = time.perf_counter()
+            cpm.Replica.initialize_many(rs)
+            tb = time.perf_counter()
+            cpm.Replica.run_mcs_many(rs, 1)
+            tc = time.perf_counter()
+        finally:
+            _native.set_device_init(True)
+        return tb - ta, tc - tb
+
+    dev_i, dev_c = init_and_first_call(True)
+    host_i, host_c = init_and_first_call(False)
     init_info = {"replicas": len(reps), "create_s": t_create,
                  "device_scatter_and_grow_s": t_dev_init,
-                 "host_threads_scatter_and_grow_s": t_host_init,
+                 "device_init_s": dev_i, "device_first_call_s": dev_c,
+                 "host_threads_init_s": host_i, "host_threads_first_call_s": host_c,
                  "host_threads": min(os.cpu_count() or 1, 16)}
     print(f"[bench] {len(reps)} replicas initialised in {t_init:.2f} s (create {t_create:.2f} s, "
-          f"scatter_and_grow on the device {t_dev_init:.3f} s; the same on "
-          f"{init_info['host_threads']} host threads {t_host_init:.3f} s)", flush=True)
+          f"scatter_and_grow on the device {t_dev_init:.3f} s); init + first 1-MCS call: "
+          f"device {dev_i:.3f} + {dev_c:.3f} s, {init_info['host_threads']} host threads "
+          f"{host_i:.3f} + {host_c:.3f} s", flush=True)
 
     for w in range(args.warmup):
         c = cpm.Replica.run_mcs_many(reps, args.interval)

@@ -2402,7 +2402,7 @@ __global__ void __launch_bounds__(256) k_sp3d(SpMasks M, uint64_t* __restrict__ 
 // writes; the frontier moves between lanes and is fenced.
 // ===========================================================================
 struct InitJob {
-    uint32_t* sigma;           // padded lattice, zero on entry
+    uint32_t* sigma;           // padded lattice (the model's own device copy)
     uint32_t* frontier;        // cell i's frontier at fr_start[i]
     uint32_t* seeds;           // 3 coordinates per cell
     const uint32_t* fr_start;  // frontier offset of each cell
@@ -2414,7 +2414,7 @@ struct InitJob {
     int32_t copy_vec[26][3];   // copy stencil, as per-axis offsets
     uint32_t n_cells, n_copy, D, periodic;
     uint32_t status;           // out: 0 grown; 1 + i: seed i found no room
-    uint32_t pad_;
+    uint32_t n_padded;         // sigma's length (cleared first)
 };
 // cells per model the device places (order, volume and frontier length in LDS)
 constexpr uint32_t kInitMaxCells = 4096;
@@ -2442,6 +2442,8 @@ __global__ void __launch_bounds__(64) k_scatter_grow(InitJob* __restrict__ jobs)
     uint32_t* const sigma = J.sigma;
     uint32_t* const fr = J.frontier;
     Xoshiro r{{J.rng[0], J.rng[1], J.rng[2], J.rng[3]}};
+    for (uint32_t i = lane; i < J.n_padded; i += 64) sigma[i] = 0;
+    init_fence();
     auto flat_of = [&](const uint32_t* c) {
         uint32_t f = 0;
         for (uint32_t d = 0; d < D; ++d) f += (c[d] + 1) * strides[d];
@@ -2584,6 +2586,76 @@ __global__ void __launch_bounds__(64) k_scatter_grow(InitJob* __restrict__ jobs)
     if (lane == 0) {
         for (int k = 0; k < 4; ++k) J.rng[k] = r.s[k];
         J.status = 0;
+    }
+}
+
+// What initialization.rs:296-305 derives once the cells are grown, for
+// models whose state the device wrote: the periodic halo, then (after the
+// cells' tables are cleared) interfaces and position sums per site, with
+// integer atomics, so the order of the updates does not matter.  Contact
+// counts follow with k_contact_count, the energy with k_energy.
+__global__ void __launch_bounds__(256) k_halo(Rep* __restrict__ reps) {
+    Rep& R = reps[blockIdx.y];
+    if (R.boundary != CPM_BOUNDARY_PERIODIC) return;
+    const uint32_t D = R.D;
+    for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < R.n_padded;
+         f += gridDim.x * blockDim.x) {
+        uint32_t rem = f, src = 0;
+        bool halo = false;
+        for (uint32_t d = 0; d < D; ++d) {
+            const uint32_t p = rem / R.strides[d];
+            rem -= p * R.strides[d];
+            uint32_t w = p;
+            if (p == 0) {
+                w = R.dims[d];
+                halo = true;
+            } else if (p == R.dims[d] + 1) {
+                w = 1;
+                halo = true;
+            }
+            src += w * R.strides[d];
+        }
+        if (halo) R.sigma[f] = R.sigma[src];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_state_clear(Rep* __restrict__ reps) {
+    Rep& R = reps[blockIdx.y];
+    const uint64_t n = R.n_cells, D = R.D, n1 = n + 1;
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint64_t i = t0; i < n; i += step) R.interface[i] = 0;
+    for (uint64_t i = t0; i < n * D; i += step) R.sum_pos[i] = 0;
+    for (uint64_t i = t0; i < n * D * D; i += step) R.sum_outer[i] = 0;
+    for (uint64_t i = t0; i < n1 * n1; i += step) R.contact[i] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_recompute_cells(Rep* __restrict__ reps) {
+    Rep& R = reps[blockIdx.y];
+    const uint32_t D = R.D;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < R.n_sites;
+         q += gridDim.x * blockDim.x) {
+        uint32_t c[3] = {0, 0, 0}, rem = q;
+        uint64_t fi = 0;
+        for (int d = (int)D - 1; d >= 0; --d) {
+            c[d] = rem % R.dims[d];
+            rem /= R.dims[d];
+            fi += (uint64_t)(c[d] + 1) * R.strides[d];
+        }
+        const uint32_t ci = R.sigma[fi];
+        if (!ci) continue;
+        const uint64_t i = ci - 1;
+        uint32_t unlike = 0;
+        for (uint32_t k = 0; k < R.n_energy; ++k)
+            unlike += R.sigma[(uint64_t)((int64_t)fi + R.energy_off[k])] != ci ? 1u : 0u;
+        if (unlike) atomicAdd(&R.interface[i], unlike);
+        unsigned long long* sp = (unsigned long long*)(R.sum_pos + i * D);
+        unsigned long long* so = (unsigned long long*)(R.sum_outer + i * D * D);
+        for (uint32_t d = 0; d < D; ++d) {
+            atomicAdd(&sp[d], (unsigned long long)c[d]);
+            for (uint32_t e2 = 0; e2 < D; ++e2)
+                atomicAdd(&so[d * D + e2], (unsigned long long)((uint64_t)c[d] * c[e2]));
+        }
     }
 }
 
@@ -2838,9 +2910,12 @@ static uint32_t lds_class(const Engine* e, uint32_t bytes) {
     return std::min((bytes + 1023) / 1024 * 1024, e->max_lds - kLdsStaticReserve);
 }
 
-static void upload(Engine* e, Model* m) {
+// Allocates the model's device state and copies its parameters; with
+// `with_state` also its lattice and per-cell state (the device-side
+// initialisation writes those itself).
+static void upload(Engine* e, Model* m, bool with_state = true) {
     if (m->dev) return;
-    if (!m->initialized) throw Error(CPM_ERR_ARG, "model is not initialized");
+    if (with_state && !m->initialized) throw Error(CPM_ERR_ARG, "model is not initialized");
     uint32_t lds_bytes = 0;
     bool gsig = false;
     const bool lds = lds_eligible(e, *m, &lds_bytes, &gsig);
@@ -2936,15 +3011,16 @@ static void upload(Engine* e, Model* m) {
         if (bytes) HIP_OK(hipMemcpyAsync(d->base + off, src, bytes, hipMemcpyHostToDevice, e->stream));
     };
     const size_t n = m->n_cells, D = m->D;
-    put(L.sigma, m->sigma.data(), m->n_padded * 4);
     put(L.type_index, m->type_index.data(), n * 4);
+    put(L.params, params.data(), params.size() * 8);
+    put(L.targets, targets.data(), targets.size() * 4);
+    if (!with_state) return;  // the caller's kernels write the state (same stream)
+    put(L.sigma, m->sigma.data(), m->n_padded * 4);
     put(L.volume, m->volume.data(), n * 4);
     put(L.interface, m->interface.data(), n * 4);
     put(L.sum_pos, m->sum_pos.data(), n * D * 8);
     put(L.sum_outer, m->sum_outer.data(), n * D * D * 8);
     put(L.contact, m->contact.data(), (n + 1) * (n + 1) * 4);
-    put(L.params, params.data(), params.size() * 8);
-    put(L.targets, targets.data(), targets.size() * 4);
     if (m->cfg.act) put(L.birth, m->birth.data(), m->n_padded * 4);
     if (edges) {
         put(L.edge_sites, m->edge_sites.data(), m->n_sites * 4);
@@ -3009,6 +3085,13 @@ void sync_to_host(Model* m) {
     if (!m->dev || !m->device_dirty) return;
     const Layout L = layout_of(*m, m->dev->lds);
     const size_t n = m->n_cells, D = m->D;
+    // a model initialised on the device has host arrays of the right size
+    // only from its first sync on
+    m->volume.resize(n);
+    m->interface.resize(n);
+    m->sum_pos.resize(n * D);
+    m->sum_outer.resize(n * D * D);
+    m->contact.resize((n + 1) * (n + 1));
     const size_t contact_bytes = (n + 1) * (n + 1) * 4;
     // sigma through the position sums are adjacent in the allocation: one
     // copy for them (each copy call costs more than a few KiB of traffic),
@@ -3062,6 +3145,8 @@ static void sync_lattice_tables(Model* m) {
     if (!m->dev || !m->device_dirty) return;
     const Layout L = layout_of(*m, m->dev->lds);
     const size_t n = m->n_cells;
+    m->volume.resize(n);
+    m->interface.resize(n);
     std::vector<uint8_t> stage(L.interface + n * 4 - L.sigma);
     hipStream_t st = copy_stream_for(m->dev->eng);
     HIP_OK(hipStreamWaitEvent(st, m->dev->eng->work_done, 0));
@@ -3984,12 +4069,14 @@ void run_mcs_plain(Engine* e, Model* m, uint64_t n_mcs, cpm_counters* counters) 
 namespace cpm {
 
 // initialization::initialize of many models: scatter_and_grow on the device
-// for the default-initialised ones (at most kInitMaxCells cells), placed
-// and grown in one launch (one wave per model); explicit placements and
-// larger models initialise on the host threads.  The host then finishes
-// each model as initialization.rs:296-305 does (halo, derived state,
-// energy).  On failure the error of the lowest failing index is thrown,
-// with the reference's message.
+// for the default-initialised ones (at most kInitMaxCells cells, uniform
+// proposal, no Act term), placed and grown in one launch (one wave per
+// model) straight into each model's own device allocation, and finished
+// there as initialization.rs:296-305 does (halo, interfaces, position sums,
+// contacts, energy): the models start device-resident, their host arrays
+// filled at the first sync.  Explicit placements and the other models
+// initialise on the host threads.  On failure the error of the lowest
+// failing index is thrown, with the reference's message.
 void device_initialize(Engine* e, Model* const* ms, size_t n) {
     std::vector<std::exception_ptr> errs(n);
     std::vector<ScatterPlan> plans(n);
@@ -3997,7 +4084,8 @@ void device_initialize(Engine* e, Model* const* ms, size_t n) {
     std::vector<uint64_t> cap_of(n, 0);
     for (size_t i = 0; i < n; ++i) {
         Model& m = *ms[i];
-        bool dev = !m.cfg.init_explicit && m.n_padded < (1ull << 31);
+        bool dev = !m.cfg.init_explicit && m.n_padded < (1ull << 31) &&
+                   m.cfg.proposal == CPM_PROPOSAL_UNIFORM && !m.cfg.act;
         if (dev) {
             try {
                 plans[i] = m.scatter_plan();
@@ -4023,46 +4111,53 @@ void device_initialize(Engine* e, Model* const* ms, size_t n) {
         }
     });
     if (!on_dev.empty()) {
-        // one allocation: jobs, then the lattices (cleared), then per model
-        // seeds, frontier offsets, targets, volumes and frontiers
         const size_t nj = on_dev.size();
-        std::vector<size_t> sig_off(nj), aux_off(nj), fr_off(nj);
-        size_t off = align256(nj * sizeof(InitJob));
-        const size_t sig_begin = off;
+        std::lock_guard<std::mutex> lock(e->mu);
+        HIP_OK(hipSetDevice(e->device));
+        // each model's device allocation and parameters (the state follows)
         for (size_t k = 0; k < nj; ++k) {
-            sig_off[k] = off;
-            off += align256(ms[on_dev[k]]->n_padded * 4);
+            Model& m = *ms[on_dev[k]];
+            const ScatterPlan& p = plans[on_dev[k]];
+            device_release(&m);
+            m.initialized = false;
+            m.n_cells = p.type_of.size();
+            m.type_index = p.type_of;
+            upload(e, &m, false);
         }
-        const size_t sig_end = off;
+        // one scratch allocation: jobs, then per model seeds, frontier
+        // offsets and targets, then the frontiers
+        std::vector<size_t> aux_off(nj), fr_off(nj);
+        size_t off = align256(nj * sizeof(InitJob));
+        const size_t aux_begin = off;
         for (size_t k = 0; k < nj; ++k) {
             aux_off[k] = off;
-            off += align256(plans[on_dev[k]].type_of.size() * 4 * 6);
+            off += align256(plans[on_dev[k]].type_of.size() * 4 * 5);
         }
-        const size_t aux_end = off;  // copied back up to here; frontiers after
+        const size_t aux_end = off;
         for (size_t k = 0; k < nj; ++k) {
             fr_off[k] = off;
             off += align256(cap_of[on_dev[k]] * 4);
         }
-        std::lock_guard<std::mutex> lock(e->mu);
-        HIP_OK(hipSetDevice(e->device));
         char* buf = nullptr;
         HIP_OK(hipMalloc(&buf, off));
         std::vector<InitJob> jobs(nj);
-        std::vector<char> host_aux(aux_end - sig_end);  // fr_start and target, staged
+        std::vector<Rep> reps(nj);
+        std::vector<char> host_aux(aux_end - aux_begin);  // fr_start and target, staged
         uint32_t max_cells = 1;
         for (size_t k = 0; k < nj; ++k) {
             const Model& m = *ms[on_dev[k]];
             const ScatterPlan& p = plans[on_dev[k]];
             const uint32_t cells = (uint32_t)p.type_of.size();
             max_cells = std::max(max_cells, cells);
+            reps[k] = m.dev->rep;
             InitJob& J = jobs[k];
             std::memset(&J, 0, sizeof J);
             char* a = buf + aux_off[k];
-            J.sigma = (uint32_t*)(buf + sig_off[k]);
+            J.sigma = reps[k].sigma;
+            J.volume = reps[k].volume;
             J.seeds = (uint32_t*)a;
             J.fr_start = (const uint32_t*)(a + cells * 12);
             J.target = (const uint32_t*)(a + cells * 16);
-            J.volume = (uint32_t*)(a + cells * 20);
             J.frontier = (uint32_t*)(buf + fr_off[k]);
             for (int q = 0; q < 4; ++q) J.rng[q] = m.rng.s[q];
             J.sep_sq = p.sep_sq;
@@ -4078,7 +4173,8 @@ void device_initialize(Engine* e, Model* const* ms, size_t n) {
             J.n_copy = (uint32_t)nc;
             J.D = (uint32_t)m.D;
             J.periodic = m.cfg.boundary == CPM_BOUNDARY_PERIODIC ? 1u : 0u;
-            char* h = host_aux.data() + (aux_off[k] - sig_end);
+            J.n_padded = (uint32_t)m.n_padded;
+            char* h = host_aux.data() + (aux_off[k] - aux_begin);
             uint32_t* fs = (uint32_t*)(h + cells * 12);
             uint32_t* tg = (uint32_t*)(h + cells * 16);
             uint64_t at = 0;
@@ -4089,35 +4185,60 @@ void device_initialize(Engine* e, Model* const* ms, size_t n) {
                 at += (uint64_t)nc * std::max<uint32_t>(t, 1u);
             }
         }
-        HIP_OK(hipMemsetAsync(buf + sig_begin, 0, sig_end - sig_begin, e->stream));
         HIP_OK(hipMemcpyAsync(buf, jobs.data(), nj * sizeof(InitJob), hipMemcpyHostToDevice,
                               e->stream));
-        if (!host_aux.empty())
-            HIP_OK(hipMemcpyAsync(buf + sig_end, host_aux.data(), host_aux.size(),
-                                  hipMemcpyHostToDevice, e->stream));
+        HIP_OK(hipMemcpyAsync(buf + aux_begin, host_aux.data(), host_aux.size(),
+                              hipMemcpyHostToDevice, e->stream));
         hipLaunchKernelGGL(k_scatter_grow, dim3((uint32_t)nj), dim3(64), max_cells * 12, e->stream,
                            (InitJob*)buf);
         HIP_OK(hipGetLastError());
-        std::vector<char> back(aux_end);
-        HIP_OK(hipMemcpyAsync(back.data(), buf, aux_end, hipMemcpyDeviceToHost, e->stream));
+        // the derived state, over the models' descriptors (at most 65535 per launch)
+        ensure_buf(e->reps_dev, e->reps_cap, nj);
+        ensure_buf(e->idx_dev, e->idx_cap, nj);
+        std::vector<uint32_t> order(nj);
+        for (size_t k = 0; k < nj; ++k) order[k] = (uint32_t)k;
+        HIP_OK(hipMemcpyAsync(e->reps_dev, reps.data(), nj * sizeof(Rep), hipMemcpyHostToDevice,
+                              e->stream));
+        HIP_OK(hipMemcpyAsync(e->idx_dev, order.data(), nj * 4, hipMemcpyHostToDevice, e->stream));
+        for (size_t at = 0; at < nj; at += 65535) {
+            const dim3 g(64, (uint32_t)std::min<size_t>(nj - at, 65535));
+            hipLaunchKernelGGL(k_halo, g, dim3(256), 0, e->stream, e->reps_dev + at);
+            HIP_OK(hipGetLastError());
+            hipLaunchKernelGGL(k_state_clear, g, dim3(256), 0, e->stream, e->reps_dev + at);
+            HIP_OK(hipGetLastError());
+            hipLaunchKernelGGL(k_recompute_cells, g, dim3(256), 0, e->stream, e->reps_dev + at);
+            HIP_OK(hipGetLastError());
+            hipLaunchKernelGGL(k_contact_count, g, dim3(256), 0, e->stream, e->reps_dev,
+                               e->idx_dev + at);
+            HIP_OK(hipGetLastError());
+        }
+        hipLaunchKernelGGL(k_energy, dim3((uint32_t)nj), dim3(64), 0, e->stream, e->reps_dev,
+                           e->idx_dev);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipEventRecord(e->work_done, e->stream));
+        HIP_OK(hipMemcpyAsync(jobs.data(), buf, nj * sizeof(InitJob), hipMemcpyDeviceToHost,
+                              e->stream));
+        HIP_OK(hipMemcpyAsync(reps.data(), e->reps_dev, nj * sizeof(Rep), hipMemcpyDeviceToHost,
+                              e->stream));
         HIP_OK(hipStreamSynchronize(e->stream));
         HIP_OK(hipFree(buf));
-        const InitJob* out = (const InitJob*)back.data();
-        parallel_for(nj, [&](size_t k) {
+        for (size_t k = 0; k < nj; ++k) {
             Model& m = *ms[on_dev[k]];
-            const ScatterPlan& p = plans[on_dev[k]];
-            try {
-                if (out[k].status) m.crowded_error(p);
-                const size_t cells = p.type_of.size();
-                std::memcpy(m.sigma.data(), back.data() + sig_off[k], m.n_padded * 4);
-                const uint32_t* v = (const uint32_t*)(back.data() + aux_off[k] + cells * 20);
-                m.volume.assign(v, v + cells);
-                for (int q = 0; q < 4; ++q) m.rng.s[q] = out[k].rng[q];
-                m.finish_scatter(p);
-            } catch (...) {
-                errs[on_dev[k]] = std::current_exception();
+            if (jobs[k].status) {
+                device_release(&m);
+                try {
+                    m.crowded_error(plans[on_dev[k]]);
+                } catch (...) {
+                    errs[on_dev[k]] = std::current_exception();
+                }
+                continue;
             }
-        });
+            for (int q = 0; q < 4; ++q) m.rng.s[q] = jobs[k].rng[q];
+            m.energy = reps[k].energy;
+            m.used_default = true;
+            m.initialized = true;
+            m.device_dirty = true;  // the host arrays are filled at the first sync
+        }
     }
     for (auto& err : errs)
         if (err) std::rethrow_exception(err);

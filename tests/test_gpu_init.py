@@ -78,6 +78,20 @@ def test_device_scatter_and_grow_equals_the_host_initialisation():
         _compare(label, _state(r), want)
 
 
+def test_device_initialised_models_run_as_the_host_initialised_ones():
+    """The device-initialised state (derived tables included) is what the
+    first run starts from; every replica then equals its host-initialised
+    twin after 30 MCS, in one batched call."""
+    sims = [s for _, s in _sims()]
+    host = [cpm.Replica(s) for s in sims]
+    dev = [cpm.Replica(s, initialize=False) for s in sims]
+    cpm.Replica.initialize_many(dev)
+    cpm.Replica.run_mcs_many(host, 30)
+    cpm.Replica.run_mcs_many(dev, 30)
+    for i, (a, b) in enumerate(zip(dev, host)):
+        _compare(f"replica {i}", _state(a), _state(b))
+
+
 def test_device_initialisation_matches_the_oracle_and_runs_on():
     """The oracle's scatter_and_grow (pinned by the notebook) gives the same
     state, and the device-initialised model then replays the oracle's run."""
@@ -88,10 +102,11 @@ def test_device_initialisation_matches_the_oracle_and_runs_on():
     got, want = rep.snapshot(), ref.snapshot()
     for k in ("lattice", "volume", "interface", "sum_position", "sum_outer", "contact", "rng"):
         assert np.array_equal(got[k], want[k]), k
+    # the device-initialised state is resident: the run starts from it as is
     rep.run_mcs(200)
     ref.run_mcs(200)
     got, want = rep.snapshot(), ref.snapshot()
-    for k in ("lattice", "volume", "rng"):
+    for k in ("lattice", "volume", "interface", "sum_position", "sum_outer", "contact", "rng"):
         assert np.array_equal(got[k], want[k]), k
     assert _bits(got["energy"]) == _bits(want["energy"])
 
