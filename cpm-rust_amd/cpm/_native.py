@@ -236,6 +236,8 @@ def lib():
         L.cpm_binomial_check.restype = st
         L.cpm_engine_set_pipelining.argtypes = [vp, C.c_int32]
         L.cpm_engine_set_pipelining.restype = st
+        L.cpm_model_chain_rate.argtypes = [vp, C.POINTER(C.c_double)]
+        L.cpm_model_chain_rate.restype = st
         L.cpm_engine_set_device_init.argtypes = [vp, C.c_int32]
         L.cpm_engine_set_device_init.restype = st
         L.cpm_engine_kernel_launches.argtypes = [vp, C.POINTER(C.c_int32), u64p, C.c_size_t,

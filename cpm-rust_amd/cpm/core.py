@@ -94,6 +94,13 @@ class Replica:
     def energy(self) -> float:
         return float(self.info().conservative_energy)
 
+    def chain_rate(self) -> float:
+        """Copy attempts/s of this replica's own chain in its last
+        exact-replay call on the wave path (device clock; 0 before one)."""
+        out = C.c_double()
+        N.check(N.lib().cpm_model_chain_rate(self.h, C.byref(out)))
+        return float(out.value)
+
     def rng_state(self) -> np.ndarray:
         return np.array(list(self.info().rng_state), dtype=np.uint64)
 

@@ -691,6 +691,13 @@ cpm_status cpm_engine_set_device_init(cpm_engine* engine, int32_t on) {
     });
 }
 
+cpm_status cpm_model_chain_rate(cpm_model* model, double* attempts_per_s) {
+    return guarded([&] {
+        need(model && attempts_per_s, "model/attempts_per_s");
+        *attempts_per_s = model->m->attempt_rate;
+    });
+}
+
 cpm_status cpm_engine_kernel_launches(cpm_engine* engine, int32_t* keys, uint64_t* launches,
                                       size_t cap, size_t* n) {
     return guarded([&] {

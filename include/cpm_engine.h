@@ -461,6 +461,12 @@ cpm_status cpm_engine_set_pipelining(cpm_engine* engine, int32_t on);
    threads, as cpm_model_initialize does. */
 cpm_status cpm_engine_set_device_init(cpm_engine* engine, int32_t on);
 
+/* Copy attempts per second of the model's own chain in its last
+   exact-replay call on the wave path, from the device clock around the
+   chain (0 before any such call).  No counterpart in the reference: a
+   diagnostic for schedulers and load balance. */
+cpm_status cpm_model_chain_rate(cpm_model* model, double* attempts_per_s);
+
 /* How often each exact-replay wave-kernel variant has been launched by
    this engine: writes up to cap (key, launches) pairs and sets *n to the
    number of variants launched so far.  Key = variant * 2000 + base, where
