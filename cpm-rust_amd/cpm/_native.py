@@ -59,7 +59,8 @@ class CpmCounters(C.Structure):
     _fields_ = [("attempts", C.c_uint64), ("non_null", C.c_uint64), ("topology", C.c_uint64),
                 ("energy", C.c_uint64), ("accepted", C.c_uint64),
                 ("exp_near_ties", C.c_uint64), ("kernel_ms", C.c_double),
-                ("replay_launches", C.c_uint64), ("replay_launch_ms", C.c_double)]
+                ("replay_launches", C.c_uint64), ("replay_launch_ms", C.c_double),
+                ("tail_rounds", C.c_uint64)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

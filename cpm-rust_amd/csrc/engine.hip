@@ -479,6 +479,12 @@ __constant__ uint64_t c_jump[kSegs][4];
 // Orders one wave's LDS and global accesses around a point where its lanes
 // exchange data (a workgroup barrier minus the rendezvous, so that it is
 // safe inside code only one wave of a workgroup runs).
+// Orders this thread's earlier global-memory stores before its later
+// accesses for every thread of its workgroup (all of them on one CU, one
+// L1): the exact-replay kernels' lattice words in HBM are written and read
+// by the replica's own wave(s) only, with plain loads and stores.
+__device__ __forceinline__ void wg_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
+
 __device__ __forceinline__ void wsync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
@@ -503,7 +509,7 @@ __device__ void wave_fill(uint64_t* __restrict__ draws, uint64_t* __restrict__ s
     for (int k = 0; k < 4; ++k) segs[lane * 4 + k] = s.s[k];
     uint64_t* __restrict__ out = draws + (size_t)lane * kSegLen;
     for (uint32_t i = 0; i < kSegLen; ++i) out[i] = xo_next(s);
-    __threadfence();  // the wave reads other lanes' segments next
+    wg_fence();  // the wave reads other lanes' segments next
     wsync();
 }
 
@@ -720,7 +726,7 @@ __global__ void __launch_bounds__(PW ? 128 : 64) __attribute__((amdgpu_waves_per
         const uint32_t id = R.sigma[i];
         sig.st(i, id ? id | ((gtyp[id - 1] + 1) << 12) : 0u);
     }
-    if constexpr (SG) __threadfence();
+    if constexpr (SG) wg_fence();
     for (uint32_t i = threadIdx.x; i < n_cells; i += NTH) {
         vol[i] = R.volume[i];
         itf[i] = R.interface[i];
@@ -1068,7 +1074,7 @@ __global__ void __launch_bounds__(PW ? 128 : 64) __attribute__((amdgpu_waves_per
                 }                                                                               \
             }                                                                                   \
         }                                                                                       \
-        if constexpr (SG) __threadfence();  /* the next evaluation reads sigma from HBM */      \
+        if constexpr (SG) wg_fence();  /* the next evaluation reads sigma from HBM */          \
         wsync();                                                                                \
         energy += dh;                                                                           \
         /* the lattice changed: every later attempt is evaluated again */                       \
@@ -1367,7 +1373,7 @@ __global__ void __launch_bounds__(PW ? 128 : 64) __attribute__((amdgpu_waves_per
                     hr_ready = false;
                 }
                 if constexpr (SG) {
-                    if (s_cmt[cur]) __threadfence();  // sigma words the walker rewrote
+                    if (s_cmt[cur]) wg_fence();  // sigma words the walker rewrote
                 }
                 CPM_PRODUCE(nxt, cmd == kRestart || s_nv[cur] == 0);
             } else {
@@ -1438,7 +1444,7 @@ __global__ void __launch_bounds__(PW ? 128 : 64) __attribute__((amdgpu_waves_per
             __syncthreads();
         }
         if (wave == 1) return;
-        __threadfence();  // the producer's last refill of the segment states
+        wg_fence();  // the producer's last refill of the segment states
     }
     base += p;  // buffer position of the next unconsumed draw
 #undef CPM_DECODE
@@ -1607,7 +1613,7 @@ __global__ void __launch_bounds__(NT) k_sublattice(Rep* __restrict__ reps,
         dims[d] = R.dims[d];
         stride[d] = R.strides[d];
     }
-    const uint32_t n_copy = R.n_copy, n_energy = R.n_energy, n_topo = R.n_topo;
+    const uint32_t n_copy = R.n_copy, n_energy = R.n_energy;
     const bool periodic = R.boundary == CPM_BOUNDARY_PERIODIC;
     const uint64_t key = R.fill_state[0];
     unsigned long long c_att = 0, c_nn = 0, c_top = 0, c_en = 0, c_acc = 0;
@@ -1670,8 +1676,15 @@ __global__ void __launch_bounds__(NT) k_sublattice(Rep* __restrict__ reps,
                         const uint32_t tl = lidx ? lidx - 1 : 0u, tg = gidx ? gidx - 1 : 0u;
                         int nl = 0, ng = 0;
                         double dadh = 0.0;  // energy.rs:242-264, stencil order
-                        for (uint32_t q = 0; q < n_energy; ++q) {
-                            const uint32_t w = sig.ld((int32_t)t + R.energy_off[q]);
+                        // the stencil's words, all reads in flight together
+                        uint32_t ew[26];
+#pragma unroll
+                        for (uint32_t q = 0; q < 26; ++q)
+                            ew[q] = q < n_energy ? sig.ld((int32_t)t + R.energy_off[q]) : 0u;
+#pragma unroll
+                        for (uint32_t q = 0; q < 26; ++q) {
+                            if (q >= n_energy) break;
+                            const uint32_t w = ew[q];
                             const uint32_t nb = w & kIdMask, ni = w >> 12;
                             nl += nb == L;
                             ng += nb == G;
@@ -1783,11 +1796,18 @@ __global__ void __launch_bounds__(NT) k_sublattice(Rep* __restrict__ reps,
                     uint32_t pj[KS], pw[KS];  // site index; packed words L | G << 16
                     unsigned long long ppr[KS];
                     uint32_t pmask = 0;  // bit q: slot q holds a pending attempt
+                    // the thread's attempts of this batch: targets and sources
+                    // first, then every lattice read of the null checks at
+                    // once (one memory round trip, not one per attempt)
+                    uint32_t tq[KS], sq[KS], lq[KS], gq[KS];
+                    uint32_t amask = 0;  // bit q: slot q is attempted
 #pragma unroll
                     for (uint32_t q = 0; q < KS; ++q) {
                         pj[q] = 0;
                         pw[q] = 0;
                         ppr[q] = 0;
+                        tq[q] = 0;
+                        sq[q] = 0;
                         const uint32_t jb = b0 + q * NT + tid;
                         if (jb >= nsite) continue;
                         uint32_t j = jb + rot;
@@ -1798,24 +1818,45 @@ __global__ void __launch_bounds__(NT) k_sublattice(Rep* __restrict__ reps,
                         const uint64_t h0 = splitmix_at(key, ctr);
                         if ((h0 & (kSlSub - 1)) != 0) continue;  // this site is not attempted
                         const uint32_t k = (uint32_t)(((h0 >> 32) * (uint64_t)n_copy) >> 32);
-                        const uint32_t s = (uint32_t)((int32_t)t + R.copy_off[k]);
+                        amask |= 1u << q;
+                        pj[q] = j;
+                        tq[q] = t;
+                        sq[q] = (uint32_t)((int32_t)t + R.copy_off[k]);
+                    }
+#pragma unroll
+                    for (uint32_t q = 0; q < KS; ++q) {
+                        lq[q] = (amask >> q) & 1u ? sig.ld(tq[q]) : 0u;
+                        gq[q] = (amask >> q) & 1u ? sig.ld(sq[q]) : 0u;
+                    }
+#pragma unroll
+                    for (uint32_t q = 0; q < KS; ++q) {
+                        if (!((amask >> q) & 1u)) continue;
+                        const uint32_t j = pj[q], t = tq[q];
+                        const uint64_t ctr = (phase * (uint64_t)R.n_sites + j) * 4;
                         c_att++;
-                        const uint32_t Lw = sig.ld(t), Gw = sig.ld(s);
+                        const uint32_t Lw = lq[q], Gw = gq[q];
                         if (Lw == Gw) continue;
                         c_nn++;
                         const uint32_t L = Lw & kIdMask;
                         bool simple = true;
                         if (L != 0) {
+                            // the full shell (3^D - 1 sites, as the packed-word
+                            // upload checks), all reads in flight together
+                            constexpr uint32_t NTOPO = D == 2 ? 8u : 26u;
+                            uint32_t nbw[NTOPO];
+#pragma unroll
+                            for (uint32_t i = 0; i < NTOPO; ++i)
+                                nbw[i] = sig.ld((int32_t)t + R.topo_off[i]);
                             uint32_t pat = 0;
-                            for (uint32_t i = 0; i < n_topo; ++i)
-                                pat |= (uint32_t)((sig.ld((int32_t)t + R.topo_off[i]) & kIdMask) == L) << i;
+#pragma unroll
+                            for (uint32_t i = 0; i < NTOPO; ++i)
+                                pat |= (uint32_t)((nbw[i] & kIdMask) == L) << i;
                             c_top++;
                             simple = D == 2 ? ((c_sp2d[pat >> 5] >> (pat & 31)) & 1u)
                                             : ((sp3d[pat >> 6] >> (pat & 63)) & 1ull);
                         }
                         if (!simple) continue;
                         pmask |= 1u << q;
-                        pj[q] = j;
                         pw[q] = Lw | Gw << 16;
                         // unique within the phase: site index in the low 24 bits
                         ppr[q] = 0x8000000000000000ull |
@@ -3300,6 +3341,9 @@ static void launch_lds_engine(int key, uint32_t lds, uint32_t count, hipStream_t
 // noalloc: allocate no device memory and create no events — every model
 // must be resident already, the descriptors and launch order go to
 // `scratch`, and launches are not timed one by one.
+constexpr double kHbmSpeed = 0.73;     // chain speed with sigma in HBM over in LDS (measured)
+constexpr double kTailMargin = 1.05;   // a chain past the LDS chains by this much splits
+constexpr double kTailMaxShare = 0.10; // at most this share of a batch's chains splits
 static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t* todo,
                       Placement placement, cpm_counters* counters, double budget_s = 0.0,
                       bool noalloc = false) {
@@ -3428,6 +3472,39 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
             place();
         }
     }
+    // The longest chains left with sigma in HBM by the LDS budget would run
+    // well past every chain whose lattice is in LDS (an HBM lattice costs
+    // its chain ~1/kHbmSpeed per attempt): they run, beside the others, only
+    // the MCS an LDS chain of the longest length leaves time for, and
+    // finish in a second round on the emptied chip, where their lattices
+    // fit in LDS and each gets the pipelined kernel (so the second round
+    // belongs to the pipelining switch).  Exact replay does not depend on
+    // how MCS are split into launches.
+    std::vector<uint64_t> td(todo, todo + n), tail(n, 0);
+    size_t n_tail = 0;
+    if (budget_s == 0.0 && !noalloc && e->pipelining) {
+        auto work = [&](size_t i) { return (double)todo[i] * (double)models[i]->n_sites; };
+        double lds_max = 0.0;
+        size_t waves = 0;
+        for (size_t i = 0; i < n; ++i)
+            if (todo[i] && models[i]->dev->wave) {
+                ++waves;
+                if (!in_hbm[i] || piped[i]) lds_max = std::max(lds_max, work(i));
+            }
+        std::vector<size_t> cand;
+        for (size_t i = 0; i < n; ++i)
+            if (todo[i] && models[i]->dev->wave && in_hbm[i] && !piped[i] && !models[i]->dev->gsig &&
+                work(i) > kTailMargin * kHbmSpeed * lds_max)
+                cand.push_back(i);
+        if (!cand.empty() && (double)cand.size() <= kTailMaxShare * (double)waves)
+            for (size_t i : cand) {
+                const uint64_t first = (uint64_t)((double)todo[i] * kHbmSpeed * lds_max / work(i));
+                td[i] = first;
+                tail[i] = todo[i] - first;
+                ++n_tail;
+            }
+    }
+    todo = td.data();
     // partition: global-path replicas, and wave-path replicas by kernel
     // instantiation (the key of launch_lds_engine)
     std::vector<uint32_t> order, global_idx;
@@ -3616,6 +3693,17 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
                 m->attempt_rate = (double)reps[i].cnt[0] / ((double)reps[i].t_run * 1e-8);
         }
         add_counters(counters, reps[i]);
+    }
+    if (n_tail) {
+        if (counters) counters->tail_rounds += 1;
+        std::vector<Model*> ms;
+        std::vector<uint64_t> rest;
+        for (size_t i = 0; i < n; ++i)
+            if (tail[i]) {
+                ms.push_back(models[i]);
+                rest.push_back(tail[i]);
+            }
+        run_exact(e, ms.data(), ms.size(), rest.data(), placement, counters);
     }
 }
 
@@ -3862,6 +3950,7 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
             counters->kernel_ms += cc.kernel_ms;
             counters->replay_launches += cc.replay_launches;
             counters->replay_launch_ms += cc.replay_launch_ms;
+            counters->tail_rounds += cc.tail_rounds;
         }
     }
     int best = 0;

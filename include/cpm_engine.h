@@ -101,7 +101,8 @@ typedef struct {
 /* Per-launch event counters: A attempts, N non-null attempts (sigma_t !=
    sigma_s), Nt attempts that read the simple-point table, E energy
    evaluations, C accepted copies; exp_near_ties counts Metropolis draws
-   that fell within 4 ulp of exp(-dH) (decided with the device exp). */
+   within 2^-40 (relative) of exp(-dH), decided with the correctly rounded
+   exp instead of the device libm's. */
 typedef struct {
     uint64_t attempts, non_null, topology, energy, accepted, exp_near_ties;
     double kernel_ms; /* device time of the launches, HIP events */
@@ -110,6 +111,9 @@ typedef struct {
        stream it runs on */
     uint64_t replay_launches;
     double replay_launch_ms;
+    /* exact-replay batches whose longest chains (lattices left in HBM by
+       the LDS budget) finished in a second round on the emptied device */
+    uint64_t tail_rounds;
 } cpm_counters;
 
 typedef struct cpm_engine cpm_engine;

@@ -323,3 +323,37 @@ def test_device_simple_point_tables_match_the_definition():
     assert "".join(str(int(b)) for b in t2) == PINS["sp_table_2d_bits"]
     assert hashlib.sha256(t3.astype("<u8").tobytes()).hexdigest() == PINS["sp_table_3d_sha256"]
     del C
+
+
+def test_tail_round_matches_one_round():
+    """The batched workload (4096 replicas of the prior box, sides 48-128):
+    with the longest-chains-first placement the 128x128 lattices that the
+    LDS budget leaves in HBM run part of their MCS beside the others and
+    finish in a second round on the emptied chip.  Exact replay does not
+    depend on the split: every replica equals its twin run in one round
+    (pipelining, and with it the second round, switched off)."""
+    import bench
+
+    params = bench.draw_batch(4096)
+    sims = [bench.make_sim(cpm, p, cpm.derive_seed_u64(42, p["index"])) for p in params]
+    out = {}
+    try:
+        cpm._native.set_sigma_placement("longest", 1.0)
+        for piped in (True, False):
+            cpm._native.set_pipelining(piped)
+            reps = [cpm.Replica(s, initialize=False) for s in sims]
+            cpm.Replica.initialize_many(reps)
+            tails = 0
+            for _ in range(2):
+                tails += cpm.Replica.run_mcs_many(reps, 60)["tail_rounds"]
+            print(f"[case] pipelining {piped}: {tails} tail rounds", flush=True)
+            assert (tails > 0) == piped
+            out[piped] = [r.snapshot() for r in reps]
+            del reps
+    finally:
+        cpm._native.set_sigma_placement("auto", 0.0)
+        cpm._native.set_pipelining(True)
+    for i, (a, b) in enumerate(zip(out[True], out[False])):
+        for key in ("lattice", "volume", "interface", "sum_position", "sum_outer", "contact", "rng"):
+            assert np.array_equal(a[key], b[key]), (i, key)
+        assert a["energy"] == b["energy"], i
