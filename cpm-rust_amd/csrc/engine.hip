@@ -1604,7 +1604,7 @@ __global__ void __launch_bounds__(NT) k_sublattice(Rep* __restrict__ reps,
         ti[i] = R.targets[T + i];
     }
     if (tid < 6) s_cnt[tid] = 0;
-    if constexpr (SG) __threadfence();  // the staged sigma words, HBM
+    if constexpr (SG) wg_fence();  // the staged sigma words, HBM
     __syncthreads();
 
     uint32_t dims[D], stride[D];
@@ -1749,7 +1749,12 @@ __global__ void __launch_bounds__(NT) k_sublattice(Rep* __restrict__ reps,
                                 const uint32_t vbg = max(vg_new - 1, 1u);
                                 int64_t img[D];
                                 for (int d = 0; d < D; ++d) {
-                                    const double cen = (double)sp[d] / (double)vbg;
+                                    // an atomic (device-coherent) read: the sums
+                                    // are updated with atomics by other threads
+                                    const double cen =
+                                        (double)__hip_atomic_load(&sp[d], __ATOMIC_RELAXED,
+                                                                  __HIP_MEMORY_SCOPE_AGENT) /
+                                        (double)vbg;
                                     int64_t best = (int64_t)c[d];
                                     if (periodic) {
                                         double bd = fabs((double)best - cen);
@@ -1904,7 +1909,7 @@ __global__ void __launch_bounds__(NT) k_sublattice(Rep* __restrict__ reps,
                             pmask &= ~(1u << qw);
                             commit_win(jw, ww & 0xffffu, ww >> 16);
                         }
-                        if constexpr (SG) __threadfence();  // sigma writes, HBM
+                        if constexpr (SG) wg_fence();  // sigma writes, HBM
                     }
                 }
                 __syncthreads();
