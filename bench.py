@@ -584,6 +584,9 @@ def main():
         print(f"[bench] step {s}: {c['kernel_ms']:.0f} ms, {c['attempts'] / c['kernel_ms'] / 1e6:.2f}"
               f" G attempts/s", flush=True)
     wall = time.perf_counter() - t0
+    from cpm import _native as _nat
+    placement = "%s-first, %.0f %% of the LDS" % (lambda o, sh: (o, 100 * sh))(
+        *_nat.sigma_placement())
     value = totals["attempts"] / wall
     # roofline of the dominant kernel: algorithmic bytes per launch over its
     # average launch duration (HIP events around each launch, on its stream)
@@ -652,7 +655,8 @@ def main():
         "config": {"workload": f"run_batch: {n_reps} theta-replicas, 2D periodic sides "
                                f"{{48,64,96,128}}, 2 types, phi=0.7, {args.interval} MCS per step",
                    "replicas": n_reps, "interval_mcs": args.interval,
-                   "parallelism": "1 GPU, replicas batched in one launch"},
+                   "parallelism": "1 GPU, replicas batched in one launch",
+                   "sigma_placement": placement},
         "plain_attempts_per_s": plain_rate,
         "batched_vs_plain": value / plain_rate,
         "sims_per_hour": None if full is None else full["sims_per_hour"],

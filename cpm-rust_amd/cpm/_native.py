@@ -237,6 +237,9 @@ def lib():
         L.cpm_binomial_check.restype = st
         L.cpm_engine_set_pipelining.argtypes = [vp, C.c_int32]
         L.cpm_engine_set_pipelining.restype = st
+        L.cpm_engine_sigma_placement.argtypes = [vp, C.POINTER(C.c_int32),
+                                                 C.POINTER(C.c_double)]
+        L.cpm_engine_sigma_placement.restype = st
         L.cpm_model_chain_rate.argtypes = [vp, C.POINTER(C.c_double)]
         L.cpm_model_chain_rate.restype = st
         L.cpm_engine_set_device_init.argtypes = [vp, C.c_int32]
@@ -358,6 +361,14 @@ def binomial_check(states, n, p, device: bool = True):
                                    nn.ctypes.data_as(u64p), pp.ctypes.data_as(f64p), nn.size,
                                    k.ctypes.data_as(u64p)))
     return k, st
+
+
+def sigma_placement() -> tuple[str, float]:
+    """The placement ("smallest"/"longest" first, LDS share) of the last
+    exact-replay call ("none", 0.0 before any)."""
+    o, sh = C.c_int32(0), C.c_double(0.0)
+    check(lib().cpm_engine_sigma_placement(engine(), C.byref(o), C.byref(sh)))
+    return {0: "none", 1: "smallest", 2: "longest"}[o.value], float(sh.value)
 
 
 def set_pipelining(on: bool) -> None:

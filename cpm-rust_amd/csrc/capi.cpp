@@ -609,6 +609,15 @@ cpm_status cpm_engine_set_sigma_placement(cpm_engine* engine, int32_t order, dou
     });
 }
 
+cpm_status cpm_engine_sigma_placement(cpm_engine* engine, int32_t* order, double* share) {
+    return guarded([&] {
+        need(engine && order && share, "engine/order/share");
+        int o = 0;
+        engine_last_placement(engine->e, &o, share);
+        *order = o;
+    });
+}
+
 cpm_status cpm_metropolis_check(cpm_engine* engine, const double* delta_h, const double* u,
                                 size_t n, double* exp_libm, double* exp_cr, uint8_t* accept) {
     return guarded([&] {

@@ -54,6 +54,8 @@ void engine_sp_tables(Engine* e, uint8_t* t2, uint64_t* t3);
 // lattices first, 2: longest chains first, filling `share` of the chip's
 // LDS) or hand it back to the autotuner (order 0).
 void engine_set_placement(Engine* e, int order, double share);
+// The placement the last exact-replay call used (order 0: none yet).
+void engine_last_placement(Engine* e, int* order, double* share);
 // exp(-dh[i]) with the device libm and correctly rounded, and (when u is
 // given) the exact-replay kernels' Metropolis decision for u[i].
 void engine_metropolis_check(Engine* e, const double* dh, const double* u, size_t n,

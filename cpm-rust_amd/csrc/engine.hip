@@ -2772,6 +2772,8 @@ struct Engine {
     hipEvent_t work_done = nullptr;
     // a sigma placement fixed by the caller (fill 0: the autotuner decides)
     Placement forced{0.0, false};
+    // the placement of the last exact-replay call (fill 0: none yet)
+    Placement last_placement{0.0, false};
     // false: no replica runs on the pipelined (two-wave) kernel
     bool pipelining = true;
     // this device's copy of `scratch` (hipGetSymbolAddress at creation)
@@ -3348,7 +3350,7 @@ static void launch_lds_engine(int key, uint32_t lds, uint32_t count, hipStream_t
 // `scratch`, and launches are not timed one by one.
 constexpr double kHbmSpeed = 0.73;     // chain speed with sigma in HBM over in LDS (measured)
 constexpr double kTailMargin = 1.05;   // a chain past the LDS chains by this much splits
-constexpr double kTailMaxShare = 0.10; // at most this share of a batch's chains splits
+constexpr double kTailMaxShare = 0.20; // at most this share of a batch's chains splits
 static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t* todo,
                       Placement placement, cpm_counters* counters, double budget_s = 0.0,
                       bool noalloc = false) {
@@ -3894,6 +3896,20 @@ static Placement fixed_placement() {
 // large enough call of a new batch signature is split: two slices of its
 // MCS under each placement candidate, timed, then the rest under the fastest,
 // which later calls with that signature reuse.
+static void add_call_counters(cpm_counters* to, const cpm_counters& cc) {
+    if (!to) return;
+    to->attempts += cc.attempts;
+    to->non_null += cc.non_null;
+    to->topology += cc.topology;
+    to->energy += cc.energy;
+    to->accepted += cc.accepted;
+    to->exp_near_ties += cc.exp_near_ties;
+    to->kernel_ms += cc.kernel_ms;
+    to->replay_launches += cc.replay_launches;
+    to->replay_launch_ms += cc.replay_launch_ms;
+    to->tail_rounds += cc.tail_rounds;
+}
+
 void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* todo, int mode,
                   cpm_counters* counters, double budget_s) {
     if (mode != CPM_MODE_EXACT_REPLAY && mode != CPM_MODE_SUBLATTICE)
@@ -3910,11 +3926,12 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
     for (size_t i = 0; i < n; ++i) upload(e, models[i]);
     const Placement fixed = e->forced.fill > 0.0 ? e->forced : fixed_placement();
     if (budget_s > 0) {  // a time-limited call: no tuning slices
-        run_exact(e, models, n, todo, fixed.fill > 0.0 ? fixed : kDefaultPlacement, counters,
-                  budget_s);
+        e->last_placement = fixed.fill > 0.0 ? fixed : kDefaultPlacement;
+        run_exact(e, models, n, todo, e->last_placement, counters, budget_s);
         return;
     }
     if (fixed.fill > 0.0) {
+        e->last_placement = fixed;
         run_exact(e, models, n, todo, fixed, counters);
         return;
     }
@@ -3928,9 +3945,9 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
     // short calls (serving rounds, sampling intervals of a few MCS) are not
     // worth the extra launches: they run with the default placement
     if (it != e->tuned_placement.end() || slice < kTuneMinSliceMcs || n < 2) {
-        run_exact(e, models, n, todo,
-                  it != e->tuned_placement.end() ? kPlacements[it->second] : kDefaultPlacement,
-                  counters);
+        e->last_placement =
+            it != e->tuned_placement.end() ? kPlacements[it->second] : kDefaultPlacement;
+        run_exact(e, models, n, todo, e->last_placement, counters);
         return;
     }
     std::vector<uint64_t> left(todo, todo + n), part(n);
@@ -3945,26 +3962,44 @@ void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* tod
         run_exact(e, models, n, part.data(), kPlacements[c], &cc);
         att[c] += (double)cc.attempts;
         ms[c] += cc.kernel_ms;
-        if (counters) {
-            counters->attempts += cc.attempts;
-            counters->non_null += cc.non_null;
-            counters->topology += cc.topology;
-            counters->energy += cc.energy;
-            counters->accepted += cc.accepted;
-            counters->exp_near_ties += cc.exp_near_ties;
-            counters->kernel_ms += cc.kernel_ms;
-            counters->replay_launches += cc.replay_launches;
-            counters->replay_launch_ms += cc.replay_launch_ms;
-            counters->tail_rounds += cc.tail_rounds;
-        }
+        add_call_counters(counters, cc);
     }
     int best = 0;
     for (int c = 1; c < kNumPlacements; ++c)
         if (att[c] * ms[best] > att[best] * ms[c]) best = c;
+    // the runner-up and the best again on slices twice as long (best,
+    // runner-up, runner-up, best), which decide between them
+    int second = best == 0 ? 1 : 0;
+    for (int c = 0; c < kNumPlacements; ++c)
+        if (c != best && att[c] * ms[second] > att[second] * ms[c]) second = c;
+    double att2[2] = {}, ms2[2] = {};
+    for (int r = 0; r < 4; ++r) {
+        const int w = (r == 0 || r == 3) ? 0 : 1;
+        bool any = false;
+        for (size_t i = 0; i < n; ++i) {
+            part[i] = std::min(left[i], 2 * slice);
+            left[i] -= part[i];
+            any |= part[i] != 0;
+        }
+        if (!any) break;
+        cpm_counters cc{};
+        run_exact(e, models, n, part.data(), kPlacements[w ? second : best], &cc);
+        att2[w] += (double)cc.attempts;
+        ms2[w] += cc.kernel_ms;
+        add_call_counters(counters, cc);
+    }
+    if (att2[0] > 0 && att2[1] > 0 && att2[1] * ms2[0] > att2[0] * ms2[1]) best = second;
     e->tuned_placement[sig] = best;
+    e->last_placement = kPlacements[best];
     bool more = false;
     for (size_t i = 0; i < n; ++i) more |= left[i] != 0;
     if (more) run_exact(e, models, n, left.data(), kPlacements[best], counters);
+}
+
+void engine_last_placement(Engine* e, int* order, double* share) {
+    std::lock_guard<std::mutex> lock(e->mu);
+    *order = e->last_placement.fill > 0.0 ? (e->last_placement.longest_first ? 2 : 1) : 0;
+    *share = e->last_placement.fill;
 }
 
 void engine_set_placement(Engine* e, int order, double share) {

@@ -448,6 +448,10 @@ cpm_status cpm_binomial_check(cpm_engine* engine, uint64_t* rng_state, const uin
    not depend on the placement. */
 enum { CPM_PLACEMENT_AUTO = 0, CPM_PLACEMENT_SMALLEST = 1, CPM_PLACEMENT_LONGEST = 2 };
 cpm_status cpm_engine_set_sigma_placement(cpm_engine* engine, int32_t order, double share);
+/* The placement the engine's last exact-replay call used (the caller's,
+   or the autotuner's choice for that batch): order and share as above,
+   order 0 before any such call. */
+cpm_status cpm_engine_sigma_placement(cpm_engine* engine, int32_t* order, double* share);
 
 /* Exact-replay pipelining (no counterpart in the reference): by default
    the replicas with the longest chains run on a two-wave kernel (one wave
