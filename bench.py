@@ -575,7 +575,7 @@ def main():
 
     totals = {k: 0.0 for k in ("attempts", "non_null", "topology", "energy", "accepted",
                                "exp_near_ties", "kernel_ms", "replay_launches",
-                               "replay_launch_ms", "tail_rounds")}
+                               "replay_launch_ms", "piped_stragglers")}
     t0 = time.perf_counter()
     for s in range(args.steps):
         c = cpm.Replica.run_mcs_many(reps, args.interval)

@@ -60,7 +60,7 @@ class CpmCounters(C.Structure):
                 ("energy", C.c_uint64), ("accepted", C.c_uint64),
                 ("exp_near_ties", C.c_uint64), ("kernel_ms", C.c_double),
                 ("replay_launches", C.c_uint64), ("replay_launch_ms", C.c_double),
-                ("tail_rounds", C.c_uint64)]
+                ("piped_stragglers", C.c_uint64)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

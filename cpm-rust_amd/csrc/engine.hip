@@ -2737,8 +2737,9 @@ struct Placement {
 };
 constexpr Placement kDefaultPlacement = {0.75, false};
 // the candidates the autotuner tries
-constexpr Placement kPlacements[] = {{0.0, false},  {0.5, false}, {0.75, false}, {1.0, false},
-                                     {8.0, false},  {0.75, true}, {0.9, true},   {1.0, true}};
+constexpr Placement kPlacements[] = {{0.0, false}, {0.5, false}, {0.75, false},
+                                     {1.0, false}, {8.0, false}, {0.6, true},
+                                     {0.75, true}, {0.85, true}, {1.0, true}};
 constexpr int kNumPlacements = sizeof(kPlacements) / sizeof(kPlacements[0]);
 constexpr uint64_t kTuneMinSliceMcs = 4;  // tune only calls of at least 4 MCS per slice
 
@@ -3349,8 +3350,8 @@ static void launch_lds_engine(int key, uint32_t lds, uint32_t count, hipStream_t
 // must be resident already, the descriptors and launch order go to
 // `scratch`, and launches are not timed one by one.
 constexpr double kHbmSpeed = 0.73;     // chain speed with sigma in HBM over in LDS (measured)
-constexpr double kTailMargin = 1.05;   // a chain past the LDS chains by this much splits
-constexpr double kTailMaxShare = 0.20; // at most this share of a batch's chains splits
+constexpr double kStragglerMargin = 1.05;  // an HBM chain past the LDS chains by this much
+constexpr double kStragglerShare = 0.20;   // at most this share of a batch's chains
 static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t* todo,
                       Placement placement, cpm_counters* counters, double budget_s = 0.0,
                       bool noalloc = false) {
@@ -3480,38 +3481,44 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
         }
     }
     // The longest chains left with sigma in HBM by the LDS budget would run
-    // well past every chain whose lattice is in LDS (an HBM lattice costs
-    // its chain ~1/kHbmSpeed per attempt): they run, beside the others, only
-    // the MCS an LDS chain of the longest length leaves time for, and
-    // finish in a second round on the emptied chip, where their lattices
-    // fit in LDS and each gets the pipelined kernel (so the second round
-    // belongs to the pipelining switch).  Exact replay does not depend on
-    // how MCS are split into launches.
-    std::vector<uint64_t> td(todo, todo + n), tail(n, 0);
-    size_t n_tail = 0;
-    if (budget_s == 0.0 && !noalloc && e->pipelining) {
+    // well past every chain whose lattice is in LDS (an HBM lattice costs a
+    // single-wave chain ~1/kHbmSpeed per attempt).  They run on the
+    // pipelined kernel instead, whose second wave decodes and evaluates
+    // ahead and hides the HBM latency (in a full batch ~180 ns per attempt
+    // against ~280 single-wave): their extra waves take SIMD slots from the
+    // batch's short chains, which start when the first chains end and
+    // still finish long before the longest ones.  At most kStragglerShare
+    // of the batch's chains is piped this way, and only while the kernels
+    // keep a class stream each.
+    size_t n_stragglers = 0;
+    if (budget_s == 0.0 && e->pipelining) {
         auto work = [&](size_t i) { return (double)todo[i] * (double)models[i]->n_sites; };
         double lds_max = 0.0;
         size_t waves = 0;
+        std::set<int> keys;
         for (size_t i = 0; i < n; ++i)
             if (todo[i] && models[i]->dev->wave) {
                 ++waves;
                 if (!in_hbm[i] || piped[i]) lds_max = std::max(lds_max, work(i));
+                keys.insert((piped[i] ? 2000 : 0) + (in_hbm[i] ? 1000 : 0) + models[i]->D * 100 +
+                            models[i]->n_energy);
             }
         std::vector<size_t> cand;
         for (size_t i = 0; i < n; ++i)
-            if (todo[i] && models[i]->dev->wave && in_hbm[i] && !piped[i] && !models[i]->dev->gsig &&
-                work(i) > kTailMargin * kHbmSpeed * lds_max)
+            if (todo[i] && models[i]->dev->wave && in_hbm[i] && !piped[i] &&
+                work(i) > kStragglerMargin * kHbmSpeed * lds_max &&
+                lds_need(i, true) + pipe_extra_bytes(models[i]->D) + kLdsStaticReserve <=
+                    e->max_lds) {
                 cand.push_back(i);
-        if (!cand.empty() && (double)cand.size() <= kTailMaxShare * (double)waves)
+                keys.insert(3000 + models[i]->D * 100 + models[i]->n_energy);
+            }
+        if (!cand.empty() && (double)cand.size() <= kStragglerShare * (double)waves &&
+            keys.size() <= (size_t)Engine::kClassStreams)
             for (size_t i : cand) {
-                const uint64_t first = (uint64_t)((double)todo[i] * kHbmSpeed * lds_max / work(i));
-                td[i] = first;
-                tail[i] = todo[i] - first;
-                ++n_tail;
+                piped[i] = 1;
+                ++n_stragglers;
             }
     }
-    todo = td.data();
     // partition: global-path replicas, and wave-path replicas by kernel
     // instantiation (the key of launch_lds_engine)
     std::vector<uint32_t> order, global_idx;
@@ -3622,6 +3629,10 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
             }
         }
     }
+    // pipelined launches are enqueued first: their workgroups take two
+    // waves each and are dispatched before the others fill the SIMDs
+    std::stable_partition(launches.begin(), launches.end(),
+                          [](const Launch& l) { return l.D % 4000 >= 2000; });
     const uint32_t global_first = (uint32_t)order.size();
     order.insert(order.end(), global_idx.begin(), global_idx.end());
     HIP_OK(hipMemcpyAsync(reps_dev, reps.data(), n * sizeof(Rep), hipMemcpyHostToDevice,
@@ -3701,17 +3712,7 @@ static void run_exact(Engine* e, Model* const* models, size_t n, const uint64_t*
         }
         add_counters(counters, reps[i]);
     }
-    if (n_tail) {
-        if (counters) counters->tail_rounds += 1;
-        std::vector<Model*> ms;
-        std::vector<uint64_t> rest;
-        for (size_t i = 0; i < n; ++i)
-            if (tail[i]) {
-                ms.push_back(models[i]);
-                rest.push_back(tail[i]);
-            }
-        run_exact(e, ms.data(), ms.size(), rest.data(), placement, counters);
-    }
+    if (counters) counters->piped_stragglers += n_stragglers;
 }
 
 static void run_sublattice(Engine* e, Model* const* models, size_t n, const uint64_t* todo,
@@ -3907,7 +3908,7 @@ static void add_call_counters(cpm_counters* to, const cpm_counters& cc) {
     to->kernel_ms += cc.kernel_ms;
     to->replay_launches += cc.replay_launches;
     to->replay_launch_ms += cc.replay_launch_ms;
-    to->tail_rounds += cc.tail_rounds;
+    to->piped_stragglers += cc.piped_stragglers;
 }
 
 void run_mcs_each(Engine* e, Model* const* models, size_t n, const uint64_t* todo, int mode,

@@ -111,9 +111,10 @@ typedef struct {
        stream it runs on */
     uint64_t replay_launches;
     double replay_launch_ms;
-    /* exact-replay batches whose longest chains (lattices left in HBM by
-       the LDS budget) finished in a second round on the emptied device */
-    uint64_t tail_rounds;
+    /* exact-replay replicas whose long chains, with the lattice left in
+       HBM by the LDS budget, ran on the pipelined kernel (summed over
+       launches) */
+    uint64_t piped_stragglers;
 } cpm_counters;
 
 typedef struct cpm_engine cpm_engine;
