@@ -457,8 +457,9 @@ cpm_status cpm_engine_sigma_placement(cpm_engine* engine, int32_t* order, double
 /* Exact-replay pipelining (no counterpart in the reference): by default
    the replicas with the longest chains run on a two-wave kernel (one wave
    decodes and evaluates ahead while the other walks the chain) while the
-   batch leaves SIMDs free; on = 0 runs every replica on one wave.
-   Results do not depend on it. */
+   batch leaves SIMDs free, and so do the long chains whose lattices the
+   LDS budget leaves in HBM (counters.piped_stragglers); on = 0 runs every
+   replica on one wave.  Results do not depend on it. */
 cpm_status cpm_engine_set_pipelining(cpm_engine* engine, int32_t on);
 
 /* Where batches initialise (no counterpart in the reference; the result

@@ -3,7 +3,7 @@
  * ONLY (tests, smoke check, CPU-baseline leg of the benchmark).  See
  * cpm_oracle.h for what each function restates and what pins it.
  *
- * Build: make -C oracle   (gcc -O2 -ffp-contract=off, no fast-math: the
+ * Build: make -C oracle   (gcc -O3 -ffp-contract=off, no fast-math: the
  * reference's FP64 energy arithmetic is never contracted or reassociated).
  */
 #include "cpm_oracle.h"
