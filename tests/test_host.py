@@ -1,10 +1,12 @@
 """Host-side behaviour of the product library (CPU only): the C ABI loads
-and exports what include/cpm_engine.h declares; configuration errors,
+and exports what include/cpm_engine.h declares, and the ctypes mirrors
+of its structs have the header's layout; configuration errors,
 initial placement, theta and run-record hashes match the reference /
 oracle.  Nothing here launches a kernel."""
 import ctypes as C
 import os
 import re
+import subprocess
 import warnings
 
 import numpy as np
@@ -26,6 +28,40 @@ def test_library_exports_every_declared_symbol():
     missing = [s for s in sorted(declared) if not hasattr(lib, s)]
     assert not missing, missing
     assert len(declared) >= 20
+
+
+_MIRRORS = {"cpm_config": N.CpmConfig, "cpm_counters": N.CpmCounters,
+            "cpm_model_info": N.CpmModelInfo, "cpm_model_state": N.CpmModelState,
+            "cpm_result_info": N.CpmResultInfo, "cpm_sample_view": N.CpmSampleView,
+            "cpm_samples_shape": N.CpmSamplesShape, "cpm_samples_buffers": N.CpmSamplesBuffers,
+            "cpm_server_options": N.CpmServerOptions, "cpm_request_info": N.CpmRequestInfo,
+            "cpm_server_stats": N.CpmServerStats}
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    """Every struct of include/cpm_engine.h has a ctypes mirror with the same
+    fields at the same offsets and the same size, as the C compiler lays the
+    header out (a mirror that drifts corrupts memory silently)."""
+    header = open(os.path.join(ROOT, "include", "cpm_engine.h")).read()
+    declared = set(re.findall(r"typedef struct \{[^}]*\}\s*(cpm_[a-z_]+)\s*;", header))
+    assert declared == set(_MIRRORS), declared ^ set(_MIRRORS)
+    lines = ['#include <stdio.h>', '#include "cpm_engine.h"', "int main(void) {"]
+    for name, mirror in _MIRRORS.items():
+        lines.append(f'printf("{name} size %zu\\n", sizeof({name}));')
+        for field, _ in mirror._fields_:
+            lines.append(f'printf("{name} {field} %zu\\n", offsetof({name}, {field}));')
+    lines += ["return 0;", "}"]
+    src, exe = tmp_path / "layout.c", tmp_path / "layout"
+    src.write_text("\n".join(lines) + "\n")
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True)
+    got = dict(l.rsplit(" ", 1) for l in
+               subprocess.run([str(exe)], check=True, capture_output=True,
+                              text=True).stdout.splitlines())
+    for name, mirror in _MIRRORS.items():
+        assert int(got[f"{name} size"]) == C.sizeof(mirror), name
+        for field, _ in mirror._fields_:
+            assert int(got[f"{name} {field}"]) == getattr(mirror, field).offset, (name, field)
 
 
 def _same_init(sim, cfg):
